@@ -1,0 +1,176 @@
+/*
+ * dqmc_gpu.h -- C ABI of the MI355X DQMC Green's-function engine.
+ *
+ * The engine advances many independent replicas of the half-filled Hubbard
+ * model (forward BSS sweep, particle-hole symmetric path: only G_up is
+ * propagated, G_dn = I - P G_up^T P) on one AMD Instinct MI355X (gfx950).
+ * It replaces, for a batch of replicas, the block of dqmc_run_replica
+ * (src/replica_run.c:124-426) that calls dqmc_init_modes, dqmc_sweep,
+ * measure_sample and measure_szz_sperp_sample.  Host code keeps the input
+ * parser, seeds, field initialisation order, jackknife and writers.
+ *
+ * Conventions (kept from the reference):
+ *   - every function returns int 0 = ok, nonzero = failure
+ *     (src/linalg.h:48-53, src/dqmc.h:79-81); dqmc_gpu_last_error() gives
+ *     a message for the calling thread;
+ *   - matrices are column-major n x n doubles, A[i + j*n] (src/linalg.h:4);
+ *   - caller-owned host memory is copied, device memory is owned by the
+ *     context;
+ *   - per-replica status is sticky: after a numerical failure the replica's
+ *     sweeps are no-ops (src/dqmc.c:611-614) and the failure reason uses
+ *     the LINALG_FAILURE_* numbering (src/linalg.h:48-53) extended below;
+ *   - a context is not thread-safe: one host thread per device.
+ */
+#ifndef DQMC_GPU_H
+#define DQMC_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Precision modes. */
+#define DQMC_GPU_FP64 0  /* every stage in FP64 */
+#define DQMC_GPU_MIXED 1 /* wrap, delayed updates and B-chains in FP32 on
+                            MFMA; QR, rebuild, sign and measurement FP64 */
+
+/* Failure reasons: 0..3 as LINALG_FAILURE_* (src/linalg.h:48-53). */
+#define DQMC_GPU_FAIL_NONE 0
+#define DQMC_GPU_FAIL_NONFINITE 16 /* non-finite matrix at a stage */
+#define DQMC_GPU_FAIL_ZERO_D 17    /* zero diagonal in a QR (D_i == 0) */
+#define DQMC_GPU_FAIL_SIGN 18      /* det sign evaluated to 0 */
+
+/* Steps for dqmc_gpu_step (per-kernel golden tests). */
+#define DQMC_GPU_STEP_PROPOSE_SLICE 0 /* Metropolis over all sites of slice arg */
+#define DQMC_GPU_STEP_WRAP 1          /* G <- B_arg G B_arg^-1 */
+#define DQMC_GPU_STEP_REBUILD 2       /* G(tau=arg) from scratch (L lmuls) */
+
+typedef struct dqmc_gpu_ctx dqmc_gpu_ctx;
+
+typedef struct {
+    int precision;   /* DQMC_GPU_FP64 or DQMC_GPU_MIXED */
+    int use_graph;   /* capture one sweep per model as a hipGraph */
+    int drift;       /* record max|G_wrap - G_rebuild| at each rebuild */
+    int autotune;    /* choose delay depth per model when delay_k <= 0 */
+} dqmc_gpu_opts;
+
+/* Create a context on HIP device `device`.  `o` may be NULL (FP64, graphs
+   on, drift off). */
+int dqmc_gpu_create(dqmc_gpu_ctx **out, int device, const dqmc_gpu_opts *o);
+
+/* Register one model (lattice, U, dtau, L = beta/dtau slices).  expK and
+   expKinv are exp(-dtau K) and exp(+dtau K) with K = t - mu I, as
+   model_init computes them (src/model.c:8-31); thop is the hopping matrix
+   (src/lattice.c) used by the energy, bipart the sublattice signs,
+   lambda = acosh(exp(dtau U / 2)) (src/field.c:9).  stab is the
+   stabilisation interval (src/green.c:168-184), delay_k the delayed-update
+   depth (the reference's GREEN_DELAY_CAPACITY, src/green.c:10); delay_k <= 0
+   lets the engine choose.  Lx, Ly give the geometry for the structure
+   factors (Ly = 1 for a chain).  Replaces model_init + green_alloc +
+   green_stack_alloc (src/model.c:8-31, src/green.c:54-86, 168-184). */
+int dqmc_gpu_add_model(dqmc_gpu_ctx *ctx, int n, int L, int stab,
+                       int delay_k, const double *expK, const double *expKinv,
+                       const double *thop, const int *bipart, double U,
+                       double dtau, double mu, double lambda, int Lx, int Ly,
+                       int *model_id);
+
+/* Add nrep replicas of a model.  With fields == NULL each replica's field
+   is drawn from its seed exactly as rng_seed + field_init do
+   (src/rng.c:16-37, src/field.c:20-22: L*n draws, s = draw < 0.5 ? -1 : 1);
+   otherwise fields holds nrep * L * n values of +-1 and the RNG stream
+   starts right after the L*n field draws of each seed.  Replica ids are
+   consecutive from *first_replica_id. */
+int dqmc_gpu_add_replicas(dqmc_gpu_ctx *ctx, int model_id, int nrep,
+                          const unsigned long long *seeds,
+                          const signed char *fields, int *first_replica_id);
+
+/* Momenta and bins for the measurement of a model: phase_cos[disp + q*n]
+   as structure_factor_plan_init builds it (src/structure_factor.c:185-203);
+   nq = 0 disables Szz/Sperp (scalars are always measured). */
+int dqmc_gpu_set_measure(dqmc_gpu_ctx *ctx, int model_id, int nq,
+                         const double *phase_cos, int nbin);
+
+/* Initial stabilised Green's function at tau = 0 for every replica
+   (dqmc_init_modes -> green_from_scratch(Gu, 0), src/dqmc.c:418-483). */
+int dqmc_gpu_init(dqmc_gpu_ctx *ctx);
+
+/* nsweep forward sweeps of every replica (dqmc_sweep, src/dqmc.c:952-963).
+   If bin >= 0, each sweep is followed by the tau = 0 measurement
+   (measure_sample + measure_szz_sperp_sample) added with the replica's sign
+   into bin `bin`, plus the sweep's acceptance deltas
+   (src/replica_run.c:288-451). */
+int dqmc_gpu_sweep(dqmc_gpu_ctx *ctx, int nsweep, int bin);
+
+/* nwarm sweeps, then nbin * nmeas_per_bin measured sweeps
+   (src/replica_run.c:184-186, 284-451). */
+int dqmc_gpu_run(dqmc_gpu_ctx *ctx, int nwarm, int nmeas_per_bin, int nbin);
+
+/* Bins of one replica.  bins8[nbin*8] packs each ReplicaBin as
+   replica_mpi_pack_bins does (src/replica_mpi.c:102-126): sum sign*E_hub,
+   sum sign*E_gc, sum sign*E_ph, sum sign*N, sum sign*D, sum sign,
+   accepted, attempts; counts[nbin]; szz/sperp [nbin*nq] hold
+   sum sign*S(q) (src/replica.c:240-269).  Any output pointer may be NULL. */
+int dqmc_gpu_read_bins(dqmc_gpu_ctx *ctx, int replica, double *bins8,
+                       int *counts, double *szz, double *sperp);
+
+/* Per-replica status (0 ok), failure reason, accepted / attempted flips
+   (src/dqmc.h:79-86) and sign.  Arrays hold one entry per replica. */
+int dqmc_gpu_read_status(dqmc_gpu_ctx *ctx, int *status, int *failure_reason,
+                         unsigned long long *accepted,
+                         unsigned long long *attempts, double *sign);
+
+/* Stabilisation drift max|G_wrap - G_rebuild| per replica: maximum, mean
+   and sample count (dqmc_record_stab_drift, src/dqmc.c:64-98; here the
+   rebuilt G of the stack is the reference point). */
+int dqmc_gpu_read_drift(dqmc_gpu_ctx *ctx, double *max_inf, double *mean_inf,
+                        unsigned long long *samples);
+
+/* Parity and debug hooks. */
+int dqmc_gpu_read_green(dqmc_gpu_ctx *ctx, int replica, double *g);
+int dqmc_gpu_write_green(dqmc_gpu_ctx *ctx, int replica, const double *g);
+int dqmc_gpu_read_field(dqmc_gpu_ctx *ctx, int replica, signed char *s);
+int dqmc_gpu_write_field(dqmc_gpu_ctx *ctx, int replica, const signed char *s);
+int dqmc_gpu_read_rng(dqmc_gpu_ctx *ctx, int replica, unsigned long long *s4);
+/* One stage for every replica of every model (DQMC_GPU_STEP_*). */
+int dqmc_gpu_step(dqmc_gpu_ctx *ctx, int step, int arg);
+
+/* Wall time of the last dqmc_gpu_sweep / dqmc_gpu_run call measured with
+   HIP events on the engine's streams, milliseconds. */
+int dqmc_gpu_last_elapsed_ms(dqmc_gpu_ctx *ctx, double *ms);
+
+void dqmc_gpu_destroy(dqmc_gpu_ctx *ctx);
+
+/* ---- batched linear-algebra entry points on host arrays ----
+   Each takes `batch` independent problems laid out back to back
+   (stride n*n for matrices, n for vectors) and runs them on `device`.
+   They replace the reference's linalg.c calls one for one. */
+
+/* udv_lmul_work (src/linalg.c:515-642), in place. */
+int dqmc_gpu_udv_lmul(int device, int n, int batch, double *U, double *D,
+                      double *T, const double *B);
+/* udv_rmul (src/linalg.c:662-784), in place. */
+int dqmc_gpu_udv_rmul(int device, int n, int batch, double *U, double *D,
+                      double *T, const double *B);
+/* udv_combine (src/linalg.c:796-909). */
+int dqmc_gpu_udv_combine(int device, int n, int batch, const double *Ul,
+                         const double *Dl, const double *Tl, const double *Ur,
+                         const double *Dr, const double *Tr, double *Uo,
+                         double *Do, double *To);
+/* udv_inv_one_plus_work (src/linalg.c:911-1115); sign[batch]. */
+int dqmc_gpu_udv_inv_one_plus(int device, int n, int batch, const double *U,
+                              const double *D, const double *T, double *g,
+                              int *sign);
+/* measure_szz_sperp_sample + measure_sample on G_up (G_dn by the PH map),
+   (src/structure_factor.c:328-400, src/measure.c:6-39).  scal5[batch*5] =
+   {E, ekin, eint, ntot, doublon}; szz, sperp [batch*nq]. */
+int dqmc_gpu_measure(int device, int Lx, int Ly, int batch, const double *gu,
+                     const double *thop, const int *bipart, double U, int nq,
+                     const double *phase_cos, double *scal5, double *szz,
+                     double *sperp);
+
+const char *dqmc_gpu_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

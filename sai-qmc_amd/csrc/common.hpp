@@ -1,0 +1,110 @@
+// Shared device helpers for the gfx950 DQMC engine.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define DQMC_WAVE 64
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// 16x16x4 MFMA tile: lane l supplies A[l%16][l/16] and B[l/16][l%16] and
+// holds D[4*(l/16)+v][l%16], v = 0..3.
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef f64x4 acc_t;
+    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mfma<float> {
+    typedef f32x4 acc_t;
+    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Block-wide sum; `scratch` holds one slot per wave.  Every thread gets the
+// result.  Contains two barriers.
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *scratch)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+    v = wave_sum(v);
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    T s = 0;
+    for (int w = 0; w < nw; w++) s += scratch[w];
+    __syncthreads();
+    return s;
+}
+
+// xoshiro256** step returning the top 53 bits as a double in [0, 1):
+// the generator and conversion of src/rng.c:23-37 (Blackman & Vigna).
+__device__ __forceinline__ double xoshiro_double(uint64_t s[4])
+{
+    const uint64_t x = s[1] * 5;
+    const uint64_t out = ((x << 7) | (x >> 57)) * 9;
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = (s[3] << 45) | (s[3] >> 19);
+    return (double)(out >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// Optional per-replica vector used as a diagonal scale: value 1 when p is
+// null, 1/p[i] when inv is set.
+template <typename T> struct Vec {
+    const T *p;
+    long long stride;
+    int inv;
+    __device__ __forceinline__ T at(int b, int i) const
+    {
+        if (!p) return T(1);
+        const T v = p[(long long)b * stride + i];
+        return inv ? T(1) / v : v;
+    }
+};
+
+template <typename T> static inline Vec<T> vnone() { return Vec<T>{nullptr, 0, 0}; }
+template <typename T> static inline Vec<T> vec(const T *p, long long stride, int inv = 0)
+{
+    return Vec<T>{p, stride, inv};
+}
+
+// Map a linear workgroup id so that consecutive logical ids stay on one XCD
+// (the dispatcher deals workgroups round-robin over the 8 XCDs).  Logical
+// ids are grouped per batch entry by the caller, so one replica's tiles
+// share an L2.
+__device__ __forceinline__ int xcd_remap(int lin, int total)
+{
+    if ((total & 7) != 0) return lin;
+    return (lin & 7) * (total >> 3) + (lin >> 3);
+}

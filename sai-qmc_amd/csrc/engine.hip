@@ -1,0 +1,1518 @@
+// MI355X DQMC Green's-function engine: device buffers, the forward sweep
+// schedule and the C ABI declared in include/dqmc_gpu.h.
+//
+// One "model" groups the replicas that share (lattice, U, dtau, L, stab).
+// Each model owns a HIP stream; every stage is one launch over all of the
+// model's replicas (batched GEMMs over the whole chip, one workgroup per
+// replica for the sequential stages).  A sweep's launch sequence is fixed,
+// so it is captured once as a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dqmc_gpu.h"
+#include "common.hpp"
+#include "gemm.hpp"
+#include "measure.hpp"
+#include "metro.hpp"
+#include "qr.hpp"
+
+// ------------------------------------------------------------------ errors
+
+static thread_local std::string g_err;
+
+static void set_err(const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+#define HIPCHK(x)                                                                   \
+    do {                                                                            \
+        hipError_t e_ = (x);                                                        \
+        if (e_ != hipSuccess) {                                                     \
+            set_err("%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__,   \
+                    __LINE__);                                                      \
+            return 1;                                                               \
+        }                                                                           \
+    } while (0)
+
+// ------------------------------------------------------------ small kernels
+
+__global__ void k_udv_identity(double *U, long long sU, double *D, long long sD, double *T,
+                               long long sT, int *q, long long sq, int n, int batch)
+{
+    const int b = blockIdx.y;
+    if (b >= batch) return;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < (long long)n * n;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(idx % n), j = (int)(idx / n);
+        const double v = (i == j) ? 1.0 : 0.0;
+        U[b * sU + idx] = v;
+        T[b * sT + idx] = v;
+        if (j == 0) D[b * sD + i] = 1.0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) q[b * sq] = 1;
+}
+
+__global__ void k_copy_d(const double *src, long long sS, double *dst, long long sD, long long len,
+                         int batch)
+{
+    const int b = blockIdx.y;
+    if (b >= batch) return;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < len;
+         idx += (long long)gridDim.x * blockDim.x)
+        dst[b * sD + idx] = src[b * sS + idx];
+}
+
+__global__ void k_d2f(const double *src, long long sS, float *dst, long long sD, long long len, int batch)
+{
+    const int b = blockIdx.y;
+    if (b >= batch) return;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < len;
+         idx += (long long)gridDim.x * blockDim.x)
+        dst[b * sD + idx] = (float)src[b * sS + idx];
+}
+
+__global__ void k_f2d(const float *src, long long sS, double *dst, long long sD, long long len, int batch)
+{
+    const int b = blockIdx.y;
+    if (b >= batch) return;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < len;
+         idx += (long long)gridDim.x * blockDim.x)
+        dst[b * sD + idx] = (double)src[b * sS + idx];
+}
+
+// out = expK diag(d) (the single-slice B matrix, src/green.c:126-137), FP32
+__global__ void k_scale_cols_f(const float *expK, const float *d, long long sdv, float *out,
+                               long long sO, int n, int batch)
+{
+    const int b = blockIdx.y;
+    if (b >= batch) return;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < (long long)n * n;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int j = (int)(idx / n);
+        out[b * sO + idx] = expK[idx] * d[b * sdv + j];
+    }
+}
+
+// Db/Ds split of udv_inv_one_plus_work (src/linalg.c:963-984): dbinv = 1/Db,
+// ds = Ds, and the sign of prod_{|D_i|>1} D_i.
+__global__ void k_split_db(const double *D, long long sD, double *dbinv, double *ds, int *bigsign,
+                           int n, int batch, const int *status)
+{
+    const int b = blockIdx.x;
+    if (b >= batch || status[b]) return;
+    __shared__ int neg;
+    if (threadIdx.x == 0) neg = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double d = D[b * sD + i];
+        if (fabs(d) > 1.0) {
+            dbinv[(long long)b * n + i] = 1.0 / d;
+            ds[(long long)b * n + i] = 1.0;
+            if (d < 0.0) atomicXor(&neg, 1);
+        } else {
+            dbinv[(long long)b * n + i] = 1.0;
+            ds[(long long)b * n + i] = d;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) bigsign[b] = neg ? -1 : 1;
+}
+
+// det sign of 1 + U D T = sgn(prod_big D) sgn det U sgn det M, with
+// det M = det Q_M prod (D_M)_i (src/linalg.c:1039-1083).
+__global__ void k_sign_finish(const int *bigsign, const int *uq, long long suq, const int *mq,
+                              const double *DM, int n, int *det_sign, int *status, int *reason,
+                              int batch)
+{
+    const int b = blockIdx.x;
+    if (b >= batch || status[b]) return;
+    __shared__ int neg, bad;
+    if (threadIdx.x == 0) { neg = 0; bad = 0; }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double d = DM[(long long)b * n + i];
+        if (d < 0.0) atomicXor(&neg, 1);
+        if (!(d != 0.0) || !isfinite(d)) bad = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = bigsign[b] * uq[b * suq] * mq[b] * (neg ? -1 : 1);
+        if (bad) s = 0;
+        det_sign[b] = s;
+        if (s == 0) {
+            status[b] = 1;
+            reason[b] = DQMC_GPU_FAIL_SIGN;
+        }
+    }
+}
+
+__global__ void k_mul_q(const int *a, long long sa, const int *b_, long long sb, int *out, long long so,
+                        int batch, const int *status)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch || status[b]) return;
+    out[b * so] = a[b * sa] * b_[b * sb];
+}
+
+// Replace G by the rebuilt Gnew; optionally record max|G - Gnew| as the
+// stabilisation drift (dqmc_record_stab_drift, src/dqmc.c:64-98).  In mixed
+// precision the wrapped G lives in Gf and the FP32 copy is refreshed.
+__global__ void k_drift_copy(double *G, float *Gf, const double *Gnew, long long sG, int n, int batch,
+                             int drift, int from_f32, double *dmax, double *dsum,
+                             unsigned long long *dcnt, int *status, int *reason, double *sign)
+{
+    const int b = blockIdx.x;
+    if (b >= batch || status[b]) return;
+    // PH path: the running sign is reset to the exact +1 at every rebuild
+    // (src/dqmc.c:740-742, 799-801)
+    if (threadIdx.x == 0 && sign) sign[b] = 1.0;
+    __shared__ double red[8];
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    double m = 0.0;
+    for (long long idx = threadIdx.x; idx < (long long)n * n; idx += blockDim.x) {
+        const double v = Gnew[b * sG + idx];
+        if (!isfinite(v)) bad = 1;
+        if (drift) {
+            const double old = from_f32 ? (double)Gf[b * sG + idx] : G[b * sG + idx];
+            m = fmax(m, fabs(old - v));
+        }
+        G[b * sG + idx] = v;
+        if (Gf) Gf[b * sG + idx] = (float)v;
+    }
+    if (drift) {
+        m = wave_max(m);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (bad) {
+            status[b] = 1;
+            reason[b] = DQMC_GPU_FAIL_NONFINITE;
+        } else if (drift) {
+            double mm = 0.0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); w++) mm = fmax(mm, red[w]);
+            dmax[b] = fmax(dmax[b], mm);
+            dsum[b] += mm;
+            dcnt[b] += 1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- buffers
+
+template <typename T> static int dalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) return 0;
+    HIPCHK(hipMalloc((void **)p, count * sizeof(T)));
+    return 0;
+}
+
+struct Udv {
+    double *U, *D, *T;
+    int *q;
+    long long sU, sD, sq; // per-replica strides
+};
+
+struct Work {
+    int n = 0, nb = 0;
+    hipStream_t s = nullptr;
+    double *W1 = nullptr, *W2 = nullptr, *W3 = nullptr, *W4 = nullptr;
+    double *V1 = nullptr, *V2 = nullptr, *V3 = nullptr, *Tws = nullptr;
+    int *I1 = nullptr, *I2 = nullptr, *status = nullptr, *reason = nullptr, *det_sign = nullptr;
+    float *F1 = nullptr, *F2 = nullptr;
+    std::vector<void *> owned;
+
+    int alloc(int n_, int nb_, int mixed)
+    {
+        n = n_;
+        nb = nb_;
+        const size_t nn = (size_t)n * n * nb;
+        if (dalloc(&W1, nn) || dalloc(&W2, nn) || dalloc(&W3, nn) || dalloc(&W4, nn)) return 1;
+        if (dalloc(&V1, (size_t)n * nb) || dalloc(&V2, (size_t)n * nb) || dalloc(&V3, (size_t)n * nb))
+            return 1;
+        const int npan = (n + QR_NB - 1) / QR_NB;
+        if (dalloc(&Tws, (size_t)npan * QR_NB * QR_NB * nb)) return 1;
+        if (dalloc(&I1, nb) || dalloc(&I2, nb) || dalloc(&status, nb) || dalloc(&reason, nb) ||
+            dalloc(&det_sign, nb))
+            return 1;
+        if (mixed && (dalloc(&F1, nn) || dalloc(&F2, nn))) return 1;
+        void *ps[] = {W1, W2, W3, W4, V1, V2, V3, Tws, I1, I2, status, reason, det_sign, F1, F2};
+        for (void *p : ps)
+            if (p) owned.push_back(p);
+        HIPCHK(hipMemset(status, 0, sizeof(int) * nb));
+        HIPCHK(hipMemset(reason, 0, sizeof(int) * nb));
+        HIPCHK(hipMemset(det_sign, 0, sizeof(int) * nb));
+        return 0;
+    }
+    void release()
+    {
+        for (void *p : owned) (void)hipFree(p);
+        owned.clear();
+    }
+};
+
+static dim3 grid2(long long len, int batch)
+{
+    long long bx = (len + 255) / 256;
+    if (bx > 64) bx = 64;
+    if (bx < 1) bx = 1;
+    return dim3((unsigned)bx, (unsigned)batch);
+}
+
+// ------------------------------------------------------- linear algebra ops
+
+static int launch_check(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_err("launch of %s failed: %s", what, hipGetErrorString(e));
+        return 1;
+    }
+    return 0;
+}
+
+// QR of the n x n matrices A (stride n*n, destroyed): Q, Rn = D^-1 R, D and
+// sgn det Q with the given per-replica strides.
+static int qr(Work &w, double *A, double *Q, long long sQ, double *Rn, long long sR, double *D,
+              long long sD, int *qsign, long long sq)
+{
+    QrArgs a;
+    a.n = w.n; a.batch = w.nb;
+    a.A = A; a.sA = (long long)w.n * w.n;
+    a.Q = Q; a.sQ = sQ;
+    a.Rn = Rn; a.sR = sR;
+    a.D = D; a.sD = sD;
+    a.Tws = w.Tws; a.sT = (long long)((w.n + QR_NB - 1) / QR_NB) * QR_NB * QR_NB;
+    a.qsign = qsign; a.sq = sq;
+    a.status = w.status; a.reason = w.reason;
+    hipLaunchKernelGGL(k_qr_udv, dim3(w.nb), dim3(QR_THREADS), qr_lds_bytes(w.n), w.s, a);
+    return launch_check("k_qr_udv");
+}
+
+static int trtri(Work &w, const double *T, long long sT, double *X)
+{
+    TrtriArgs a;
+    a.n = w.n; a.batch = w.nb;
+    a.T = T; a.sT = sT;
+    a.X = X; a.sX = (long long)w.n * w.n;
+    a.status = w.status;
+    hipLaunchKernelGGL(k_trtri_unit, dim3(w.nb), dim3(256), trtri_lds_bytes(w.n), w.s, a);
+    return launch_check("k_trtri_unit");
+}
+
+template <typename T> static int gemm(Work &w, GemmArgs<T> g)
+{
+    g.status = w.status;
+    gemm_launch(g, w.s);
+    return launch_check("k_gemm");
+}
+
+static int copy_nn(Work &w, const double *src, long long sS, double *dst, long long sD, long long len)
+{
+    hipLaunchKernelGGL(k_copy_d, grid2(len, w.nb), dim3(256), 0, w.s, src, sS, dst, sD, len, w.nb);
+    return launch_check("k_copy_d");
+}
+
+// udv_combine (src/linalg.c:796-909): C = Dl (Tl Ur) Dr; QR(C) = Q R;
+// out = (Ul Q, diag R, (D^-1 R) Tr).
+static int udv_combine(Work &w, const Udv &l, const Udv &r, const Udv &o)
+{
+    const int n = w.n;
+    const long long nn = (long long)n * n;
+    GemmArgs<double> g = gemm_args<double>(n, n, n, w.nb, l.T, n, l.sU, 0, r.U, n, r.sU, 0, w.W1, n, nn);
+    g.ar = vec(l.D, l.sD);
+    g.bc = vec(r.D, r.sD);
+    if (gemm(w, g)) return 1;
+    if (qr(w, w.W1, w.W2, nn, w.W3, nn, o.D, o.sD, w.I1, 1)) return 1;
+    g = gemm_args<double>(n, n, n, w.nb, l.U, n, l.sU, 0, w.W2, n, nn, 0, o.U, n, o.sU);
+    if (gemm(w, g)) return 1;
+    g = gemm_args<double>(n, n, n, w.nb, w.W3, n, nn, 0, r.T, n, r.sU, 0, o.T, n, o.sU);
+    if (gemm(w, g)) return 1;
+    hipLaunchKernelGGL(k_mul_q, dim3((w.nb + 255) / 256), dim3(256), 0, w.s, l.q, l.sq, w.I1, 1LL, o.q,
+                       o.sq, w.nb, w.status);
+    return launch_check("k_mul_q");
+}
+
+// udv_inv_one_plus_work (src/linalg.c:911-1115) with M inverted through its
+// own QR: M = Q_M D_M Rn_M, so
+//   G = T^-1 M^-1 Db^-1 U^T = T^-1 Rn_M^-1 D_M^-1 Q_M^T Db^-1 U^T
+// and sgn det M = sgn det Q_M * prod sgn (D_M)_i.
+static int udv_inv_one_plus(Work &w, const Udv &u, double *Gout, long long sG)
+{
+    const int n = w.n;
+    const long long nn = (long long)n * n;
+    hipLaunchKernelGGL(k_split_db, dim3(w.nb), dim3(256), 0, w.s, u.D, u.sD, w.V1, w.V2, w.I1, n, w.nb,
+                       w.status);
+    if (launch_check("k_split_db")) return 1;
+    if (trtri(w, u.T, u.sU, w.W1)) return 1; // W1 = T^-1
+    GemmArgs<double> g = gemm_args<double>(n, n, n, w.nb, u.U, n, u.sU, 1, w.W1, n, nn, 0, w.W2, n, nn);
+    g.cr = vec<double>(w.V1, n);
+    g.dg = vec<double>(w.V2, n);
+    if (gemm(w, g)) return 1; // W2 = M = Db^-1 U^T T^-1 + Ds
+    if (qr(w, w.W2, w.W3, nn, w.W4, nn, w.V3, n, w.I2, 1)) return 1; // W3 = Q_M, V3 = D_M, W4 = Rn_M
+    if (trtri(w, w.W4, nn, w.W2)) return 1;               // W2 = Rn_M^-1
+    g = gemm_args<double>(n, n, n, w.nb, w.W3, n, nn, 1, u.U, n, u.sU, 1, w.W4, n, nn);
+    g.br = vec<double>(w.V1, n);
+    g.cr = vec<double>(w.V3, n, 1);
+    if (gemm(w, g)) return 1; // W4 = D_M^-1 Q_M^T Db^-1 U^T
+    g = gemm_args<double>(n, n, n, w.nb, w.W2, n, nn, 0, w.W4, n, nn, 0, w.W3, n, nn);
+    if (gemm(w, g)) return 1; // W3 = Rn_M^-1 W4
+    g = gemm_args<double>(n, n, n, w.nb, w.W1, n, nn, 0, w.W3, n, nn, 0, Gout, n, sG);
+    if (gemm(w, g)) return 1; // G = T^-1 W3
+    hipLaunchKernelGGL(k_sign_finish, dim3(w.nb), dim3(256), 0, w.s, w.I1, u.q, u.sq, w.I2, w.V3, n,
+                       w.det_sign, w.status, w.reason, w.nb);
+    return launch_check("k_sign_finish");
+}
+
+// ------------------------------------------------------------------ model
+
+struct Model {
+    int n = 0, L = 0, stab = 0, M = 0, W = 0, Wf = 0, nrep = 0, first = 0, cap = 0;
+    int Lx = 1, Ly = 1, nq = 0, nbin = 0;
+    double U = 0, dtau = 0, mu = 0, lambda = 0;
+    double expv[2], expvinv[2], nu[2], nd[2];
+    std::vector<double> h_expK, h_expKinv, h_thop, h_phase;
+    std::vector<int> h_bipart;
+    std::vector<unsigned long long> seeds;
+    std::vector<signed char> h_fields;
+    std::vector<uint64_t> h_rng;
+
+    double *expK = nullptr, *expKinv = nullptr, *thop = nullptr, *phase = nullptr;
+    float *expKf = nullptr, *expKinvf = nullptr;
+    int *bipart = nullptr;
+    double *G = nullptr, *Gnew = nullptr;
+    float *Gf = nullptr;
+    signed char *field = nullptr;
+    double *dvec = nullptr, *dinv = nullptr;
+    float *dvecf = nullptr, *dinvf = nullptr;
+    uint64_t *rng = nullptr;
+    double *SU = nullptr, *SD = nullptr, *ST = nullptr;
+    int *Sq = nullptr;
+    double *LU_ = nullptr, *LD = nullptr, *LT = nullptr;
+    int *Lq = nullptr;
+    double *CU = nullptr, *CD = nullptr, *CT = nullptr;
+    int *Cq = nullptr;
+    unsigned long long *acc = nullptr, *att = nullptr, *acc_prev = nullptr, *att_prev = nullptr;
+    double *sign = nullptr;
+    double *dmax = nullptr, *dsum = nullptr;
+    unsigned long long *dcnt = nullptr;
+    double *bins = nullptr, *bszz = nullptr, *bsperp = nullptr;
+    int *counts = nullptr;
+    Work w;
+    bool allocated = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<void *> owned;
+
+    Udv left() const
+    {
+        return Udv{LU_, LD, LT, Lq, (long long)n * n, (long long)n, 1};
+    }
+    Udv comb() const
+    {
+        return Udv{CU, CD, CT, Cq, (long long)n * n, (long long)n, 1};
+    }
+    Udv stack(int j) const
+    {
+        const long long nn = (long long)n * n;
+        return Udv{SU + j * nn, SD + (long long)j * n, ST + j * nn, Sq + j, (M + 1) * nn,
+                   (long long)(M + 1) * n, (long long)(M + 1)};
+    }
+};
+
+struct dqmc_gpu_ctx {
+    int device = 0;
+    dqmc_gpu_opts opts;
+    std::vector<Model *> models;
+    int nrep_total = 0;
+    bool initialised = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+};
+
+static int model_for_replica(dqmc_gpu_ctx *ctx, int replica, Model **m, int *local)
+{
+    for (Model *md : ctx->models) {
+        if (replica >= md->first && replica < md->first + md->nrep) {
+            *m = md;
+            *local = replica - md->first;
+            return 0;
+        }
+    }
+    set_err("replica %d out of range", replica);
+    return 1;
+}
+
+// Largest window (delay depth) whose four W x n tiles fit the LDS budget.
+static int metro_window(int n, int want, size_t elem)
+{
+    const size_t budget = 150 * 1024;
+    int W = want > 0 ? want : 32;
+    if (W > n) W = n;
+    while (W > 1 && (size_t)4 * W * n * elem + 11 * (size_t)n + 256 > budget) W--;
+    return W;
+}
+
+static int model_alloc(dqmc_gpu_ctx *ctx, Model &m)
+{
+    const int n = m.n, R = m.nrep, L = m.L;
+    const size_t nn = (size_t)n * n;
+    const int mixed = ctx->opts.precision == DQMC_GPU_MIXED;
+    HIPCHK(hipStreamCreateWithFlags(&m.w.s, hipStreamNonBlocking));
+    if (m.w.alloc(n, R, mixed)) return 1;
+    auto own = [&](void *p) {
+        if (p) m.owned.push_back(p);
+    };
+    if (dalloc(&m.expK, nn) || dalloc(&m.expKinv, nn) || dalloc(&m.thop, nn) || dalloc(&m.bipart, n))
+        return 1;
+    own(m.expK); own(m.expKinv); own(m.thop); own(m.bipart);
+    if (dalloc(&m.G, nn * R) || dalloc(&m.Gnew, nn * R) || dalloc(&m.field, (size_t)L * n * R) ||
+        dalloc(&m.dvec, (size_t)L * n * R) || dalloc(&m.dinv, (size_t)L * n * R) ||
+        dalloc(&m.rng, (size_t)4 * R))
+        return 1;
+    own(m.G); own(m.Gnew); own(m.field); own(m.dvec); own(m.dinv); own(m.rng);
+    if (mixed) {
+        if (dalloc(&m.expKf, nn) || dalloc(&m.expKinvf, nn) || dalloc(&m.Gf, nn * R) ||
+            dalloc(&m.dvecf, (size_t)L * n * R) || dalloc(&m.dinvf, (size_t)L * n * R))
+            return 1;
+        own(m.expKf); own(m.expKinvf); own(m.Gf); own(m.dvecf); own(m.dinvf);
+    }
+    const size_t S = (size_t)(m.M + 1);
+    if (dalloc(&m.SU, nn * S * R) || dalloc(&m.SD, (size_t)n * S * R) || dalloc(&m.ST, nn * S * R) ||
+        dalloc(&m.Sq, S * R))
+        return 1;
+    own(m.SU); own(m.SD); own(m.ST); own(m.Sq);
+    if (dalloc(&m.LU_, nn * R) || dalloc(&m.LD, (size_t)n * R) || dalloc(&m.LT, nn * R) ||
+        dalloc(&m.Lq, R) || dalloc(&m.CU, nn * R) || dalloc(&m.CD, (size_t)n * R) ||
+        dalloc(&m.CT, nn * R) || dalloc(&m.Cq, R))
+        return 1;
+    own(m.LU_); own(m.LD); own(m.LT); own(m.Lq); own(m.CU); own(m.CD); own(m.CT); own(m.Cq);
+    if (dalloc(&m.acc, R) || dalloc(&m.att, R) || dalloc(&m.acc_prev, R) || dalloc(&m.att_prev, R) ||
+        dalloc(&m.sign, R) || dalloc(&m.dmax, R) || dalloc(&m.dsum, R) || dalloc(&m.dcnt, R))
+        return 1;
+    own(m.acc); own(m.att); own(m.acc_prev); own(m.att_prev); own(m.sign); own(m.dmax); own(m.dsum);
+    own(m.dcnt);
+    const int nbin = std::max(m.nbin, 1), nq = std::max(m.nq, 1);
+    if (dalloc(&m.bins, (size_t)8 * nbin * R) || dalloc(&m.counts, (size_t)nbin * R) ||
+        dalloc(&m.bszz, (size_t)nq * nbin * R) || dalloc(&m.bsperp, (size_t)nq * nbin * R))
+        return 1;
+    own(m.bins); own(m.counts); own(m.bszz); own(m.bsperp);
+    if (m.nq > 0) {
+        if (dalloc(&m.phase, (size_t)n * m.nq)) return 1;
+        own(m.phase);
+    }
+
+    // uploads
+    HIPCHK(hipMemcpy(m.expK, m.h_expK.data(), nn * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m.expKinv, m.h_expKinv.data(), nn * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m.thop, m.h_thop.data(), nn * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m.bipart, m.h_bipart.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m.field, m.h_fields.data(), (size_t)L * n * R, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m.rng, m.h_rng.data(), (size_t)4 * R * 8, hipMemcpyHostToDevice));
+    {
+        std::vector<double> dv((size_t)L * n * R), di((size_t)L * n * R);
+        for (size_t k = 0; k < dv.size(); k++) {
+            const int sp = m.h_fields[k] > 0;
+            dv[k] = m.expv[sp];
+            di[k] = m.expvinv[sp];
+        }
+        HIPCHK(hipMemcpy(m.dvec, dv.data(), dv.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(m.dinv, di.data(), di.size() * 8, hipMemcpyHostToDevice));
+        if (mixed) {
+            std::vector<float> fv(dv.begin(), dv.end()), fi(di.begin(), di.end());
+            HIPCHK(hipMemcpy(m.dvecf, fv.data(), fv.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(m.dinvf, fi.data(), fi.size() * 4, hipMemcpyHostToDevice));
+            std::vector<float> k1(m.h_expK.begin(), m.h_expK.end()), k2(m.h_expKinv.begin(), m.h_expKinv.end());
+            HIPCHK(hipMemcpy(m.expKf, k1.data(), nn * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(m.expKinvf, k2.data(), nn * 4, hipMemcpyHostToDevice));
+        }
+    }
+    std::vector<double> ones(R, 1.0);
+    HIPCHK(hipMemcpy(m.sign, ones.data(), R * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(m.acc, 0, R * 8));
+    HIPCHK(hipMemset(m.att, 0, R * 8));
+    HIPCHK(hipMemset(m.acc_prev, 0, R * 8));
+    HIPCHK(hipMemset(m.att_prev, 0, R * 8));
+    HIPCHK(hipMemset(m.dmax, 0, R * 8));
+    HIPCHK(hipMemset(m.dsum, 0, R * 8));
+    HIPCHK(hipMemset(m.dcnt, 0, R * 8));
+    HIPCHK(hipMemset(m.bins, 0, (size_t)8 * nbin * R * 8));
+    HIPCHK(hipMemset(m.counts, 0, (size_t)nbin * R * sizeof(int)));
+    HIPCHK(hipMemset(m.bszz, 0, (size_t)nq * nbin * R * 8));
+    HIPCHK(hipMemset(m.bsperp, 0, (size_t)nq * nbin * R * 8));
+    if (m.nq > 0) HIPCHK(hipMemcpy(m.phase, m.h_phase.data(), (size_t)n * m.nq * 8, hipMemcpyHostToDevice));
+    // the last stack entry is the identity and never changes (src/green.c:205)
+    const Udv sm = m.stack(m.M);
+    hipLaunchKernelGGL(k_udv_identity, grid2(nn, R), dim3(256), 0, m.w.s, sm.U, sm.sU, sm.D, sm.sD, sm.T,
+                       sm.sU, sm.q, sm.sq, n, R);
+    if (launch_check("k_udv_identity")) return 1;
+    HIPCHK(hipStreamSynchronize(m.w.s));
+    m.allocated = true;
+    return 0;
+}
+
+// ----------------------------------------------------------------- stages
+
+static bool is_mixed(const dqmc_gpu_ctx *ctx) { return ctx->opts.precision == DQMC_GPU_MIXED; }
+
+// Metropolis over slice l (src/dqmc.c:656-679)
+static int stage_metro(dqmc_gpu_ctx *ctx, Model &m, int l)
+{
+    const int n = m.n;
+    const long long nn = (long long)n * n, sV = (long long)m.L * n;
+    if (is_mixed(ctx)) {
+        MetroArgs<float> a;
+        a.n = n; a.L = m.L; a.W = m.Wf; a.l = l; a.batch = m.nrep;
+        a.G = m.Gf; a.sG = nn; a.field = m.field; a.sF = sV;
+        a.dvec = m.dvecf; a.dinv = m.dinvf; a.sV = sV;
+        a.e[0] = (float)m.expv[0]; a.e[1] = (float)m.expv[1];
+        a.einv[0] = (float)m.expvinv[0]; a.einv[1] = (float)m.expvinv[1];
+        a.dvec64 = m.dvec; a.dinv64 = m.dinv;
+        a.e64[0] = m.expv[0]; a.e64[1] = m.expv[1];
+        a.einv64[0] = m.expvinv[0]; a.einv64[1] = m.expvinv[1];
+        a.rng = m.rng;
+        a.nu[0] = m.nu[0]; a.nu[1] = m.nu[1]; a.nd[0] = m.nd[0]; a.nd[1] = m.nd[1];
+        a.acc = m.acc; a.att = m.att; a.sign = m.sign; a.status = m.w.status;
+        hipLaunchKernelGGL(k_metro_slice<float>, dim3(m.nrep), dim3(256), metro_lds_bytes<float>(n, m.Wf),
+                           m.w.s, a);
+    } else {
+        MetroArgs<double> a;
+        a.n = n; a.L = m.L; a.W = m.W; a.l = l; a.batch = m.nrep;
+        a.G = m.G; a.sG = nn; a.field = m.field; a.sF = sV;
+        a.dvec = m.dvec; a.dinv = m.dinv; a.sV = sV;
+        a.e[0] = m.expv[0]; a.e[1] = m.expv[1];
+        a.einv[0] = m.expvinv[0]; a.einv[1] = m.expvinv[1];
+        a.dvec64 = nullptr; a.dinv64 = nullptr;
+        a.rng = m.rng;
+        a.nu[0] = m.nu[0]; a.nu[1] = m.nu[1]; a.nd[0] = m.nd[0]; a.nd[1] = m.nd[1];
+        a.acc = m.acc; a.att = m.att; a.sign = m.sign; a.status = m.w.status;
+        hipLaunchKernelGGL(k_metro_slice<double>, dim3(m.nrep), dim3(256), metro_lds_bytes<double>(n, m.W),
+                           m.w.s, a);
+    }
+    return launch_check("k_metro_slice");
+}
+
+// G <- B_l G B_l^-1 = expK diag(d_l) G diag(1/d_l) expKinv (src/green.c:509-527)
+static int stage_wrap(dqmc_gpu_ctx *ctx, Model &m, int l)
+{
+    const int n = m.n, R = m.nrep;
+    const long long nn = (long long)n * n, sV = (long long)m.L * n;
+    if (is_mixed(ctx)) {
+        GemmArgs<float> g = gemm_args<float>(n, n, n, R, m.expKf, n, 0, 0, m.Gf, n, nn, 0, m.w.F1, n, nn);
+        g.br = vec<float>(m.dvecf + (long long)l * n, sV);
+        g.bc = vec<float>(m.dinvf + (long long)l * n, sV);
+        if (gemm(m.w, g)) return 1;
+        g = gemm_args<float>(n, n, n, R, m.w.F1, n, nn, 0, m.expKinvf, n, 0, 0, m.Gf, n, nn);
+        return gemm(m.w, g);
+    }
+    GemmArgs<double> g = gemm_args<double>(n, n, n, R, m.expK, n, 0, 0, m.G, n, nn, 0, m.w.W1, n, nn);
+    g.br = vec<double>(m.dvec + (long long)l * n, sV);
+    g.bc = vec<double>(m.dinv + (long long)l * n, sV);
+    if (gemm(m.w, g)) return 1;
+    g = gemm_args<double>(n, n, n, R, m.w.W1, n, nn, 0, m.expKinv, n, 0, 0, m.G, n, nn);
+    return gemm(m.w, g);
+}
+
+// FP32 B-block B_{lb+len-1} ... B_{lb} (green_build_Bblock, src/green.c:151-166)
+// into F1 or F2; returns the buffer holding it.
+static float *bblock_f32(Model &m, int lb, int len)
+{
+    const int n = m.n, R = m.nrep;
+    const long long nn = (long long)n * n, sV = (long long)m.L * n;
+    hipLaunchKernelGGL(k_scale_cols_f, grid2(nn, R), dim3(256), 0, m.w.s, m.expKf,
+                       m.dvecf + (long long)lb * n, sV, m.w.F1, nn, n, R);
+    float *cur = m.w.F1, *oth = m.w.F2;
+    for (int l = lb + 1; l < lb + len; l++) {
+        GemmArgs<float> g = gemm_args<float>(n, n, n, R, m.expKf, n, 0, 0, cur, n, nn, 0, oth, n, nn);
+        g.br = vec<float>(m.dvecf + (long long)l * n, sV);
+        g.status = m.w.status;
+        gemm_launch(g, m.w.s);
+        std::swap(cur, oth);
+    }
+    return cur;
+}
+
+// left <- B_{lb+len-1} ... B_{lb} left   (udv_lmul_work, src/linalg.c:515-642)
+static int stage_lmul(dqmc_gpu_ctx *ctx, Model &m, const Udv &lf, int lb, int len)
+{
+    const int n = m.n, R = m.nrep;
+    const long long nn = (long long)n * n, sV = (long long)m.L * n;
+    double *cur;
+    if (is_mixed(ctx)) {
+        float *Bf = bblock_f32(m, lb, len);
+        hipLaunchKernelGGL(k_f2d, grid2(nn, R), dim3(256), 0, m.w.s, Bf, nn, m.w.W3, nn, nn, R);
+        GemmArgs<double> g = gemm_args<double>(n, n, n, R, m.w.W3, n, nn, 0, lf.U, n, lf.sU, 0, m.w.W1, n, nn);
+        g.bc = vec(lf.D, lf.sD);
+        if (gemm(m.w, g)) return 1;
+        cur = m.w.W1;
+    } else {
+        // (B U) diag(D) with B applied one slice at a time
+        GemmArgs<double> g = gemm_args<double>(n, n, n, R, m.expK, n, 0, 0, lf.U, n, lf.sU, 0, m.w.W1, n, nn);
+        g.br = vec<double>(m.dvec + (long long)lb * n, sV);
+        g.bc = vec(lf.D, lf.sD);
+        if (gemm(m.w, g)) return 1;
+        cur = m.w.W1;
+        double *oth = m.w.W2;
+        for (int l = lb + 1; l < lb + len; l++) {
+            g = gemm_args<double>(n, n, n, R, m.expK, n, 0, 0, cur, n, nn, 0, oth, n, nn);
+            g.br = vec<double>(m.dvec + (long long)l * n, sV);
+            if (gemm(m.w, g)) return 1;
+            std::swap(cur, oth);
+        }
+    }
+    // QR: U' = Q, D' = diag R, T' = (D'^-1 R) T
+    if (qr(m.w, cur, lf.U, lf.sU, m.w.W3, nn, lf.D, lf.sD, lf.q, lf.sq)) return 1;
+    GemmArgs<double> g = gemm_args<double>(n, n, n, R, m.w.W3, n, nn, 0, lf.T, n, lf.sU, 0, m.w.W4, n, nn);
+    if (gemm(m.w, g)) return 1;
+    return copy_nn(m.w, m.w.W4, nn, lf.T, lf.sU, nn);
+}
+
+// S[j] = S[j+1] B_{b_{j+1}-1} ... B_{b_j}   (green_stack_build_suffix +
+// udv_rmul, src/green.c:203-226, src/linalg.c:662-784)
+static int stage_rmul(dqmc_gpu_ctx *ctx, Model &m, int j)
+{
+    const int n = m.n, R = m.nrep;
+    const long long nn = (long long)n * n, sV = (long long)m.L * n;
+    const int bj = std::min(j * m.stab, m.L), bj1 = std::min((j + 1) * m.stab, m.L);
+    const Udv s1 = m.stack(j + 1), s0 = m.stack(j);
+    double *cur;
+    if (is_mixed(ctx)) {
+        float *Bf = bblock_f32(m, bj, bj1 - bj);
+        hipLaunchKernelGGL(k_f2d, grid2(nn, R), dim3(256), 0, m.w.s, Bf, nn, m.w.W3, nn, nn, R);
+        GemmArgs<double> g = gemm_args<double>(n, n, n, R, s1.T, n, s1.sU, 0, m.w.W3, n, nn, 0, m.w.W1, n, nn);
+        g.ar = vec(s1.D, s1.sD);
+        if (gemm(m.w, g)) return 1;
+        cur = m.w.W1;
+    } else {
+        // diag(D) T B_top ... B_bj, B applied one slice at a time from the right
+        const int top = bj1 - 1;
+        GemmArgs<double> g = gemm_args<double>(n, n, n, R, s1.T, n, s1.sU, 0, m.expK, n, 0, 0, m.w.W1, n, nn);
+        g.ar = vec(s1.D, s1.sD);
+        g.cc = vec<double>(m.dvec + (long long)top * n, sV);
+        if (gemm(m.w, g)) return 1;
+        cur = m.w.W1;
+        double *oth = m.w.W2;
+        for (int l = top - 1; l >= bj; l--) {
+            g = gemm_args<double>(n, n, n, R, cur, n, nn, 0, m.expK, n, 0, 0, oth, n, nn);
+            g.cc = vec<double>(m.dvec + (long long)l * n, sV);
+            if (gemm(m.w, g)) return 1;
+            std::swap(cur, oth);
+        }
+    }
+    double *Qb = (cur == m.w.W1) ? m.w.W2 : m.w.W1;
+    if (qr(m.w, cur, Qb, nn, s0.T, s0.sU, s0.D, s0.sD, m.w.I1, 1)) return 1;
+    GemmArgs<double> g = gemm_args<double>(n, n, n, R, s1.U, n, s1.sU, 0, Qb, n, nn, 0, s0.U, n, s0.sU);
+    if (gemm(m.w, g)) return 1;
+    hipLaunchKernelGGL(k_mul_q, dim3((R + 255) / 256), dim3(256), 0, m.w.s, s1.q, s1.sq, m.w.I1, 1LL, s0.q,
+                       s0.sq, R, m.w.status);
+    return launch_check("k_mul_q");
+}
+
+static int udv_identity(Model &m, const Udv &u)
+{
+    const long long nn = (long long)m.n * m.n;
+    hipLaunchKernelGGL(k_udv_identity, grid2(nn, m.nrep), dim3(256), 0, m.w.s, u.U, u.sU, u.D, u.sD, u.T,
+                       u.sU, u.q, u.sq, m.n, m.nrep);
+    return launch_check("k_udv_identity");
+}
+
+// G <- Gnew with drift bookkeeping and the FP32 copy
+static int stage_accept_rebuild(dqmc_gpu_ctx *ctx, Model &m, int record_drift)
+{
+    const long long nn = (long long)m.n * m.n;
+    hipLaunchKernelGGL(k_drift_copy, dim3(m.nrep), dim3(256), 0, m.w.s, m.G, m.Gf, m.Gnew, nn, m.n, m.nrep,
+                       record_drift && ctx->opts.drift ? 1 : 0, is_mixed(ctx) ? 1 : 0, m.dmax, m.dsum, m.dcnt,
+                       m.w.status, m.w.reason, m.sign);
+    return launch_check("k_drift_copy");
+}
+
+// green_from_scratch(G, l0) (src/green.c:262-306): B_{l0-1} ... B_0 B_{L-1}
+// ... B_{l0} accumulated into the left UDV in blocks of at most stab slices.
+static int stage_from_scratch(dqmc_gpu_ctx *ctx, Model &m, int l0)
+{
+    const Udv lf = m.left();
+    if (udv_identity(m, lf)) return 1;
+    int segs[2][2] = {{l0, m.L}, {0, l0}};
+    for (auto &sg : segs)
+        for (int lb = sg[0]; lb < sg[1]; lb += m.stab)
+            if (stage_lmul(ctx, m, lf, lb, std::min(m.stab, sg[1] - lb))) return 1;
+    if (udv_inv_one_plus(m.w, lf, m.Gnew, (long long)m.n * m.n)) return 1;
+    return stage_accept_rebuild(ctx, m, 0);
+}
+
+// One forward sweep (dqmc_sweep_forward, PH branch, src/dqmc.c:609-813)
+static int sweep_once(dqmc_gpu_ctx *ctx, Model &m)
+{
+    for (int j = m.M - 1; j >= 1; j--)
+        if (stage_rmul(ctx, m, j)) return 1;
+    const Udv lf = m.left();
+    if (udv_identity(m, lf)) return 1;
+    int tau_left = 0;
+    for (int l = 0; l < m.L; l++) {
+        if (stage_metro(ctx, m, l)) return 1;
+        if (stage_wrap(ctx, m, l)) return 1;
+        if (((l + 1) % m.stab) == 0 && (l + 1) < m.L) {
+            const int tau = l + 1, j = tau / m.stab;
+            if (stage_lmul(ctx, m, lf, tau_left, tau - tau_left)) return 1;
+            tau_left = tau;
+            if (udv_combine(m.w, lf, m.stack(j), m.comb())) return 1;
+            if (udv_inv_one_plus(m.w, m.comb(), m.Gnew, (long long)m.n * m.n)) return 1;
+            if (stage_accept_rebuild(ctx, m, 1)) return 1;
+        }
+    }
+    if (m.L - tau_left > 0)
+        if (stage_lmul(ctx, m, lf, tau_left, m.L - tau_left)) return 1;
+    if (udv_inv_one_plus(m.w, lf, m.Gnew, (long long)m.n * m.n)) return 1;
+    return stage_accept_rebuild(ctx, m, 1);
+}
+
+static int launch_sweep(dqmc_gpu_ctx *ctx, Model &m)
+{
+    if (!ctx->opts.use_graph) return sweep_once(ctx, m);
+    if (!m.gexec) {
+        HIPCHK(hipStreamBeginCapture(m.w.s, hipStreamCaptureModeThreadLocal));
+        const int rc = sweep_once(ctx, m);
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(m.w.s, &gr);
+        if (rc) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return 1;
+        }
+        if (e != hipSuccess) {
+            set_err("hipStreamEndCapture: %s", hipGetErrorString(e));
+            return 1;
+        }
+        m.graph = gr;
+        HIPCHK(hipGraphInstantiate(&m.gexec, m.graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(m.gexec, m.w.s));
+    return 0;
+}
+
+static int stage_measure(Model &m, int bin)
+{
+    MeasureArgs a;
+    memset(&a, 0, sizeof a);
+    a.n = m.n; a.Lx = m.Lx; a.Ly = m.Ly; a.nq = m.nq; a.batch = m.nrep;
+    a.G = m.G; a.sG = (long long)m.n * m.n;
+    a.thop = m.thop; a.bipart = m.bipart; a.phase = m.phase;
+    a.U = m.U; a.mu = m.mu;
+    a.bin = bin; a.nbin = std::max(m.nbin, 1);
+    a.bins = m.bins; a.counts = m.counts; a.bszz = m.bszz; a.bsperp = m.bsperp;
+    a.sign = m.sign; a.acc = m.acc; a.att = m.att; a.acc_prev = m.acc_prev; a.att_prev = m.att_prev;
+    a.status = m.w.status; a.reason = m.w.reason;
+    hipLaunchKernelGGL(k_measure, dim3(m.nrep), dim3(256), measure_lds_bytes(m.n), m.w.s, a);
+    return launch_check("k_measure");
+}
+
+static int set_lds_limits()
+{
+    static bool done = false;
+    if (done) return 0;
+    const int big = 160 * 1024;
+    HIPCHK(hipFuncSetAttribute((const void *)k_metro_slice<double>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+    HIPCHK(hipFuncSetAttribute((const void *)k_metro_slice<float>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+    HIPCHK(hipFuncSetAttribute((const void *)k_qr_udv, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+    HIPCHK(hipFuncSetAttribute((const void *)k_trtri_unit, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+    HIPCHK(hipFuncSetAttribute((const void *)k_measure, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+    done = true;
+    return 0;
+}
+
+// ------------------------------------------------------------------- ABI
+
+static uint64_t splitmix_next(uint64_t *x)
+{
+    uint64_t z = (*x += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+static double host_xoshiro(uint64_t s[4])
+{
+    const uint64_t x = s[1] * 5;
+    const uint64_t out = ((x << 7) | (x >> 57)) * 9;
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = (s[3] << 45) | (s[3] >> 19);
+    return (double)(out >> 11) * (1.0 / 9007199254740992.0);
+}
+
+extern "C" {
+
+const char *dqmc_gpu_last_error(void) { return g_err.c_str(); }
+
+int dqmc_gpu_create(dqmc_gpu_ctx **out, int device, const dqmc_gpu_opts *o)
+{
+    if (!out) {
+        set_err("null output pointer");
+        return 1;
+    }
+    *out = nullptr;
+    int count = 0;
+    HIPCHK(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) {
+        set_err("device %d not present (%d devices)", device, count);
+        return 1;
+    }
+    HIPCHK(hipSetDevice(device));
+    if (set_lds_limits()) return 1;
+    dqmc_gpu_ctx *c = new dqmc_gpu_ctx();
+    c->device = device;
+    c->opts.precision = DQMC_GPU_FP64;
+    c->opts.use_graph = 1;
+    c->opts.drift = 0;
+    c->opts.autotune = 0;
+    if (o) c->opts = *o;
+    if (c->opts.precision != DQMC_GPU_FP64 && c->opts.precision != DQMC_GPU_MIXED) {
+        set_err("unknown precision %d", c->opts.precision);
+        delete c;
+        return 1;
+    }
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        set_err("hipEventCreate failed");
+        delete c;
+        return 1;
+    }
+    *out = c;
+    return 0;
+}
+
+int dqmc_gpu_add_model(dqmc_gpu_ctx *ctx, int n, int L, int stab, int delay_k, const double *expK,
+                       const double *expKinv, const double *thop, const int *bipart, double U, double dtau,
+                       double mu, double lambda, int Lx, int Ly, int *model_id)
+{
+    if (!ctx || !expK || !expKinv || !thop || !bipart || !model_id) {
+        set_err("null argument");
+        return 1;
+    }
+    if (ctx->initialised) {
+        set_err("models must be added before dqmc_gpu_init");
+        return 1;
+    }
+    if (n <= 0 || n > 1024 || L <= 0 || stab <= 0 || Lx <= 0 || Ly <= 0 || Lx * Ly != n) {
+        set_err("invalid model sizes n=%d L=%d stab=%d Lx=%d Ly=%d", n, L, stab, Lx, Ly);
+        return 1;
+    }
+    if (!(dtau > 0.0) || !(U >= 0.0) || !std::isfinite(lambda)) {
+        set_err("invalid U=%g dtau=%g lambda=%g", U, dtau, lambda);
+        return 1;
+    }
+    Model *m = new Model();
+    const size_t nn = (size_t)n * n;
+    m->n = n; m->L = L; m->stab = stab; m->M = (L + stab - 1) / stab;
+    m->Lx = Lx; m->Ly = Ly; m->U = U; m->dtau = dtau; m->mu = mu; m->lambda = lambda;
+    m->cap = delay_k;
+    m->W = metro_window(n, delay_k > 0 ? delay_k : 16, sizeof(double));
+    m->Wf = metro_window(n, delay_k > 0 ? delay_k : 32, sizeof(float));
+    // expv for sigma = +1 (src/green.c:63-67), N = exp(-2 lambda sigma s) - 1 (src/field.c:10-17)
+    const double c = -dtau * U / 2.0;
+    m->expv[0] = std::exp(-lambda + c);
+    m->expv[1] = std::exp(lambda + c);
+    m->expvinv[0] = 1.0 / m->expv[0];
+    m->expvinv[1] = 1.0 / m->expv[1];
+    m->nu[0] = std::exp(-2.0 * lambda * 1.0 * -1.0) - 1.0;
+    m->nu[1] = std::exp(-2.0 * lambda * 1.0 * 1.0) - 1.0;
+    m->nd[0] = std::exp(-2.0 * lambda * -1.0 * -1.0) - 1.0;
+    m->nd[1] = std::exp(-2.0 * lambda * -1.0 * 1.0) - 1.0;
+    m->h_expK.assign(expK, expK + nn);
+    m->h_expKinv.assign(expKinv, expKinv + nn);
+    m->h_thop.assign(thop, thop + nn);
+    m->h_bipart.assign(bipart, bipart + n);
+    *model_id = (int)ctx->models.size();
+    ctx->models.push_back(m);
+    return 0;
+}
+
+int dqmc_gpu_add_replicas(dqmc_gpu_ctx *ctx, int model_id, int nrep, const unsigned long long *seeds,
+                          const signed char *fields, int *first_replica_id)
+{
+    if (!ctx || !seeds || !first_replica_id || model_id < 0 || model_id >= (int)ctx->models.size() ||
+        nrep <= 0) {
+        set_err("invalid add_replicas arguments");
+        return 1;
+    }
+    if (ctx->initialised) {
+        set_err("replicas must be added before dqmc_gpu_init");
+        return 1;
+    }
+    Model &m = *ctx->models[model_id];
+    if (m.nrep > 0 && ctx->models.size() > 1 && model_id != (int)ctx->models.size() - 1) {
+        set_err("replicas of a model must be added before the next model");
+        return 1;
+    }
+    if (m.nrep == 0) {
+        int first = 0;
+        for (Model *o : ctx->models)
+            if (o != &m) first += o->nrep;
+        m.first = first;
+    }
+    const size_t Ln = (size_t)m.L * m.n;
+    for (int r = 0; r < nrep; r++) {
+        uint64_t st[4], x = seeds[r];
+        for (int q = 0; q < 4; q++) st[q] = splitmix_next(&x); // rng_seed, src/rng.c:16-21
+        for (size_t k = 0; k < Ln; k++) {                       // field_init, src/field.c:20-22
+            const double u = host_xoshiro(st);
+            if (fields) {
+                const signed char v = fields[(size_t)r * Ln + k];
+                if (v != 1 && v != -1) {
+                    set_err("field values must be +-1");
+                    return 1;
+                }
+                m.h_fields.push_back(v);
+            } else {
+                m.h_fields.push_back(u < 0.5 ? -1 : 1);
+            }
+        }
+        for (int q = 0; q < 4; q++) m.h_rng.push_back(st[q]);
+        m.seeds.push_back(seeds[r]);
+    }
+    *first_replica_id = m.first + m.nrep;
+    m.nrep += nrep;
+    ctx->nrep_total += nrep;
+    return 0;
+}
+
+int dqmc_gpu_set_measure(dqmc_gpu_ctx *ctx, int model_id, int nq, const double *phase_cos, int nbin)
+{
+    if (!ctx || model_id < 0 || model_id >= (int)ctx->models.size() || nq < 0 || nbin < 0 ||
+        (nq > 0 && !phase_cos)) {
+        set_err("invalid set_measure arguments");
+        return 1;
+    }
+    if (ctx->initialised) {
+        set_err("measurement must be configured before dqmc_gpu_init");
+        return 1;
+    }
+    Model &m = *ctx->models[model_id];
+    m.nq = nq;
+    m.nbin = nbin;
+    m.h_phase.assign(phase_cos, phase_cos + (size_t)m.n * nq);
+    return 0;
+}
+
+static int sync_all(dqmc_gpu_ctx *ctx)
+{
+    for (Model *m : ctx->models) HIPCHK(hipStreamSynchronize(m->w.s));
+    return 0;
+}
+
+int dqmc_gpu_init(dqmc_gpu_ctx *ctx)
+{
+    if (!ctx) {
+        set_err("null context");
+        return 1;
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    for (Model *m : ctx->models) {
+        if (m->nrep == 0) continue;
+        if (!m->allocated && model_alloc(ctx, *m)) return 1;
+        if (stage_from_scratch(ctx, *m, 0)) return 1;
+    }
+    if (sync_all(ctx)) return 1;
+    ctx->initialised = true;
+    return 0;
+}
+
+static int require_init(dqmc_gpu_ctx *ctx)
+{
+    if (!ctx) {
+        set_err("null context");
+        return 1;
+    }
+    if (!ctx->initialised) {
+        set_err("dqmc_gpu_init has not been called");
+        return 1;
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    return 0;
+}
+
+int dqmc_gpu_sweep(dqmc_gpu_ctx *ctx, int nsweep, int bin)
+{
+    if (require_init(ctx)) return 1;
+    for (Model *m : ctx->models)
+        if (bin >= 0 && (m->nbin <= 0 || bin >= m->nbin)) {
+            set_err("bin %d outside [0, %d)", bin, m->nbin);
+            return 1;
+        }
+    if (sync_all(ctx)) return 1;
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->models.empty() ? nullptr : ctx->models[0]->w.s));
+    for (int s = 0; s < nsweep; s++) {
+        for (Model *m : ctx->models) {
+            if (m->nrep == 0) continue;
+            if (launch_sweep(ctx, *m)) return 1;
+            if (bin >= 0 && stage_measure(*m, bin)) return 1;
+        }
+    }
+    for (size_t k = 1; k < ctx->models.size(); k++) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(e, ctx->models[k]->w.s));
+        HIPCHK(hipStreamWaitEvent(ctx->models[0]->w.s, e, 0));
+        HIPCHK(hipEventDestroy(e));
+    }
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->models.empty() ? nullptr : ctx->models[0]->w.s));
+    HIPCHK(hipEventSynchronize(ctx->ev1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_ms = ms;
+    return sync_all(ctx);
+}
+
+int dqmc_gpu_run(dqmc_gpu_ctx *ctx, int nwarm, int nmeas_per_bin, int nbin)
+{
+    if (require_init(ctx)) return 1;
+    double total = 0.0;
+    if (nwarm > 0) {
+        if (dqmc_gpu_sweep(ctx, nwarm, -1)) return 1;
+        total += ctx->last_ms;
+    }
+    for (int b = 0; b < nbin; b++) {
+        if (dqmc_gpu_sweep(ctx, nmeas_per_bin, b)) return 1;
+        total += ctx->last_ms;
+    }
+    ctx->last_ms = total;
+    return 0;
+}
+
+int dqmc_gpu_last_elapsed_ms(dqmc_gpu_ctx *ctx, double *ms)
+{
+    if (!ctx || !ms) return 1;
+    *ms = ctx->last_ms;
+    return 0;
+}
+
+int dqmc_gpu_read_bins(dqmc_gpu_ctx *ctx, int replica, double *bins8, int *counts, double *szz, double *sperp)
+{
+    if (require_init(ctx)) return 1;
+    Model *m;
+    int r;
+    if (model_for_replica(ctx, replica, &m, &r)) return 1;
+    if (sync_all(ctx)) return 1;
+    const int nbin = std::max(m->nbin, 1);
+    if (bins8) HIPCHK(hipMemcpy(bins8, m->bins + (size_t)8 * nbin * r, (size_t)8 * nbin * 8, hipMemcpyDeviceToHost));
+    if (counts) HIPCHK(hipMemcpy(counts, m->counts + (size_t)nbin * r, (size_t)nbin * sizeof(int), hipMemcpyDeviceToHost));
+    if (m->nq > 0) {
+        if (szz) HIPCHK(hipMemcpy(szz, m->bszz + (size_t)m->nq * nbin * r, (size_t)m->nq * nbin * 8, hipMemcpyDeviceToHost));
+        if (sperp) HIPCHK(hipMemcpy(sperp, m->bsperp + (size_t)m->nq * nbin * r, (size_t)m->nq * nbin * 8, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int dqmc_gpu_read_status(dqmc_gpu_ctx *ctx, int *status, int *failure_reason, unsigned long long *accepted,
+                         unsigned long long *attempts, double *sign)
+{
+    if (require_init(ctx)) return 1;
+    if (sync_all(ctx)) return 1;
+    for (Model *m : ctx->models) {
+        const int R = m->nrep, f = m->first;
+        if (status) HIPCHK(hipMemcpy(status + f, m->w.status, R * sizeof(int), hipMemcpyDeviceToHost));
+        if (failure_reason) HIPCHK(hipMemcpy(failure_reason + f, m->w.reason, R * sizeof(int), hipMemcpyDeviceToHost));
+        if (accepted) HIPCHK(hipMemcpy(accepted + f, m->acc, R * 8, hipMemcpyDeviceToHost));
+        if (attempts) HIPCHK(hipMemcpy(attempts + f, m->att, R * 8, hipMemcpyDeviceToHost));
+        if (sign) HIPCHK(hipMemcpy(sign + f, m->sign, R * 8, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int dqmc_gpu_read_drift(dqmc_gpu_ctx *ctx, double *max_inf, double *mean_inf, unsigned long long *samples)
+{
+    if (require_init(ctx)) return 1;
+    if (sync_all(ctx)) return 1;
+    for (Model *m : ctx->models) {
+        const int R = m->nrep, f = m->first;
+        std::vector<double> mx(R), sm(R);
+        std::vector<unsigned long long> ct(R);
+        HIPCHK(hipMemcpy(mx.data(), m->dmax, R * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(sm.data(), m->dsum, R * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ct.data(), m->dcnt, R * 8, hipMemcpyDeviceToHost));
+        for (int r = 0; r < R; r++) {
+            if (max_inf) max_inf[f + r] = mx[r];
+            if (mean_inf) mean_inf[f + r] = ct[r] ? sm[r] / (double)ct[r] : 0.0;
+            if (samples) samples[f + r] = ct[r];
+        }
+    }
+    return 0;
+}
+
+int dqmc_gpu_read_green(dqmc_gpu_ctx *ctx, int replica, double *g)
+{
+    if (require_init(ctx)) return 1;
+    Model *m;
+    int r;
+    if (!g || model_for_replica(ctx, replica, &m, &r)) return 1;
+    if (sync_all(ctx)) return 1;
+    const size_t nn = (size_t)m->n * m->n;
+    if (is_mixed(ctx)) {
+        // the live G of the slice loop is the FP32 copy
+        std::vector<float> f(nn);
+        HIPCHK(hipMemcpy(f.data(), m->Gf + nn * r, nn * 4, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < nn; k++) g[k] = (double)f[k];
+        return 0;
+    }
+    HIPCHK(hipMemcpy(g, m->G + nn * r, nn * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int dqmc_gpu_write_green(dqmc_gpu_ctx *ctx, int replica, const double *g)
+{
+    if (require_init(ctx)) return 1;
+    Model *m;
+    int r;
+    if (!g || model_for_replica(ctx, replica, &m, &r)) return 1;
+    if (sync_all(ctx)) return 1;
+    const size_t nn = (size_t)m->n * m->n;
+    HIPCHK(hipMemcpy(m->G + nn * r, g, nn * 8, hipMemcpyHostToDevice));
+    if (m->Gf) {
+        std::vector<float> f(g, g + nn);
+        HIPCHK(hipMemcpy(m->Gf + nn * r, f.data(), nn * 4, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int dqmc_gpu_read_field(dqmc_gpu_ctx *ctx, int replica, signed char *s)
+{
+    if (require_init(ctx)) return 1;
+    Model *m;
+    int r;
+    if (!s || model_for_replica(ctx, replica, &m, &r)) return 1;
+    if (sync_all(ctx)) return 1;
+    const size_t Ln = (size_t)m->L * m->n;
+    HIPCHK(hipMemcpy(s, m->field + Ln * r, Ln, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int dqmc_gpu_write_field(dqmc_gpu_ctx *ctx, int replica, const signed char *s)
+{
+    if (require_init(ctx)) return 1;
+    Model *m;
+    int r;
+    if (!s || model_for_replica(ctx, replica, &m, &r)) return 1;
+    if (sync_all(ctx)) return 1;
+    const size_t Ln = (size_t)m->L * m->n;
+    std::vector<double> dv(Ln), di(Ln);
+    for (size_t k = 0; k < Ln; k++) {
+        if (s[k] != 1 && s[k] != -1) {
+            set_err("field values must be +-1");
+            return 1;
+        }
+        dv[k] = m->expv[s[k] > 0];
+        di[k] = m->expvinv[s[k] > 0];
+    }
+    HIPCHK(hipMemcpy(m->field + Ln * r, s, Ln, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m->dvec + Ln * r, dv.data(), Ln * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m->dinv + Ln * r, di.data(), Ln * 8, hipMemcpyHostToDevice));
+    if (m->dvecf) {
+        std::vector<float> fv(dv.begin(), dv.end()), fi(di.begin(), di.end());
+        HIPCHK(hipMemcpy(m->dvecf + Ln * r, fv.data(), Ln * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(m->dinvf + Ln * r, fi.data(), Ln * 4, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int dqmc_gpu_read_rng(dqmc_gpu_ctx *ctx, int replica, unsigned long long *s4)
+{
+    if (require_init(ctx)) return 1;
+    Model *m;
+    int r;
+    if (!s4 || model_for_replica(ctx, replica, &m, &r)) return 1;
+    if (sync_all(ctx)) return 1;
+    HIPCHK(hipMemcpy(s4, m->rng + 4 * (size_t)r, 32, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int dqmc_gpu_step(dqmc_gpu_ctx *ctx, int step, int arg)
+{
+    if (require_init(ctx)) return 1;
+    for (Model *m : ctx->models) {
+        if (m->nrep == 0) continue;
+        if ((step == DQMC_GPU_STEP_PROPOSE_SLICE || step == DQMC_GPU_STEP_WRAP) && (arg < 0 || arg >= m->L)) {
+            set_err("slice %d outside [0, %d)", arg, m->L);
+            return 1;
+        }
+        int rc;
+        switch (step) {
+        case DQMC_GPU_STEP_PROPOSE_SLICE: rc = stage_metro(ctx, *m, arg); break;
+        case DQMC_GPU_STEP_WRAP: rc = stage_wrap(ctx, *m, arg); break;
+        case DQMC_GPU_STEP_REBUILD:
+            if (arg < 0 || arg >= m->L) {
+                set_err("tau %d outside [0, %d)", arg, m->L);
+                return 1;
+            }
+            rc = stage_from_scratch(ctx, *m, arg);
+            break;
+        default: set_err("unknown step %d", step); return 1;
+        }
+        if (rc) return 1;
+    }
+    return sync_all(ctx);
+}
+
+void dqmc_gpu_destroy(dqmc_gpu_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    for (Model *m : ctx->models) {
+        if (m->w.s) (void)hipStreamSynchronize(m->w.s);
+        if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
+        if (m->graph) (void)hipGraphDestroy(m->graph);
+        for (void *p : m->owned) (void)hipFree(p);
+        m->w.release();
+        if (m->w.s) (void)hipStreamDestroy(m->w.s);
+        delete m;
+    }
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    delete ctx;
+}
+
+// ------------------------------------------- batched linear algebra on host
+
+struct HostBatch {
+    Work w;
+    std::vector<double *> bufs;
+    ~HostBatch()
+    {
+        for (double *p : bufs) (void)hipFree(p);
+        w.release();
+        if (w.s) (void)hipStreamDestroy(w.s);
+    }
+};
+
+static int hb_init(HostBatch &h, int device, int n, int batch)
+{
+    if (n <= 0 || n > 1024 || batch <= 0) {
+        set_err("invalid n=%d batch=%d", n, batch);
+        return 1;
+    }
+    HIPCHK(hipSetDevice(device));
+    if (set_lds_limits()) return 1;
+    HIPCHK(hipStreamCreateWithFlags(&h.w.s, hipStreamNonBlocking));
+    return h.w.alloc(n, batch, 0);
+}
+
+static int hb_upload(HostBatch &h, const double *src, size_t count, double **dst)
+{
+    if (dalloc(dst, count)) return 1;
+    h.bufs.push_back(*dst);
+    if (src) HIPCHK(hipMemcpy(*dst, src, count * 8, hipMemcpyHostToDevice));
+    return 0;
+}
+
+static int hb_int(HostBatch &h, int count, int **dst)
+{
+    double *p;
+    if (dalloc(&p, (size_t)count)) return 1;
+    h.bufs.push_back(p);
+    *dst = (int *)p;
+    return 0;
+}
+
+static int hb_status(HostBatch &h)
+{
+    HIPCHK(hipStreamSynchronize(h.w.s));
+    std::vector<int> st(h.w.nb), rs(h.w.nb);
+    HIPCHK(hipMemcpy(st.data(), h.w.status, h.w.nb * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(rs.data(), h.w.reason, h.w.nb * sizeof(int), hipMemcpyDeviceToHost));
+    for (int b = 0; b < h.w.nb; b++)
+        if (st[b]) {
+            set_err("batch entry %d failed (reason %d)", b, rs[b]);
+            return 1;
+        }
+    return 0;
+}
+
+static int hb_lr_mul(int device, int n, int batch, double *U, double *D, double *T, const double *B, int left)
+{
+    if (!U || !D || !T || !B) {
+        set_err("null argument");
+        return 1;
+    }
+    HostBatch h;
+    if (hb_init(h, device, n, batch)) return 1;
+    const size_t nn = (size_t)n * n;
+    double *dU, *dD, *dT, *dB;
+    int *q;
+    if (hb_upload(h, U, nn * batch, &dU) || hb_upload(h, D, (size_t)n * batch, &dD) ||
+        hb_upload(h, T, nn * batch, &dT) || hb_upload(h, B, nn * batch, &dB) || hb_int(h, batch, &q))
+        return 1;
+    Work &w = h.w;
+    const long long snn = (long long)nn;
+    if (left) {
+        // M = (B U) diag(D); QR -> U' = Q, D' = diag R, T' = (D'^-1 R) T
+        GemmArgs<double> g = gemm_args<double>(n, n, n, batch, dB, n, snn, 0, dU, n, snn, 0, w.W1, n, snn);
+        g.bc = vec<double>(dD, n);
+        if (gemm(w, g) || qr(w, w.W1, dU, snn, w.W3, snn, dD, n, q, 1)) return 1;
+        g = gemm_args<double>(n, n, n, batch, w.W3, n, snn, 0, dT, n, snn, 0, w.W4, n, snn);
+        if (gemm(w, g) || copy_nn(w, w.W4, snn, dT, snn, snn)) return 1;
+    } else {
+        // M = diag(D) (T B); QR -> U' = U Q, D' = diag R, T' = D'^-1 R
+        GemmArgs<double> g = gemm_args<double>(n, n, n, batch, dT, n, snn, 0, dB, n, snn, 0, w.W1, n, snn);
+        g.cr = vec<double>(dD, n);
+        if (gemm(w, g) || qr(w, w.W1, w.W2, snn, dT, snn, dD, n, q, 1)) return 1;
+        g = gemm_args<double>(n, n, n, batch, dU, n, snn, 0, w.W2, n, snn, 0, w.W4, n, snn);
+        if (gemm(w, g) || copy_nn(w, w.W4, snn, dU, snn, snn)) return 1;
+    }
+    if (hb_status(h)) return 1;
+    HIPCHK(hipMemcpy(U, dU, nn * batch * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(D, dD, (size_t)n * batch * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(T, dT, nn * batch * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int dqmc_gpu_udv_lmul(int device, int n, int batch, double *U, double *D, double *T, const double *B)
+{
+    return hb_lr_mul(device, n, batch, U, D, T, B, 1);
+}
+
+int dqmc_gpu_udv_rmul(int device, int n, int batch, double *U, double *D, double *T, const double *B)
+{
+    return hb_lr_mul(device, n, batch, U, D, T, B, 0);
+}
+
+// Orthogonal U from outside the engine: its determinant sign is not tracked,
+// so it is taken from an LU of U on the host (la_logdet, src/linalg.c:255-286).
+static int host_det_sign(int n, const double *A)
+{
+    std::vector<double> M(A, A + (size_t)n * n);
+    int s = 1;
+    for (int j = 0; j < n; j++) {
+        int p = j;
+        double best = std::fabs(M[j + (size_t)j * n]);
+        for (int i = j + 1; i < n; i++)
+            if (std::fabs(M[i + (size_t)j * n]) > best) {
+                best = std::fabs(M[i + (size_t)j * n]);
+                p = i;
+            }
+        if (M[p + (size_t)j * n] == 0.0) return 0;
+        if (p != j) {
+            s = -s;
+            for (int k = 0; k < n; k++) std::swap(M[j + (size_t)k * n], M[p + (size_t)k * n]);
+        }
+        const double piv = M[j + (size_t)j * n];
+        if (piv < 0.0) s = -s;
+        for (int i = j + 1; i < n; i++) {
+            const double f = M[i + (size_t)j * n] / piv;
+            for (int k = j + 1; k < n; k++) M[i + (size_t)k * n] -= f * M[j + (size_t)k * n];
+        }
+    }
+    return s;
+}
+
+int dqmc_gpu_udv_combine(int device, int n, int batch, const double *Ul, const double *Dl, const double *Tl,
+                         const double *Ur, const double *Dr, const double *Tr, double *Uo, double *Do,
+                         double *To)
+{
+    if (!Ul || !Dl || !Tl || !Ur || !Dr || !Tr || !Uo || !Do || !To) {
+        set_err("null argument");
+        return 1;
+    }
+    HostBatch h;
+    if (hb_init(h, device, n, batch)) return 1;
+    const size_t nn = (size_t)n * n;
+    double *a[9];
+    int *q[3];
+    const double *src[9] = {Ul, Dl, Tl, Ur, Dr, Tr, nullptr, nullptr, nullptr};
+    const size_t cnt[9] = {nn, (size_t)n, nn, nn, (size_t)n, nn, nn, (size_t)n, nn};
+    for (int k = 0; k < 9; k++)
+        if (hb_upload(h, src[k], cnt[k] * batch, &a[k])) return 1;
+    for (int k = 0; k < 3; k++)
+        if (hb_int(h, batch, &q[k])) return 1;
+    std::vector<int> qs(batch);
+    for (int b = 0; b < batch; b++) qs[b] = host_det_sign(n, Ul + nn * b);
+    HIPCHK(hipMemcpy(q[0], qs.data(), batch * sizeof(int), hipMemcpyHostToDevice));
+    const Udv l{a[0], a[1], a[2], q[0], (long long)nn, n, 1};
+    const Udv r{a[3], a[4], a[5], q[1], (long long)nn, n, 1};
+    const Udv o{a[6], a[7], a[8], q[2], (long long)nn, n, 1};
+    if (udv_combine(h.w, l, r, o) || hb_status(h)) return 1;
+    HIPCHK(hipMemcpy(Uo, a[6], nn * batch * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(Do, a[7], (size_t)n * batch * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(To, a[8], nn * batch * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int dqmc_gpu_udv_inv_one_plus(int device, int n, int batch, const double *U, const double *D, const double *T,
+                              double *g, int *sign)
+{
+    if (!U || !D || !T || !g) {
+        set_err("null argument");
+        return 1;
+    }
+    HostBatch h;
+    if (hb_init(h, device, n, batch)) return 1;
+    const size_t nn = (size_t)n * n;
+    double *dU, *dD, *dT, *dG;
+    int *q;
+    if (hb_upload(h, U, nn * batch, &dU) || hb_upload(h, D, (size_t)n * batch, &dD) ||
+        hb_upload(h, T, nn * batch, &dT) || hb_upload(h, nullptr, nn * batch, &dG) || hb_int(h, batch, &q))
+        return 1;
+    std::vector<int> qs(batch);
+    for (int b = 0; b < batch; b++) qs[b] = host_det_sign(n, U + nn * b);
+    HIPCHK(hipMemcpy(q, qs.data(), batch * sizeof(int), hipMemcpyHostToDevice));
+    const Udv u{dU, dD, dT, q, (long long)nn, n, 1};
+    if (udv_inv_one_plus(h.w, u, dG, (long long)nn) || hb_status(h)) return 1;
+    HIPCHK(hipMemcpy(g, dG, nn * batch * 8, hipMemcpyDeviceToHost));
+    if (sign) HIPCHK(hipMemcpy(sign, h.w.det_sign, batch * sizeof(int), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int dqmc_gpu_measure(int device, int Lx, int Ly, int batch, const double *gu, const double *thop,
+                     const int *bipart, double U, int nq, const double *phase_cos, double *scal5, double *szz,
+                     double *sperp)
+{
+    const int n = Lx * Ly;
+    if (!gu || !thop || !bipart || !scal5 || (nq > 0 && (!phase_cos || !szz || !sperp)) || nq < 0) {
+        set_err("null argument");
+        return 1;
+    }
+    HostBatch h;
+    if (hb_init(h, device, n, batch)) return 1;
+    const size_t nn = (size_t)n * n;
+    double *dG, *dth, *dph, *ds5, *dz, *dp;
+    int *dbp;
+    if (hb_upload(h, gu, nn * batch, &dG) || hb_upload(h, thop, nn, &dth) ||
+        hb_upload(h, phase_cos, (size_t)n * std::max(nq, 1), &dph) || hb_upload(h, nullptr, (size_t)5 * batch, &ds5) ||
+        hb_upload(h, nullptr, (size_t)std::max(nq, 1) * batch, &dz) ||
+        hb_upload(h, nullptr, (size_t)std::max(nq, 1) * batch, &dp) || hb_int(h, n, &dbp))
+        return 1;
+    HIPCHK(hipMemcpy(dbp, bipart, n * sizeof(int), hipMemcpyHostToDevice));
+    MeasureArgs a;
+    memset(&a, 0, sizeof a);
+    a.n = n; a.Lx = Lx; a.Ly = Ly; a.nq = nq; a.batch = batch;
+    a.G = dG; a.sG = (long long)nn; a.thop = dth; a.bipart = dbp; a.phase = dph; a.U = U;
+    a.scal5 = ds5; a.szz = dz; a.sperp = dp; a.bin = -1;
+    a.status = h.w.status; a.reason = h.w.reason;
+    hipLaunchKernelGGL(k_measure, dim3(batch), dim3(256), measure_lds_bytes(n), h.w.s, a);
+    if (launch_check("k_measure") || hb_status(h)) return 1;
+    HIPCHK(hipMemcpy(scal5, ds5, (size_t)5 * batch * 8, hipMemcpyDeviceToHost));
+    if (nq > 0) {
+        HIPCHK(hipMemcpy(szz, dz, (size_t)nq * batch * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(sperp, dp, (size_t)nq * batch * 8, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,255 @@
+"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+
+Each oracle function is checked against something it cannot simply restate:
+golden vectors written by the compiled reference (tests/golden/*.npz), the
+reference's own test fixtures, published test vectors, LAPACK through numpy,
+closed forms and invariants.  A deliberately wrong oracle (dropped term, wrong
+sign or index, transposed operand) fails at least one of these.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+P = lambda a: a.ctypes.data_as(C.c_void_p)
+
+
+def gold(name):
+    return np.load(os.path.join(GOLD, name), allow_pickle=False)
+
+
+# ------------------------------------------------------------------ RNG
+
+def test_splitmix_published_vector(orc):
+    # Vigna's splitmix64 reference outputs for seed 1234567
+    s = np.zeros(4, dtype=np.uint64)
+    orc.orc().orc_rng_seed(P(s), 1234567)
+    assert [int(x) for x in s] == [6457827717110365317, 3203168211198807973, 9817491932198370423,
+                                   4593380528125082431]
+
+
+def test_field_init_stream_matches_reference(orc):
+    g = gold("traj_chain4.npz")
+    r = orc.Replica("chain", 4, 1, 4.0, 0.1, 40, 8, 1)
+    # a fresh replica holds the reference's initial field and RNG state
+    r2 = orc.Replica("chain", 4, 1, 4.0, 0.1, 40, 8, 1)
+    assert np.array_equal(r2.field(), g["field0"])
+    assert np.array_equal(r2.rng_state(), g["rng0"])
+    del r
+
+
+def test_replica_seed(orc):
+    g = gold("seeds.npz")
+    for (b, i, r), v in zip(g["cases"], g["values"]):
+        assert orc.orc().orc_replica_seed(int(b), int(i), int(r)) == int(v)
+
+
+# --------------------------------------------------------------- model
+
+def test_lattice_and_model_match_reference(orc):
+    g = gold("model.npz")
+    for name, kind, Lx, Ly in (("chain4", "chain", 4, 1), ("sq4", "square", 4, 4)):
+        th, bp, bip = orc.lattice(kind, Lx, Ly)
+        assert bip
+        assert np.array_equal(th, g[f"{name}_thop"])
+        assert np.array_equal(bp, g[f"{name}_bipart"])
+        eK, eKi, lam = orc.model(th, 4.0, 0.1)
+        assert np.abs(eK - g[f"{name}_expK"]).max() < 1e-14
+        assert np.abs(eKi - g[f"{name}_expKinv"]).max() < 1e-14
+        assert lam == pytest.approx(float(g[f"{name}_lambda"]), rel=1e-15)
+        assert np.abs(eK @ eKi - np.eye(th.shape[0])).max() < 1e-13
+
+
+def test_expm_closed_form_2x2(orc):
+    # exp(s [[a, b], [b, a]]) = e^{sa} [[cosh sb, sinh sb], [sinh sb, cosh sb]]
+    a, b, s = 0.7, -1.3, 0.4
+    A = np.array([[a, b], [b, a]])
+    out = np.zeros(4)
+    orc.orc().orc_expm_sym(2, P(np.asfortranarray(A)), s, P(out))
+    e = np.exp(s * a) * np.array([[np.cosh(s * b), np.sinh(s * b)], [np.sinh(s * b), np.cosh(s * b)]])
+    assert np.abs(out.reshape(2, 2, order="F") - e).max() < 1e-15
+
+
+def test_lambda_identity(orc):
+    # cosh(lambda) = exp(dtau U / 2) (tests/test_field.c:13)
+    _, _, lam = orc.model(np.zeros((2, 2)), 6.0, 0.05)
+    assert np.cosh(lam) == pytest.approx(np.exp(0.05 * 6.0 / 2), rel=1e-15)
+
+
+# ------------------------------------------------------- linear algebra
+
+@pytest.mark.parametrize("n", [1, 5, 16, 33])
+def test_qr_matches_lapack_convention(orc, n):
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    a = np.asfortranarray(A).ravel(order="F").copy()
+    tau = np.zeros(n)
+    orc.orc().orc_geqrf(n, P(a), P(tau))
+    R = np.triu(a.reshape(n, n, order="F"))
+    Rl = np.linalg.qr(A, mode="r")  # LAPACK dgeqrf: same reflector signs
+    assert np.abs(R - Rl).max() < 1e-12 * max(1.0, np.abs(Rl).max())
+    orc.orc().orc_orgqr(n, P(a), P(tau))
+    Q = a.reshape(n, n, order="F")
+    assert np.abs(Q.T @ Q - np.eye(n)).max() < 1e-13
+    assert np.abs(Q @ R - A).max() < 1e-12
+
+
+def test_inverse_and_trtri(orc):
+    rng = np.random.default_rng(3)
+    n = 20
+    A = rng.standard_normal((n, n))
+    Ai = np.zeros(n * n)
+    assert orc.orc().orc_inverse(n, P(np.asfortranarray(A)), P(Ai)) == 0
+    assert np.abs(Ai.reshape(n, n, order="F") - np.linalg.inv(A)).max() < 1e-11
+    T = np.triu(rng.standard_normal((n, n)) * 0.4, 1) + np.diag(rng.standard_normal(n))
+    t = np.asfortranarray(T).ravel(order="F").copy()
+    orc.orc().orc_trtri_unit_upper(n, P(t))
+    Tu = np.triu(T, 1) + np.eye(n)  # the diagonal is taken as 1
+    assert np.abs(t.reshape(n, n, order="F") - np.linalg.inv(Tu)).max() < 1e-12
+
+
+def test_udv_ops_match_reference(orc):
+    g = gold("udv_ops_n16.npz")
+    n = 16
+    for name, fn in (("lmul", orc.orc().orc_udv_lmul), ("rmul", orc.orc().orc_udv_rmul)):
+        U, D, T = g["U"].copy(), g["D"].copy(), g["T"].copy()
+        assert fn(n, P(U), P(D), P(T), P(g["B"].copy())) == 0
+        assert np.allclose(D, g[f"{name}_D"], rtol=1e-12, atol=0)
+        assert np.abs(U - g[f"{name}_U"]).max() < 1e-12
+        assert np.abs(T - g[f"{name}_T"]).max() < 1e-12 * np.abs(g[f"{name}_T"]).max()
+    uo, do, to = np.zeros(n * n), np.zeros(n), np.zeros(n * n)
+    assert orc.orc().orc_udv_combine(n, P(g["U"]), P(g["D"]), P(g["T"]), P(g["U2"]), P(g["D2"]), P(g["T2"]),
+                                     P(uo), P(do), P(to)) == 0
+    assert np.allclose(do, g["comb_D"], rtol=1e-11, atol=0)
+    assert np.abs(uo - g["comb_U"]).max() < 1e-11
+    gg = np.zeros(n * n)
+    s = C.c_int()
+    assert orc.orc().orc_udv_inv_one_plus(n, P(uo), P(do), P(to), P(gg), C.byref(s)) == 0
+    assert s.value == int(g["inv_sign"])
+    assert np.abs(gg - g["inv_G"]).max() / np.abs(g["inv_G"]).max() < 1e-11
+
+
+def test_inv_one_plus_reference_fixture(orc):
+    """The reference's mpmath fixture: logD in [-645, 706], G to 1e-9
+    relative, exact sign (tests/test_udv_phase6a_fixture.c:97-130)."""
+    g = gold("ref_fixtures.npz")
+    n = 16
+    assert float(g["max_logD"]) - float(g["min_logD"]) == pytest.approx(1350.7731737311086, abs=1e-9)
+    G = np.zeros(n * n)
+    s = C.c_int()
+    assert orc.orc().orc_udv_inv_one_plus(n, P(g["U"].copy()), P(g["D"].copy()), P(g["T"].copy()), P(G),
+                                          C.byref(s)) == 0
+    assert s.value == int(g["det_sign"])
+    assert np.abs(G - g["G"]).max() / np.abs(g["G"]).max() <= 1e-9
+
+
+def test_inv_one_plus_dense_formula(orc):
+    rng = np.random.default_rng(11)
+    n = 12
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    T = np.triu(rng.standard_normal((n, n)) * 0.3, 1) + np.eye(n)
+    D = np.sign(rng.standard_normal(n)) * np.exp(rng.uniform(-2, 2, n))
+    G = np.zeros(n * n)
+    s = C.c_int()
+    assert orc.orc().orc_udv_inv_one_plus(n, P(np.asfortranarray(q)), P(D), P(np.asfortranarray(T)), P(G),
+                                          C.byref(s)) == 0
+    Mx = np.eye(n) + q @ np.diag(D) @ T
+    assert np.abs(G.reshape(n, n, order="F") - np.linalg.inv(Mx)).max() < 1e-12
+    assert s.value == int(np.sign(np.linalg.det(Mx)))
+
+
+# ------------------------------------------------------------- sweeps
+
+def test_trajectory_chain4_matches_reference(orc):
+    g = gold("traj_chain4.npz")
+    r = orc.Replica("chain", 4, 1, 4.0, 0.1, 40, 8, 1)
+    th, bp = r.thop, r.bipart
+    for s in range(20):
+        assert r.sweep() == 0
+        assert np.array_equal(r.field(), g["fields"][s]), s
+        assert r.stats()[0] == int(g["accepted"][s])
+        gu = r.green()[0]
+        assert np.abs(gu - g["green"][s]).max() < 1e-12
+        m, z, p = orc.measure(gu, th, bp, 4.0, 4, 1)
+        assert np.abs(m - g["meas"][s]).max() < 1e-11
+        assert np.abs(z - g["szz"][s]).max() < 1e-12
+        assert np.abs(p - g["sperp"][s]).max() < 1e-12
+
+
+def test_trajectory_8x8_matches_reference(orc):
+    g = gold("traj_sq8.npz")
+    for seed in (1, 2):
+        r = orc.Replica("square", 8, 8, 4.0, 0.1, 40, 10, seed)
+        assert np.array_equal(r.field(), g[f"field0_s{seed}"])
+        for s in range(3):
+            assert r.sweep() == 0
+            assert np.array_equal(r.field(), g[f"fields_s{seed}"][s])
+            assert r.stats()[0] == int(g[f"accepted_s{seed}"][s])
+            gu = r.green()[0]
+            assert np.trace(gu) == pytest.approx(float(g[f"trace_s{seed}"][s]), rel=1e-11)
+            assert np.linalg.norm(gu) == pytest.approx(float(g[f"fro_s{seed}"][s]), rel=1e-11)
+        ref = g[f"green_s{seed}"]
+        assert np.abs(r.green()[0] - ref).max() / np.abs(ref).max() < 1e-10
+
+
+def test_u0_green_closed_form(orc):
+    # U = 0: B_l = expK for every slice, G = (1 + exp(-beta K))^-1
+    # (tests/test_green_init.c:29-46)
+    L, dtau = 30, 0.1
+    r = orc.Replica("square", 4, 4, 0.0, dtau, L, 6, 5)
+    th = r.thop
+    w, V = np.linalg.eigh(th)
+    Gx = (V * (1.0 / (1.0 + np.exp(-L * dtau * w)))) @ V.T
+    assert np.abs(r.green()[0] - Gx).max() < 1e-12
+
+
+def test_drift_bound(orc):
+    # tests/test_dqmc_stab_drift.c:53-58 (PH case): Ltr=8, stab=4, U=4,
+    # 2 sweeps, 2 samples per sweep, max drift < 1e-8
+    r = orc.Replica("chain", 4, 1, 4.0, 0.1, 8, 4, 777, drift=True)
+    for _ in range(2):
+        assert r.sweep() == 0
+    mx, mn, cnt = r.drift()
+    assert cnt == 4
+    assert 0.0 <= mn <= mx < 1e-8
+
+
+# ----------------------------------------------------------- measurement
+
+def test_sum_rules_and_free_fermion(orc):
+    # sum_q Szz(q) = 1/4 (ntot - 2 n doublon), sum_q Sperp = 1/2 (...)
+    # (src/structure_factor.c:472-474); at U=0 Sperp = 2 Szz
+    # (tests/test_integration.c:75, 152)
+    g = gold("traj_chain4.npz")
+    r = orc.Replica("chain", 4, 1, 4.0, 0.1, 40, 8, 1)
+    gu = g["green"][-1]
+    m, z, p = orc.measure(gu, r.thop, r.bipart, 4.0, 4, 1)
+    rhs = m[3] - 2 * 4 * m[4]
+    assert z.sum() == pytest.approx(0.25 * rhs, abs=1e-12)
+    assert p.sum() == pytest.approx(0.5 * rhs, abs=1e-12)
+    r0 = orc.Replica("square", 4, 4, 0.0, 0.1, 20, 5, 3)
+    m0, z0, p0 = orc.measure(r0.green()[0], r0.thop, r0.bipart, 0.0, 4, 4)
+    assert np.abs(p0 - 2 * z0).max() < 1e-12
+
+
+def test_jackknife_is_standard_error(orc):
+    x = np.random.default_rng(1).standard_normal(37)
+    m, e = orc.jackknife(x)
+    assert m == pytest.approx(x.mean(), rel=1e-14)
+    assert e == pytest.approx(np.sqrt(((x - x.mean()) ** 2).sum() / (37 * 36)), rel=1e-12)
+
+
+@pytest.mark.skipif(not os.path.exists("/root/reference/src"), reason="compiled reference only in the build container")
+def test_oracle_vs_live_reference_4x4(orc):
+    # more seeds than the fixtures hold, straight against the compiled reference
+    for seed in (11, 12):
+        a = orc.Replica("square", 4, 4, 6.0, 0.125, 32, 6, seed)
+        b = orc.RefReplica("square", 4, 4, 6.0, 0.125, 32, 6, seed)
+        for _ in range(4):
+            a.sweep()
+            b.sweep()
+            assert np.array_equal(a.field(), b.field())
+            ga, gb = a.green()[0], b.green()[0]
+            assert np.abs(ga - gb).max() / np.abs(gb).max() < 1e-11
