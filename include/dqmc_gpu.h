@@ -30,8 +30,10 @@ extern "C" {
 
 /* Precision modes. */
 #define DQMC_GPU_FP64 0  /* every stage in FP64 */
-#define DQMC_GPU_MIXED 1 /* wrap, delayed updates and B-chains in FP32 on
-                            MFMA; QR, rebuild, sign and measurement FP64 */
+#define DQMC_GPU_MIXED 1 /* wrap and delayed updates in FP32 on MFMA;
+                            B-chains, QR, rebuild, sign, measurement FP64 */
+#define DQMC_GPU_MIXED_FP32_CHAIN 2 /* as MIXED, and the B-chain products
+                            B_{l+s-1}...B_l also in FP32 */
 
 /* Failure reasons: 0..3 as LINALG_FAILURE_* (src/linalg.h:48-53). */
 #define DQMC_GPU_FAIL_NONE 0
@@ -136,6 +138,27 @@ int dqmc_gpu_step(dqmc_gpu_ctx *ctx, int step, int arg);
 /* Wall time of the last dqmc_gpu_sweep / dqmc_gpu_run call measured with
    HIP events on the engine's streams, milliseconds. */
 int dqmc_gpu_last_elapsed_ms(dqmc_gpu_ctx *ctx, double *ms);
+
+/* Stage classes of dqmc_gpu_profile_sweep. */
+#define DQMC_GPU_PROF_METRO 0   /* Metropolis + delayed updates, one slice */
+#define DQMC_GPU_PROF_WRAP 1    /* G <- B G B^-1 */
+#define DQMC_GPU_PROF_STACK 2   /* one suffix-stack entry (B-chain + rmul) */
+#define DQMC_GPU_PROF_LMUL 3    /* left B-chain + lmul */
+#define DQMC_GPU_PROF_COMBINE 4 /* udv_combine */
+#define DQMC_GPU_PROF_INV 5     /* udv_inv_one_plus */
+#define DQMC_GPU_PROF_ACCEPT 6  /* drift bookkeeping and copy of the rebuilt G */
+#define DQMC_GPU_PROF_MEASURE 7 /* tau = 0 measurement into bin 0 */
+#define DQMC_GPU_PROF_QR 8      /* every QR (also counted in its stage) */
+#define DQMC_GPU_PROF_TRTRI 9   /* every triangular inverse (also in INV) */
+#define DQMC_GPU_PROF_N 10
+
+/* Profiling aid without a counterpart in the reference (which uses its
+   PROF_BEGIN/PROF_END region timers, src/profiler.h:93-94): runs nsweep
+   sweeps without graph capture (plus a measurement into bin 0 when bins are
+   configured), timing every stage with HIP events on its stream.
+   ms[DQMC_GPU_PROF_N] gets the summed milliseconds per class and
+   launches[DQMC_GPU_PROF_N] the number of timed stage invocations. */
+int dqmc_gpu_profile_sweep(dqmc_gpu_ctx *ctx, int nsweep, double *ms, int *launches);
 
 void dqmc_gpu_destroy(dqmc_gpu_ctx *ctx);
 

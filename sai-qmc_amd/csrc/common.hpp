@@ -8,8 +8,10 @@
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// 16x16x4 MFMA tile: lane l supplies A[l%16][l/16] and B[l/16][l%16] and
-// holds D[4*(l/16)+v][l%16], v = 0..3.
+// 16x16x4 MFMA tile: lane l supplies A[l%16][l/16] and B[l/16][l%16]; its
+// four accumulator values v = 0..3 hold column l%16 of D at the rows given by
+// Mfma<T>::row (measured on gfx950: FP32 keeps rows 4*(l/16)+v, FP64 keeps
+// rows l/16 + 4v).
 template <typename T> struct Mfma;
 template <> struct Mfma<double> {
     typedef f64x4 acc_t;
@@ -17,6 +19,7 @@ template <> struct Mfma<double> {
     {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
 };
 template <> struct Mfma<float> {
     typedef f32x4 acc_t;
@@ -24,6 +27,7 @@ template <> struct Mfma<float> {
     {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
 __device__ __forceinline__ double wave_sum(double v)

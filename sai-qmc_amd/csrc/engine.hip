@@ -107,27 +107,30 @@ __global__ void k_scale_cols_f(const float *expK, const float *d, long long sdv,
     }
 }
 
-// Db/Ds split of udv_inv_one_plus_work (src/linalg.c:963-984): dbinv = 1/Db,
-// ds = Ds, and the sign of prod_{|D_i|>1} D_i.
-__global__ void k_split_db(const double *D, long long sD, double *dbinv, double *ds, int *bigsign,
-                           int n, int batch, const int *status)
+// Rebuild operands of (1 + U D T)^-1 with D split as Db Ds (|Db| >= 1,
+// |Ds| <= 1, src/linalg.c:963-984): since 1 + U D T = U Db N with
+// N = Db^-1 U^T + Ds T, the Green's function is G = N^-1 X, X = Db^-1 U^T.
+// Writes N, X and the sign of prod_{|D_i|>1} D_i.
+__global__ void k_build_N(const double *U, const double *D, const double *T, long long sU, long long sD, double *N,
+                          double *X, int *bigsign, int n, int batch, const int *status)
 {
     const int b = blockIdx.x;
     if (b >= batch || status[b]) return;
     __shared__ int neg;
     if (threadIdx.x == 0) neg = 0;
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const double d = D[b * sD + i];
-        if (fabs(d) > 1.0) {
-            dbinv[(long long)b * n + i] = 1.0 / d;
-            ds[(long long)b * n + i] = 1.0;
-            if (d < 0.0) atomicXor(&neg, 1);
-        } else {
-            dbinv[(long long)b * n + i] = 1.0;
-            ds[(long long)b * n + i] = d;
-        }
+    const double *Ub = U + b * sU, *Tb = T + b * sU, *Db = D + b * sD;
+    double *Nb = N + (long long)b * n * n, *Xb = X + (long long)b * n * n;
+    for (long long idx = threadIdx.x; idx < (long long)n * n; idx += blockDim.x) {
+        const int i = (int)(idx % n), j = (int)(idx / n);
+        const double d = Db[i];
+        const bool big = fabs(d) > 1.0;
+        const double x = big ? Ub[j + (long long)i * n] / d : Ub[j + (long long)i * n];
+        Xb[idx] = x;
+        Nb[idx] = x + (big ? 1.0 : d) * Tb[idx];
     }
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        if (fabs(Db[i]) > 1.0 && Db[i] < 0.0) atomicXor(&neg, 1);
     __syncthreads();
     if (threadIdx.x == 0) bigsign[b] = neg ? -1 : 1;
 }
@@ -276,6 +279,35 @@ static dim3 grid2(long long len, int batch)
     return dim3((unsigned)bx, (unsigned)batch);
 }
 
+// --------------------------------------------------------------- profiling
+// With g_prof set (dqmc_gpu_profile_sweep), every stage is bracketed by HIP
+// events on the stream it runs on; classes are DQMC_GPU_PROF_*.
+struct ProfRec {
+    int cls;
+    hipEvent_t a, b;
+};
+static thread_local std::vector<ProfRec> *g_prof = nullptr;
+struct ProfScope {
+    ProfRec r;
+    hipStream_t s = nullptr;
+    bool on;
+    ProfScope(int cls, hipStream_t st) : on(g_prof != nullptr)
+    {
+        if (!on) return;
+        r.cls = cls;
+        s = st;
+        (void)hipEventCreate(&r.a);
+        (void)hipEventCreate(&r.b);
+        (void)hipEventRecord(r.a, s);
+    }
+    ~ProfScope()
+    {
+        if (!on) return;
+        (void)hipEventRecord(r.b, s);
+        g_prof->push_back(r);
+    }
+};
+
 // ------------------------------------------------------- linear algebra ops
 
 static int launch_check(const char *what)
@@ -293,6 +325,7 @@ static int launch_check(const char *what)
 static int qr(Work &w, double *A, double *Q, long long sQ, double *Rn, long long sR, double *D,
               long long sD, int *qsign, long long sq)
 {
+    ProfScope ps_(DQMC_GPU_PROF_QR, w.s);
     QrArgs a;
     a.n = w.n; a.batch = w.nb;
     a.A = A; a.sA = (long long)w.n * w.n;
@@ -308,6 +341,7 @@ static int qr(Work &w, double *A, double *Q, long long sQ, double *Rn, long long
 
 static int trtri(Work &w, const double *T, long long sT, double *X)
 {
+    ProfScope ps_(DQMC_GPU_PROF_TRTRI, w.s);
     TrtriArgs a;
     a.n = w.n; a.batch = w.nb;
     a.T = T; a.sT = sT;
@@ -334,6 +368,7 @@ static int copy_nn(Work &w, const double *src, long long sS, double *dst, long l
 // out = (Ul Q, diag R, (D^-1 R) Tr).
 static int udv_combine(Work &w, const Udv &l, const Udv &r, const Udv &o)
 {
+    ProfScope ps_(DQMC_GPU_PROF_COMBINE, w.s);
     const int n = w.n;
     const long long nn = (long long)n * n;
     GemmArgs<double> g = gemm_args<double>(n, n, n, w.nb, l.T, n, l.sU, 0, r.U, n, r.sU, 0, w.W1, n, nn);
@@ -350,32 +385,27 @@ static int udv_combine(Work &w, const Udv &l, const Udv &r, const Udv &o)
     return launch_check("k_mul_q");
 }
 
-// udv_inv_one_plus_work (src/linalg.c:911-1115) with M inverted through its
-// own QR: M = Q_M D_M Rn_M, so
-//   G = T^-1 M^-1 Db^-1 U^T = T^-1 Rn_M^-1 D_M^-1 Q_M^T Db^-1 U^T
-// and sgn det M = sgn det Q_M * prod sgn (D_M)_i.
+// udv_inv_one_plus_work (src/linalg.c:911-1115) in the form
+//   G = (Db^-1 U^T + Ds T)^-1 Db^-1 U^T = Rn_N^-1 D_N^-1 Q_N^T X,
+// N = Q_N D_N Rn_N by Householder QR, so no T^-1 is ever formed; the sign is
+// sgn det(1 + U D T) = sgn det U * prod_{|D_i|>1} sgn D_i * sgn det N with
+// det N = det Q_N prod (D_N)_i (src/linalg.c:1039-1083 computes the same
+// quantity through the LU of the reference's M).
 static int udv_inv_one_plus(Work &w, const Udv &u, double *Gout, long long sG)
 {
+    ProfScope ps_(DQMC_GPU_PROF_INV, w.s);
     const int n = w.n;
     const long long nn = (long long)n * n;
-    hipLaunchKernelGGL(k_split_db, dim3(w.nb), dim3(256), 0, w.s, u.D, u.sD, w.V1, w.V2, w.I1, n, w.nb,
-                       w.status);
-    if (launch_check("k_split_db")) return 1;
-    if (trtri(w, u.T, u.sU, w.W1)) return 1; // W1 = T^-1
-    GemmArgs<double> g = gemm_args<double>(n, n, n, w.nb, u.U, n, u.sU, 1, w.W1, n, nn, 0, w.W2, n, nn);
-    g.cr = vec<double>(w.V1, n);
-    g.dg = vec<double>(w.V2, n);
-    if (gemm(w, g)) return 1; // W2 = M = Db^-1 U^T T^-1 + Ds
-    if (qr(w, w.W2, w.W3, nn, w.W4, nn, w.V3, n, w.I2, 1)) return 1; // W3 = Q_M, V3 = D_M, W4 = Rn_M
-    if (trtri(w, w.W4, nn, w.W2)) return 1;               // W2 = Rn_M^-1
-    g = gemm_args<double>(n, n, n, w.nb, w.W3, n, nn, 1, u.U, n, u.sU, 1, w.W4, n, nn);
-    g.br = vec<double>(w.V1, n);
+    hipLaunchKernelGGL(k_build_N, dim3(w.nb), dim3(256), 0, w.s, u.U, u.D, u.T, u.sU, u.sD, w.W1, w.W4, w.I1, n,
+                       w.nb, w.status);
+    if (launch_check("k_build_N")) return 1;                         // W1 = N, W4 = X
+    if (qr(w, w.W1, w.W2, nn, w.W3, nn, w.V3, n, w.I2, 1)) return 1; // W2 = Q_N, V3 = D_N, W3 = Rn_N
+    if (trtri(w, w.W3, nn, w.W1)) return 1;                          // W1 = Rn_N^-1
+    GemmArgs<double> g = gemm_args<double>(n, n, n, w.nb, w.W2, n, nn, 1, w.W4, n, nn, 0, w.W3, n, nn);
     g.cr = vec<double>(w.V3, n, 1);
-    if (gemm(w, g)) return 1; // W4 = D_M^-1 Q_M^T Db^-1 U^T
-    g = gemm_args<double>(n, n, n, w.nb, w.W2, n, nn, 0, w.W4, n, nn, 0, w.W3, n, nn);
-    if (gemm(w, g)) return 1; // W3 = Rn_M^-1 W4
+    if (gemm(w, g)) return 1; // W3 = D_N^-1 Q_N^T X
     g = gemm_args<double>(n, n, n, w.nb, w.W1, n, nn, 0, w.W3, n, nn, 0, Gout, n, sG);
-    if (gemm(w, g)) return 1; // G = T^-1 W3
+    if (gemm(w, g)) return 1; // G = Rn_N^-1 W3
     hipLaunchKernelGGL(k_sign_finish, dim3(w.nb), dim3(256), 0, w.s, w.I1, u.q, u.sq, w.I2, w.V3, n,
                        w.det_sign, w.status, w.reason, w.nb);
     return launch_check("k_sign_finish");
@@ -460,21 +490,23 @@ static int model_for_replica(dqmc_gpu_ctx *ctx, int replica, Model **m, int *loc
     return 1;
 }
 
-// Largest window (delay depth) whose four W x n tiles fit the LDS budget.
+// Delay depth (window) for the Metropolis kernel: the largest of 32, 16, 8
+// not above the request whose two W x n pending-update tiles fit in LDS.
 static int metro_window(int n, int want, size_t elem)
 {
     const size_t budget = 150 * 1024;
     int W = want > 0 ? want : 32;
-    if (W > n) W = n;
-    while (W > 1 && (size_t)4 * W * n * elem + 11 * (size_t)n + 256 > budget) W--;
-    return W;
+    const int opts[] = {32, 16, 8};
+    for (int o : opts)
+        if (o <= W && metro_lds_bytes<double>(n, o) * elem / 8 <= budget) return o;
+    return 8;
 }
 
 static int model_alloc(dqmc_gpu_ctx *ctx, Model &m)
 {
     const int n = m.n, R = m.nrep, L = m.L;
     const size_t nn = (size_t)n * n;
-    const int mixed = ctx->opts.precision == DQMC_GPU_MIXED;
+    const int mixed = ctx->opts.precision != DQMC_GPU_FP64;
     HIPCHK(hipStreamCreateWithFlags(&m.w.s, hipStreamNonBlocking));
     if (m.w.alloc(n, R, mixed)) return 1;
     auto own = [&](void *p) {
@@ -570,11 +602,27 @@ static int model_alloc(dqmc_gpu_ctx *ctx, Model &m)
 
 // ----------------------------------------------------------------- stages
 
-static bool is_mixed(const dqmc_gpu_ctx *ctx) { return ctx->opts.precision == DQMC_GPU_MIXED; }
+static bool is_mixed(const dqmc_gpu_ctx *ctx) { return ctx->opts.precision != DQMC_GPU_FP64; }
+// B-chains (green_build_Bblock) in FP32 only in DQMC_GPU_MIXED_FP32_CHAIN
+static bool chain_f32(const dqmc_gpu_ctx *ctx) { return ctx->opts.precision == DQMC_GPU_MIXED_FP32_CHAIN; }
+
+template <typename T, int W> static void launch_metro(const MetroArgs<T> &a, hipStream_t s)
+{
+    const int threads = std::min(METRO_THREADS, std::max(64, ((a.n + 63) / 64) * 64));
+    hipLaunchKernelGGL((k_metro_slice<T, W>), dim3(a.batch), dim3(threads), metro_lds_bytes<T>(a.n, W), s, a);
+}
+
+template <typename T> static void launch_metro_w(const MetroArgs<T> &a, hipStream_t s)
+{
+    if (a.W >= 32) launch_metro<T, 32>(a, s);
+    else if (a.W >= 16) launch_metro<T, 16>(a, s);
+    else launch_metro<T, 8>(a, s);
+}
 
 // Metropolis over slice l (src/dqmc.c:656-679)
 static int stage_metro(dqmc_gpu_ctx *ctx, Model &m, int l)
 {
+    ProfScope ps_(DQMC_GPU_PROF_METRO, m.w.s);
     const int n = m.n;
     const long long nn = (long long)n * n, sV = (long long)m.L * n;
     if (is_mixed(ctx)) {
@@ -590,8 +638,7 @@ static int stage_metro(dqmc_gpu_ctx *ctx, Model &m, int l)
         a.rng = m.rng;
         a.nu[0] = m.nu[0]; a.nu[1] = m.nu[1]; a.nd[0] = m.nd[0]; a.nd[1] = m.nd[1];
         a.acc = m.acc; a.att = m.att; a.sign = m.sign; a.status = m.w.status;
-        hipLaunchKernelGGL(k_metro_slice<float>, dim3(m.nrep), dim3(256), metro_lds_bytes<float>(n, m.Wf),
-                           m.w.s, a);
+        launch_metro_w<float>(a, m.w.s);
     } else {
         MetroArgs<double> a;
         a.n = n; a.L = m.L; a.W = m.W; a.l = l; a.batch = m.nrep;
@@ -603,8 +650,7 @@ static int stage_metro(dqmc_gpu_ctx *ctx, Model &m, int l)
         a.rng = m.rng;
         a.nu[0] = m.nu[0]; a.nu[1] = m.nu[1]; a.nd[0] = m.nd[0]; a.nd[1] = m.nd[1];
         a.acc = m.acc; a.att = m.att; a.sign = m.sign; a.status = m.w.status;
-        hipLaunchKernelGGL(k_metro_slice<double>, dim3(m.nrep), dim3(256), metro_lds_bytes<double>(n, m.W),
-                           m.w.s, a);
+        launch_metro_w<double>(a, m.w.s);
     }
     return launch_check("k_metro_slice");
 }
@@ -612,6 +658,7 @@ static int stage_metro(dqmc_gpu_ctx *ctx, Model &m, int l)
 // G <- B_l G B_l^-1 = expK diag(d_l) G diag(1/d_l) expKinv (src/green.c:509-527)
 static int stage_wrap(dqmc_gpu_ctx *ctx, Model &m, int l)
 {
+    ProfScope ps_(DQMC_GPU_PROF_WRAP, m.w.s);
     const int n = m.n, R = m.nrep;
     const long long nn = (long long)n * n, sV = (long long)m.L * n;
     if (is_mixed(ctx)) {
@@ -652,10 +699,11 @@ static float *bblock_f32(Model &m, int lb, int len)
 // left <- B_{lb+len-1} ... B_{lb} left   (udv_lmul_work, src/linalg.c:515-642)
 static int stage_lmul(dqmc_gpu_ctx *ctx, Model &m, const Udv &lf, int lb, int len)
 {
+    ProfScope ps_(DQMC_GPU_PROF_LMUL, m.w.s);
     const int n = m.n, R = m.nrep;
     const long long nn = (long long)n * n, sV = (long long)m.L * n;
     double *cur;
-    if (is_mixed(ctx)) {
+    if (chain_f32(ctx)) {
         float *Bf = bblock_f32(m, lb, len);
         hipLaunchKernelGGL(k_f2d, grid2(nn, R), dim3(256), 0, m.w.s, Bf, nn, m.w.W3, nn, nn, R);
         GemmArgs<double> g = gemm_args<double>(n, n, n, R, m.w.W3, n, nn, 0, lf.U, n, lf.sU, 0, m.w.W1, n, nn);
@@ -688,12 +736,13 @@ static int stage_lmul(dqmc_gpu_ctx *ctx, Model &m, const Udv &lf, int lb, int le
 // udv_rmul, src/green.c:203-226, src/linalg.c:662-784)
 static int stage_rmul(dqmc_gpu_ctx *ctx, Model &m, int j)
 {
+    ProfScope ps_(DQMC_GPU_PROF_STACK, m.w.s);
     const int n = m.n, R = m.nrep;
     const long long nn = (long long)n * n, sV = (long long)m.L * n;
     const int bj = std::min(j * m.stab, m.L), bj1 = std::min((j + 1) * m.stab, m.L);
     const Udv s1 = m.stack(j + 1), s0 = m.stack(j);
     double *cur;
-    if (is_mixed(ctx)) {
+    if (chain_f32(ctx)) {
         float *Bf = bblock_f32(m, bj, bj1 - bj);
         hipLaunchKernelGGL(k_f2d, grid2(nn, R), dim3(256), 0, m.w.s, Bf, nn, m.w.W3, nn, nn, R);
         GemmArgs<double> g = gemm_args<double>(n, n, n, R, s1.T, n, s1.sU, 0, m.w.W3, n, nn, 0, m.w.W1, n, nn);
@@ -736,6 +785,7 @@ static int udv_identity(Model &m, const Udv &u)
 // G <- Gnew with drift bookkeeping and the FP32 copy
 static int stage_accept_rebuild(dqmc_gpu_ctx *ctx, Model &m, int record_drift)
 {
+    ProfScope ps_(DQMC_GPU_PROF_ACCEPT, m.w.s);
     const long long nn = (long long)m.n * m.n;
     hipLaunchKernelGGL(k_drift_copy, dim3(m.nrep), dim3(256), 0, m.w.s, m.G, m.Gf, m.Gnew, nn, m.n, m.nrep,
                        record_drift && ctx->opts.drift ? 1 : 0, is_mixed(ctx) ? 1 : 0, m.dmax, m.dsum, m.dcnt,
@@ -808,6 +858,7 @@ static int launch_sweep(dqmc_gpu_ctx *ctx, Model &m)
 
 static int stage_measure(Model &m, int bin)
 {
+    ProfScope ps_(DQMC_GPU_PROF_MEASURE, m.w.s);
     MeasureArgs a;
     memset(&a, 0, sizeof a);
     a.n = m.n; a.Lx = m.Lx; a.Ly = m.Ly; a.nq = m.nq; a.batch = m.nrep;
@@ -826,12 +877,17 @@ static int set_lds_limits()
 {
     static bool done = false;
     if (done) return 0;
-    const int big = 160 * 1024;
-    HIPCHK(hipFuncSetAttribute((const void *)k_metro_slice<double>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-    HIPCHK(hipFuncSetAttribute((const void *)k_metro_slice<float>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-    HIPCHK(hipFuncSetAttribute((const void *)k_qr_udv, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-    HIPCHK(hipFuncSetAttribute((const void *)k_trtri_unit, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-    HIPCHK(hipFuncSetAttribute((const void *)k_measure, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+    // 160 KiB of LDS per CU on gfx950, less each kernel's static LDS
+    const void *kernels[] = {(const void *)k_metro_slice<double, 32>, (const void *)k_metro_slice<double, 16>,
+                             (const void *)k_metro_slice<double, 8>,  (const void *)k_metro_slice<float, 32>,
+                             (const void *)k_metro_slice<float, 16>,  (const void *)k_metro_slice<float, 8>,
+                             (const void *)k_qr_udv, (const void *)k_trtri_unit, (const void *)k_measure};
+    for (const void *k : kernels) {
+        hipFuncAttributes at;
+        HIPCHK(hipFuncGetAttributes(&at, k));
+        const int dyn = 160 * 1024 - (int)at.sharedSizeBytes;
+        HIPCHK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+    }
     done = true;
     return 0;
 }
@@ -886,7 +942,7 @@ int dqmc_gpu_create(dqmc_gpu_ctx **out, int device, const dqmc_gpu_opts *o)
     c->opts.drift = 0;
     c->opts.autotune = 0;
     if (o) c->opts = *o;
-    if (c->opts.precision != DQMC_GPU_FP64 && c->opts.precision != DQMC_GPU_MIXED) {
+    if (c->opts.precision < DQMC_GPU_FP64 || c->opts.precision > DQMC_GPU_MIXED_FP32_CHAIN) {
         set_err("unknown precision %d", c->opts.precision);
         delete c;
         return 1;
@@ -912,7 +968,7 @@ int dqmc_gpu_add_model(dqmc_gpu_ctx *ctx, int n, int L, int stab, int delay_k, c
         set_err("models must be added before dqmc_gpu_init");
         return 1;
     }
-    if (n <= 0 || n > 1024 || L <= 0 || stab <= 0 || Lx <= 0 || Ly <= 0 || Lx * Ly != n) {
+    if (n <= 0 || n > METRO_MAX_N || L <= 0 || stab <= 0 || Lx <= 0 || Ly <= 0 || Lx * Ly != n) {
         set_err("invalid model sizes n=%d L=%d stab=%d Lx=%d Ly=%d", n, L, stab, Lx, Ly);
         return 1;
     }
@@ -1269,6 +1325,42 @@ int dqmc_gpu_step(dqmc_gpu_ctx *ctx, int step, int arg)
         if (rc) return 1;
     }
     return sync_all(ctx);
+}
+
+int dqmc_gpu_profile_sweep(dqmc_gpu_ctx *ctx, int nsweep, double *ms, int *launches)
+{
+    if (require_init(ctx)) return 1;
+    if (!ms || !launches || nsweep < 1) {
+        set_err("invalid profile_sweep arguments");
+        return 1;
+    }
+    if (sync_all(ctx)) return 1;
+    std::vector<ProfRec> recs;
+    g_prof = &recs;
+    int rc = 0;
+    for (int s = 0; s < nsweep && !rc; s++)
+        for (Model *m : ctx->models) {
+            if (m->nrep == 0) continue;
+            rc = sweep_once(ctx, *m);
+            if (!rc && m->nbin > 0) rc = stage_measure(*m, 0);
+            if (rc) break;
+        }
+    g_prof = nullptr;
+    const int sync_rc = sync_all(ctx);
+    for (int c = 0; c < DQMC_GPU_PROF_N; c++) {
+        ms[c] = 0.0;
+        launches[c] = 0;
+    }
+    for (ProfRec &r : recs) {
+        float t = 0.0f;
+        if (hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+            ms[r.cls] += t;
+            launches[r.cls] += 1;
+        }
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    return rc || sync_rc;
 }
 
 void dqmc_gpu_destroy(dqmc_gpu_ctx *ctx)

@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs<T> g)
         for (int nj = 0; nj < 2; nj++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int i = m0 + wr * 32 + mi * 16 + 4 * (lane >> 4) + v;
+                const int i = m0 + wr * 32 + mi * 16 + Mfma<T>::row(lane, v);
                 const int j = n0 + wc * 32 + nj * 16 + (lane & 15);
                 if (i < g.M && j < g.N) {
                     T r = g.alpha * acc[mi][nj][v] * g.cr.at(b, i) * g.cc.at(b, j);

@@ -1,21 +1,25 @@
 // Single-site Metropolis over one imaginary-time slice with delayed
-// updates, one workgroup per replica (forward PH sweep,
-// src/dqmc.c:656-679, src/green.c:384-475).
+// updates, one workgroup per replica and one thread per site (forward PH
+// sweep, src/dqmc.c:656-679, src/green.c:384-475).
 //
 // Sites are visited in windows of W consecutive sites.  At a window start
-// the window's W columns and W rows of G are staged in LDS; every accepted
-// flip appends one column pair (c, v) to the pending rank-p update
-// G~ = G - C V^T (the reference's delay_c / delay_v, src/green.c:418-460),
-// built entirely from LDS (no global traffic on the sequential path).  At
-// the window end the rank-p update is flushed into G in global memory
-// (la_gemm_nt_rect, src/green.c:462-475).  The window size W plays the
-// role of the delay depth k.
+// thread j holds in registers G~(j, i0+w) and G~(i0+w, j) for the window's W
+// sites (the window's columns and rows of the current Green's function), and
+// every thread holds the W diagonal entries G~(i0+w, i0+w).  An accepted flip
+// at site i = i0+w forms the reference's delayed-update pair
+// c = (N/R)(e_i - G~ e_i), v = G~^T e_i (green_delay_accept,
+// src/green.c:418-460) straight from those registers, publishes it to LDS,
+// and after one barrier every thread applies the rank-1 change to its
+// remaining window registers: no global traffic and no O(k) loop on the
+// sequential path.  At the window end the accumulated rank-p update
+// G -= C V^T (green_delay_flush / la_gemm_nt_rect, src/green.c:462-475,
+// src/linalg.c:209-219) is applied to G in global memory with MFMA 16x16x4
+// tiles.  The window size W is the delay depth k.
 //
-// The ratio of site i uses R = (1 + (1 - G~_ii) N_up)(1 + G~_ii N_dn)
-// (src/green.c:401-416) and the test rng_double() < |R| (src/dqmc.c:668),
-// always in FP64.  One RNG draw per proposal: the slice's n draws are taken
-// up front from the replica's xoshiro256** state (draws never depend on
-// acceptances).
+// The ratio is R = (1 + (1 - G~_ii) N_up)(1 + G~_ii N_dn)
+// (src/green.c:401-416), always evaluated in FP64, accepted iff
+// rng_double() < |R| (src/dqmc.c:668).  Every proposal consumes one draw of
+// the replica's xoshiro256** stream; the slice's n draws are taken up front.
 #pragma once
 #include "common.hpp"
 
@@ -37,37 +41,86 @@ template <typename T> struct MetroArgs {
     int *status;
 };
 
+#define METRO_THREADS 512
+#define METRO_MAX_N 512
+
 template <typename T>
 static inline size_t metro_lds_bytes(int n, int W)
 {
-    return sizeof(T) * (size_t)4 * W * n + sizeof(double) * n + 2 * (size_t)n + sizeof(T) * (W + 4) + 64;
+    return sizeof(T) * (size_t)2 * W * n + sizeof(double) * n + 2 * (size_t)n + 64;
 }
 
+// G[n x n] -= Cc[0:p]^T Vv[0:p] with Cc, Vv stored q-major (Cc[q*n + j]).
 template <typename T>
-__global__ __launch_bounds__(256) void k_metro_slice(MetroArgs<T> a)
+__device__ void flush_rank_p(T *G, int n, const T *Cc, const T *Vv, int p)
+{
+    typedef typename Mfma<T>::acc_t acc_t;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int lc = lane & 15, lr = lane >> 4;
+    const int nt = (n + 15) >> 4;
+    const int ntiles = nt * nt;
+    constexpr int B = 4; // tiles in flight per wave
+    for (int t0 = wave * B; t0 < ntiles; t0 += nw * B) {
+        acc_t acc[B];
+#pragma unroll
+        for (int u = 0; u < B; u++) {
+            const int t = t0 + u;
+            const int r0 = (t % nt) * 16, c0 = (t / nt) * 16;
+            const int col = c0 + lc;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = r0 + Mfma<T>::row(lane, v);
+                acc[u][v] = (t < ntiles && row < n && col < n) ? G[row + (long long)col * n] : T(0);
+            }
+        }
+        for (int q0 = 0; q0 < p; q0 += 4) {
+            const int q = q0 + lr;
+#pragma unroll
+            for (int u = 0; u < B; u++) {
+                const int t = t0 + u;
+                const int r0 = (t % nt) * 16, c0 = (t / nt) * 16;
+                const T a = (q < p && r0 + lc < n) ? -Cc[q * n + r0 + lc] : T(0);
+                const T b = (q < p && c0 + lc < n) ? Vv[q * n + c0 + lc] : T(0);
+                acc[u] = Mfma<T>::mma(a, b, acc[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < B; u++) {
+            const int t = t0 + u;
+            const int r0 = (t % nt) * 16, c0 = (t / nt) * 16;
+            const int col = c0 + lc;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = r0 + Mfma<T>::row(lane, v);
+                if (t < ntiles && row < n && col < n) G[row + (long long)col * n] = acc[u][v];
+            }
+        }
+    }
+}
+
+template <typename T, int W>
+__global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
 {
     const int b = blockIdx.x;
     if (b >= a.batch) return;
     if (a.status[b]) return;
-    const int n = a.n, W = a.W, tid = threadIdx.x, nt = blockDim.x;
+    const int n = a.n, j = threadIdx.x;
+    const bool act = j < n;
     T *G = a.G + (long long)b * a.sG;
     signed char *fld = a.field + (long long)b * a.sF + (long long)a.l * n;
 
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    T *Gc = (T *)lds_raw;       // W x n: Gc[w*n + j] = G(j, i0+w)
-    T *Gr = Gc + (size_t)W * n;  // W x n: Gr[w*n + j] = G(i0+w, j)
-    T *Cc = Gr + (size_t)W * n;  // W x n: pending columns c_q
-    T *Vv = Cc + (size_t)W * n;  // W x n: pending rows v_q
+    T *Cc = (T *)lds_raw;        // W x n pending columns c_q (also the row staging area)
+    T *Vv = Cc + (size_t)W * n;  // W x n pending rows v_q
     double *draws = (double *)(Vv + (size_t)W * n);
     signed char *fs = (signed char *)(draws + n);
     signed char *flip = fs + n;
-    T *dW = (T *)(((uintptr_t)(flip + n) + 15) & ~(uintptr_t)15);
 
-    for (int j = tid; j < n; j += nt) {
+    if (act) {
         fs[j] = fld[j];
         flip[j] = 0;
     }
-    if (tid == 0) {
+    if (j == 0) {
         uint64_t st[4];
         uint64_t *rs = a.rng + 4 * (long long)b;
         for (int q = 0; q < 4; q++) st[q] = rs[q];
@@ -77,61 +130,78 @@ __global__ __launch_bounds__(256) void k_metro_slice(MetroArgs<T> a)
     __syncthreads();
 
     int naccept = 0, negflips = 0;
+    T gc[W], gr[W], dW[W];
     for (int i0 = 0; i0 < n; i0 += W) {
         const int wn = min(W, n - i0);
-        for (int idx = tid; idx < wn * n; idx += nt) {
-            const int w = idx / n, j = idx % n;
-            Gc[w * n + j] = G[j + (long long)(i0 + w) * n];
-        }
-        for (int idx = tid; idx < wn * n; idx += nt) {
-            const int w = idx % wn, j = idx / wn;
-            Gr[w * n + j] = G[(i0 + w) + (long long)j * n];
+        // window columns (coalesced over j) and rows (staged through LDS)
+#pragma unroll
+        for (int w = 0; w < W; w++) gc[w] = (act && w < wn) ? G[j + (long long)(i0 + w) * n] : T(0);
+        for (int base = threadIdx.x; base < wn * n; base += 8 * blockDim.x) {
+            T v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int idx = base + u * blockDim.x;
+                v[u] = idx < wn * n ? G[(i0 + idx % wn) + (long long)(idx / wn) * n] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int idx = base + u * blockDim.x;
+                if (idx < wn * n) Cc[(idx % wn) * n + idx / wn] = v[u];
+            }
         }
         __syncthreads();
-        if (tid < wn) dW[tid] = Gc[tid * n + i0 + tid];
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            gr[w] = (act && w < wn) ? Cc[w * n + j] : T(0);
+            dW[w] = (w < wn) ? Cc[w * n + i0 + w] : T(0);
+        }
         __syncthreads();
         int p = 0;
-        for (int w = 0; w < wn; w++) {
-            const int i = i0 + w;
-            const double gii = (double)dW[w];
-            const int sp = fs[i] > 0;
-            const double Nu = a.nu[sp], Nd = a.nd[sp];
-            const double Ru = 1.0 + (1.0 - gii) * Nu;
-            const double Rd = 1.0 + gii * Nd;
-            const double R = Ru * Rd;
-            if (draws[i] < fabs(R)) {
-                const T fac = (T)(Nu / Ru);
-                for (int j = tid; j < n; j += nt) {
-                    T gji = Gc[w * n + j], gij = Gr[w * n + j];
-                    for (int q = 0; q < p; q++) {
-                        gji -= Cc[q * n + j] * Vv[q * n + i];
-                        gij -= Cc[q * n + i] * Vv[q * n + j];
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            if (w < wn) {
+                const int i = i0 + w;
+                const double gii = (double)dW[w];
+                const int sp = fs[i] > 0;
+                const double Nu = a.nu[sp], Nd = a.nd[sp];
+                const double Ru = 1.0 + (1.0 - gii) * Nu;
+                const double Rd = 1.0 + gii * Nd;
+                const double R = Ru * Rd;
+                if (draws[i] < fabs(R)) {
+                    const T fac = (T)(Nu / Ru);
+                    const T cj = fac * ((j == i ? T(1) : T(0)) - gc[w]);
+                    const T vj = gr[w];
+                    if (act) {
+                        Cc[p * n + j] = cj;
+                        Vv[p * n + j] = vj;
                     }
-                    const T cj = fac * ((j == i ? T(1) : T(0)) - gji);
-                    Cc[p * n + j] = cj;
-                    Vv[p * n + j] = gij;
-                    if (j > i && j < i0 + wn) dW[j - i0] -= cj * gij;
+                    if (j == 0) flip[i] = 1;
+                    naccept++;
+                    if (R < 0.0) negflips ^= 1;
+                    __syncthreads();
+#pragma unroll
+                    for (int w2 = w + 1; w2 < W; w2++) {
+                        if (w2 < wn) {
+                            const T cw = Cc[p * n + i0 + w2];
+                            const T vw = Vv[p * n + i0 + w2];
+                            gc[w2] -= cj * vw;
+                            gr[w2] -= cw * vj;
+                            dW[w2] -= cw * vw;
+                        }
+                    }
+                    p++;
                 }
-                if (tid == 0) flip[i] = 1;
-                naccept++;
-                if (R < 0.0) negflips ^= 1;
-                p++;
-                __syncthreads();
             }
         }
         if (p > 0) {
-            for (long long idx = tid; idx < (long long)n * n; idx += nt) {
-                const int j = (int)(idx % n), m = (int)(idx / n);
-                T g = G[idx];
-                for (int q = 0; q < p; q++) g -= Cc[q * n + j] * Vv[q * n + m];
-                G[idx] = g;
-            }
             __syncthreads();
+            flush_rank_p<T>(G, n, Cc, Vv, p);
         }
+        __syncthreads();
     }
     T *dv = a.dvec + (long long)b * a.sV + (long long)a.l * n;
     T *di = a.dinv + (long long)b * a.sV + (long long)a.l * n;
-    for (int j = tid; j < n; j += nt) {
+    if (act) {
         const signed char s = flip[j] ? (signed char)-fs[j] : fs[j];
         fld[j] = s;
         dv[j] = a.e[s > 0];
@@ -142,7 +212,7 @@ __global__ __launch_bounds__(256) void k_metro_slice(MetroArgs<T> a)
             a.dinv64[o] = a.einv64[s > 0];
         }
     }
-    if (tid == 0) {
+    if (j == 0) {
         a.acc[b] += (unsigned long long)naccept;
         a.att[b] += (unsigned long long)n;
         if (negflips) a.sign[b] = -a.sign[b];

@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(HERE, "libdqmc_gpu.so")
 
 FP64 = 0
 MIXED = 1
+MIXED_FP32_CHAIN = 2
 STEP_PROPOSE_SLICE = 0
 STEP_WRAP = 1
 STEP_REBUILD = 2
@@ -51,6 +52,7 @@ SIGNATURES = {
     "dqmc_gpu_read_rng": (_I, [_P, _I, _P]),
     "dqmc_gpu_step": (_I, [_P, _I, _I]),
     "dqmc_gpu_last_elapsed_ms": (_I, [_P, C.POINTER(_D)]),
+    "dqmc_gpu_profile_sweep": (_I, [_P, _I, _P, _P]),
     "dqmc_gpu_destroy": (None, [_P]),
     "dqmc_gpu_udv_lmul": (_I, [_I, _I, _I, _P, _P, _P, _P]),
     "dqmc_gpu_udv_rmul": (_I, [_I, _I, _I, _P, _P, _P, _P]),
@@ -282,6 +284,17 @@ class Engine:
         v = C.c_double()
         _check(lib().dqmc_gpu_last_elapsed_ms(self.h, C.byref(v)), "dqmc_gpu_last_elapsed_ms")
         return v.value
+
+    PROF_NAMES = ["metro", "wrap", "stack", "lmul", "combine", "inv_one_plus", "accept", "measure", "qr",
+                  "trtri"]
+
+    def profile(self, nsweep=1):
+        """Per-stage milliseconds and stage counts of nsweep sweeps
+        (dqmc_gpu_profile_sweep)."""
+        ms = np.zeros(len(self.PROF_NAMES))
+        cnt = np.zeros(len(self.PROF_NAMES), dtype=np.int32)
+        _check(lib().dqmc_gpu_profile_sweep(self.h, nsweep, _ptr(ms), _ptr(cnt)), "dqmc_gpu_profile_sweep")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(self.PROF_NAMES)}
 
     def _model_of(self, r):
         for m in self.models:

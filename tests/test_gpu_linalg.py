@@ -49,11 +49,11 @@ def relerr(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-@pytest.mark.parametrize("Lx,Ly", [(2, 2), (4, 4), (6, 6), (8, 8)])
+@pytest.mark.parametrize("Lx,Ly", [(2, 2), (4, 4), (6, 6), (8, 8), (16, 16), (20, 20)])
 def test_udv_lmul_rmul_vs_oracle(orc, eng_mod, Lx, Ly):
     n = Lx * Ly
-    b = 3
-    U, D, T = batch_udv(orc, Lx, Ly, b, 10, True, n)
+    b = 3 if n <= 64 else 1
+    U, D, T = batch_udv(orc, Lx, Ly, b, 10 if n <= 64 else 4, True, n)
     B = np.array(bmats(orc, Lx, Ly, b, np.random.default_rng(7 + n)))
     for left in (True, False):
         Ug, Dg, Tg = (eng_mod.udv_lmul if left else eng_mod.udv_rmul)(U, D, T, B)
@@ -66,12 +66,12 @@ def test_udv_lmul_rmul_vs_oracle(orc, eng_mod, Lx, Ly):
             assert relerr(Tg[k], To) < 1e-11
 
 
-@pytest.mark.parametrize("Lx,Ly", [(4, 4), (6, 6), (8, 8)])
-def test_udv_combine_inv_one_plus_vs_oracle(orc, eng_mod, Lx, Ly):
+@pytest.mark.parametrize("Lx,Ly,nl,nr", [(4, 4, 16, 24), (6, 6, 16, 24), (8, 8, 16, 24), (12, 12, 6, 6)])
+def test_udv_combine_inv_one_plus_vs_oracle(orc, eng_mod, Lx, Ly, nl, nr):
     n = Lx * Ly
     b = 2
-    Ul, Dl, Tl = batch_udv(orc, Lx, Ly, b, 16, True, 100 + n)
-    Ur, Dr, Tr = batch_udv(orc, Lx, Ly, b, 24, False, 200 + n)
+    Ul, Dl, Tl = batch_udv(orc, Lx, Ly, b, nl, True, 100 + n)
+    Ur, Dr, Tr = batch_udv(orc, Lx, Ly, b, nr, False, 200 + n)
     Uo, Do, To = eng_mod.udv_combine(Ul, Dl, Tl, Ur, Dr, Tr)
     g, sg = eng_mod.udv_inv_one_plus(Uo, Do, To)
     for k in range(b):
