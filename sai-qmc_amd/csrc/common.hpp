@@ -30,25 +30,48 @@ template <> struct Mfma<float> {
     static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-__device__ __forceinline__ double wave_sum(double v)
+// Wave-wide reductions on DPP lane moves (quad_perm swaps, half-row and row
+// mirrors, then row_bcast15 / row_bcast31 across the four 16-lane rows); the
+// total lands in lane 63 and is broadcast with readlane.  64-bit values move
+// as two 32-bit halves.
+template <int CTRL, int RMASK> __device__ __forceinline__ double dpp_move(double x)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, RMASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, RMASK, 0xF, false);
+    return __hiloint2double(hi, lo);
 }
-
-__device__ __forceinline__ float wave_sum(float v)
+template <int CTRL, int RMASK> __device__ __forceinline__ float dpp_move(float x)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, RMASK, 0xF, false));
+}
+__device__ __forceinline__ double lane63(double v)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
+
+template <typename T> __device__ __forceinline__ T wave_sum(T v)
+{
+    v += dpp_move<0xB1, 0xF>(v);
+    v += dpp_move<0x4E, 0xF>(v);
+    v += dpp_move<0x141, 0xF>(v);
+    v += dpp_move<0x140, 0xF>(v);
+    v += dpp_move<0x142, 0xA>(v);
+    v += dpp_move<0x143, 0xC>(v);
+    return lane63(v);
 }
 
 __device__ __forceinline__ double wave_max(double v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-    return v;
+    v = fmax(v, dpp_move<0xB1, 0xF>(v));
+    v = fmax(v, dpp_move<0x4E, 0xF>(v));
+    v = fmax(v, dpp_move<0x141, 0xF>(v));
+    v = fmax(v, dpp_move<0x140, 0xF>(v));
+    v = fmax(v, dpp_move<0x142, 0xA>(v));
+    v = fmax(v, dpp_move<0x143, 0xC>(v));
+    return lane63(v);
 }
 
 // Block-wide sum; `scratch` holds one slot per wave.  Every thread gets the

@@ -335,7 +335,14 @@ static int qr(Work &w, double *A, double *Q, long long sQ, double *Rn, long long
     a.Tws = w.Tws; a.sT = (long long)((w.n + QR_NB - 1) / QR_NB) * QR_NB * QR_NB;
     a.qsign = qsign; a.sq = sq;
     a.status = w.status; a.reason = w.reason;
-    hipLaunchKernelGGL(k_qr_udv, dim3(w.nb), dim3(QR_THREADS), qr_lds_bytes(w.n), w.s, a);
+    if (w.n <= 64)
+        hipLaunchKernelGGL(k_qr_udv<1>, dim3(w.nb), dim3(QR_THREADS), qr_lds_bytes(w.n), w.s, a);
+    else if (w.n <= 128)
+        hipLaunchKernelGGL(k_qr_udv<2>, dim3(w.nb), dim3(QR_THREADS), qr_lds_bytes(w.n), w.s, a);
+    else if (w.n <= 256)
+        hipLaunchKernelGGL(k_qr_udv<4>, dim3(w.nb), dim3(QR_THREADS), qr_lds_bytes(w.n), w.s, a);
+    else
+        hipLaunchKernelGGL(k_qr_udv<8>, dim3(w.nb), dim3(QR_THREADS), qr_lds_bytes(w.n), w.s, a);
     return launch_check("k_qr_udv");
 }
 
@@ -881,7 +888,8 @@ static int set_lds_limits()
     const void *kernels[] = {(const void *)k_metro_slice<double, 32>, (const void *)k_metro_slice<double, 16>,
                              (const void *)k_metro_slice<double, 8>,  (const void *)k_metro_slice<float, 32>,
                              (const void *)k_metro_slice<float, 16>,  (const void *)k_metro_slice<float, 8>,
-                             (const void *)k_qr_udv, (const void *)k_trtri_unit, (const void *)k_measure};
+                             (const void *)k_qr_udv<1>, (const void *)k_qr_udv<2>, (const void *)k_qr_udv<4>, (const void *)k_qr_udv<8>,
+                             (const void *)k_trtri_unit, (const void *)k_measure};
     for (const void *k : kernels) {
         hipFuncAttributes at;
         HIPCHK(hipFuncGetAttributes(&at, k));

@@ -47,7 +47,7 @@ template <typename T> struct MetroArgs {
 template <typename T>
 static inline size_t metro_lds_bytes(int n, int W)
 {
-    return sizeof(T) * (size_t)2 * W * n + sizeof(double) * n + 2 * (size_t)n + 64;
+    return sizeof(T) * ((size_t)2 * W * n + 4 * W) + sizeof(double) * n + 2 * (size_t)n + 64;
 }
 
 // G[n x n] -= Cc[0:p]^T Vv[0:p] with Cc, Vv stored q-major (Cc[q*n + j]).
@@ -59,7 +59,7 @@ __device__ void flush_rank_p(T *G, int n, const T *Cc, const T *Vv, int p)
     const int lc = lane & 15, lr = lane >> 4;
     const int nt = (n + 15) >> 4;
     const int ntiles = nt * nt;
-    constexpr int B = 4; // tiles in flight per wave
+    constexpr int B = 8; // tiles in flight per wave
     for (int t0 = wave * B; t0 < ntiles; t0 += nw * B) {
         acc_t acc[B];
 #pragma unroll
@@ -112,7 +112,12 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
     extern __shared__ __align__(16) unsigned char lds_raw[];
     T *Cc = (T *)lds_raw;        // W x n pending columns c_q (also the row staging area)
     T *Vv = Cc + (size_t)W * n;  // W x n pending rows v_q
-    double *draws = (double *)(Vv + (size_t)W * n);
+    // the accepted pair at the window's sites, double-buffered by the parity
+    // of the accept count so a fast thread never overwrites values a slow one
+    // is still reading
+    T *cwin = Vv + (size_t)W * n;
+    T *vwin = cwin + 2 * W;
+    double *draws = (double *)(vwin + 2 * W);
     signed char *fs = (signed char *)(draws + n);
     signed char *flip = fs + n;
 
@@ -174,6 +179,10 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
                     if (act) {
                         Cc[p * n + j] = cj;
                         Vv[p * n + j] = vj;
+                        if (j >= i0 && j < i0 + wn) {
+                            cwin[(p & 1) * W + j - i0] = cj;
+                            vwin[(p & 1) * W + j - i0] = vj;
+                        }
                     }
                     if (j == 0) flip[i] = 1;
                     naccept++;
@@ -182,8 +191,8 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
 #pragma unroll
                     for (int w2 = w + 1; w2 < W; w2++) {
                         if (w2 < wn) {
-                            const T cw = Cc[p * n + i0 + w2];
-                            const T vw = Vv[p * n + i0 + w2];
+                            const T cw = cwin[(p & 1) * W + w2];
+                            const T vw = vwin[(p & 1) * W + w2];
                             gc[w2] -= cj * vw;
                             gr[w2] -= cw * vj;
                             dW[w2] -= cw * vw;

@@ -52,38 +52,88 @@ __device__ __forceinline__ double panel_v(const double *P, int ld, int c, int r)
     return r > c ? P[c * ld + r] : (r == c ? 1.0 : 0.0);
 }
 
+__device__ __forceinline__ double bcast_lane(double v, int lane)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
 // X[p0:p0+m, j0:j0+16] <- (I - V op(T) V^T) X[...] for one 16-column block,
-// one wave.  V is the m x nb panel P (LDS), T is Tl (LDS, QR_NB x QR_NB,
+// one wave.  Rows are held in registers in the MFMA B-operand layout (lane
+// (lr, lc) holds rows 4u + lr of column j0 + lc, u < NU); for FP64 16x16x4
+// the B layout of four consecutive k-steps is exactly the accumulator
+// layout of a 16-row tile, so the registers that feed W = V^T X then receive
+// X - V W' in place.  Blocks of up to 4*NU rows are loaded once (64 loads in
+// flight per lane); taller blocks take a second pass that reloads each
+// chunk.  V is the panel P (LDS), T is Tl (LDS, QR_NB x QR_NB,
 // Tl[i + c*QR_NB], upper), Wl a wave-private LDS buffer of QR_NB x 16.
-// Columns at or beyond jend are left untouched (loads masked, stores skipped).
-__device__ void wy_apply_block(const double *P, int ld, const double *Tl, int nb, int m, double *X,
-                               int n, int p0, int j0, int jend, int transT, double *Wl)
+// Columns at or beyond jend are untouched.
+#define WY_NU 32
+__device__ __forceinline__ void wy_load(double (&xb)[WY_NU], const double *col, int r0, int m, bool jok, int lr)
+{
+#pragma unroll
+    for (int u = 0; u < WY_NU; u++) {
+        const int k = r0 + 4 * u + lr;
+        xb[u] = (jok && k < m) ? col[k] : 0.0;
+    }
+}
+
+__device__ __forceinline__ void wy_accum_w(const double (&xb)[WY_NU], const double *P, int ld, int nb, int r0, int m,
+                                           int lc, int lr, f64x4 &w0, f64x4 &w1)
+{
+#pragma unroll
+    for (int u = 0; u < WY_NU; u++) {
+        if (r0 + 4 * u < m) {
+            const int k = r0 + 4 * u + lr;
+            const double a0 = (lc < nb && k < m) ? panel_v(P, ld, lc, k) : 0.0;
+            const double a1 = (16 + lc < nb && k < m) ? panel_v(P, ld, 16 + lc, k) : 0.0;
+            w0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, xb[u], w0, 0, 0, 0);
+            w1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, xb[u], w1, 0, 0, 0);
+        }
+    }
+}
+
+__device__ __forceinline__ void wy_update_store(double (&xb)[WY_NU], const double (&wv)[QR_NB / 4], const double *P,
+                                                int ld, int nb, int r0, int m, double *col, bool jok, int lc, int lr)
+{
+#pragma unroll
+    for (int t = 0; t < WY_NU / 4; t++) {
+        if (r0 + 16 * t < m) {
+            f64x4 c = {xb[4 * t], xb[4 * t + 1], xb[4 * t + 2], xb[4 * t + 3]};
+            const int r = r0 + 16 * t + lc;
+#pragma unroll
+            for (int kk = 0; kk < QR_NB / 4; kk++) {
+                const int k = 4 * kk + lr;
+                const double a = (k < nb && r < m) ? -panel_v(P, ld, k, r) : 0.0;
+                c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, wv[kk], c, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) xb[4 * t + v] = c[v];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < WY_NU; u++) {
+        const int k = r0 + 4 * u + lr;
+        if (jok && k < m) col[k] = xb[u];
+    }
+}
+
+__device__ void wy_apply_block(const double *P, int ld, const double *Tl, int nb, int m, double *X, int n, int p0,
+                               int j0, int jend, int transT, double *Wl)
 {
     const int lane = threadIdx.x & 63;
     const int lc = lane & 15, lr = lane >> 4;
     const int j = j0 + lc;
     const bool jok = j < jend;
     double *col = X + (long long)(jok ? j : j0) * n + p0;
-    // W = V^T X  (QR_NB x 16): row tile t covers reflectors 16t..16t+15.
-    // Eight k-steps per batch so eight global loads are in flight per lane.
+    double xb[WY_NU];
     f64x4 w0 = {0, 0, 0, 0}, w1 = {0, 0, 0, 0};
-    for (int kb = 0; kb < m; kb += 32) {
-        double bv[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const int k = kb + 4 * u + lr;
-            bv[u] = (jok && k < m) ? col[k] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const int k = kb + 4 * u + lr;
-            const double a0 = (lc < nb && k < m) ? panel_v(P, ld, lc, k) : 0.0;
-            const double a1 = (16 + lc < nb && k < m) ? panel_v(P, ld, 16 + lc, k) : 0.0;
-            w0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv[u], w0, 0, 0, 0);
-            w1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv[u], w1, 0, 0, 0);
-        }
+    const int chunk = 4 * WY_NU;
+    for (int r0 = 0; r0 < m; r0 += chunk) {
+        wy_load(xb, col, r0, m, jok, lr);
+        wy_accum_w(xb, P, ld, nb, r0, m, lc, lr, w0, w1);
     }
-    // spill W to LDS: W[i][jj] at Wl[i*16 + jj]
 #pragma unroll
     for (int v = 0; v < 4; v++) {
         const int r = Mfma<double>::row(lane, v);
@@ -92,16 +142,17 @@ __device__ void wy_apply_block(const double *P, int ld, const double *Tl, int nb
     }
     __builtin_amdgcn_s_waitcnt(0xc07f);
     __builtin_amdgcn_wave_barrier();
-    // W' = op(T) W: op(T)[i][k] = T[k][i] (transT) or T[i][k]
     f64x4 u0 = {0, 0, 0, 0}, u1 = {0, 0, 0, 0};
+#pragma unroll
     for (int k0 = 0; k0 < QR_NB; k0 += 4) {
         const int k = k0 + lr;
-        const double b = Wl[k * 16 + lc];
+        const double bw = Wl[k * 16 + lc];
         const double t0 = transT ? Tl[k + lc * QR_NB] : Tl[lc + k * QR_NB];
         const double t1 = transT ? Tl[k + (16 + lc) * QR_NB] : Tl[(16 + lc) + k * QR_NB];
-        u0 = __builtin_amdgcn_mfma_f64_16x16x4f64(t0, b, u0, 0, 0, 0);
-        u1 = __builtin_amdgcn_mfma_f64_16x16x4f64(t1, b, u1, 0, 0, 0);
+        u0 = __builtin_amdgcn_mfma_f64_16x16x4f64(t0, bw, u0, 0, 0, 0);
+        u1 = __builtin_amdgcn_mfma_f64_16x16x4f64(t1, bw, u1, 0, 0, 0);
     }
+    __builtin_amdgcn_s_waitcnt(0xc07f);
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
     for (int v = 0; v < 4; v++) {
@@ -111,175 +162,243 @@ __device__ void wy_apply_block(const double *P, int ld, const double *Tl, int nb
     }
     __builtin_amdgcn_s_waitcnt(0xc07f);
     __builtin_amdgcn_wave_barrier();
-    // X -= V W' over 16-row tiles, four tiles (16 loads per lane) in flight
-    for (int rb = 0; rb < m; rb += 64) {
-        f64x4 c[4];
+    double wv[QR_NB / 4];
 #pragma unroll
-        for (int u = 0; u < 4; u++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int r = rb + 16 * u + Mfma<double>::row(lane, v);
-                c[u][v] = (jok && r < m) ? col[r] : 0.0;
-            }
-        for (int k0 = 0; k0 < QR_NB; k0 += 4) {
-            const int k = k0 + lr;
-            const double wv = Wl[k * 16 + lc];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int r = rb + 16 * u + lc;
-                const double a = (k < nb && r < m) ? -panel_v(P, ld, k, r) : 0.0;
-                c[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, wv, c[u], 0, 0, 0);
-            }
+    for (int kk = 0; kk < QR_NB / 4; kk++) wv[kk] = Wl[(4 * kk + lr) * 16 + lc];
+    if (m <= chunk) {
+        wy_update_store(xb, wv, P, ld, nb, 0, m, col, jok, lc, lr);
+    } else {
+        for (int r0 = 0; r0 < m; r0 += chunk) {
+            wy_load(xb, col, r0, m, jok, lr);
+            wy_update_store(xb, wv, P, ld, nb, r0, m, col, jok, lc, lr);
         }
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int r = rb + 16 * u + Mfma<double>::row(lane, v);
-                if (jok && r < m) col[r] = c[u][v];
-            }
     }
     __builtin_amdgcn_wave_barrier();
 }
 
-// Copy an m x nb block of global memory (leading dimension n) into the LDS
-// panel P (leading dimension ld) or back, eight elements per thread in flight.
-__device__ __forceinline__ void panel_load(double *P, int ld, const double *A, int n, int nb, int m, int *bad)
+// Panel factorisation with register-resident columns: wave w owns panel
+// columns c2 = 4k + w (slot k), lane l holds rows l + 64 s (slot s).  Step c
+// reads the unscaled pivot column from LDS (published by its owner during
+// step c-1), every wave updates all of its columns with one fused pass
+// (independent dot products, DPP reductions interleaved), the owner of
+// column c+1 publishes it with its norm, then one barrier.
+template <int MR>
+__device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int ld, double *xcol,
+                         double *s_alpha, double *s_nrm, double *s_sc, double *s_beta, double *Tl, int *bad,
+                         int *ntau)
 {
-    const int total = nb * m;
-    for (int base = threadIdx.x; base < total; base += 8 * blockDim.x) {
-        double v[8];
+    constexpr int NS = QR_NB / QR_WAVES; // column slots per wave
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double pc[NS][MR];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const int idx = base + u * blockDim.x;
-            v[u] = idx < total ? A[(idx % m) + (long long)(idx / m) * n] : 0.0;
+    for (int k = 0; k < NS; k++)
+#pragma unroll
+        for (int s = 0; s < MR; s++) {
+            const int c2 = NS * 0 + QR_WAVES * k + wave, r = lane + 64 * s;
+            pc[k][s] = (c2 < nb && r < m) ? A[(p0 + r) + (long long)(p0 + c2) * n] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const int idx = base + u * blockDim.x;
-            if (idx < total) {
-                if (bad && !isfinite(v[u])) *bad = 1;
-                P[(idx / m) * ld + (idx % m)] = v[u];
+    for (int k = 0; k < NS; k++)
+#pragma unroll
+        for (int s = 0; s < MR; s++)
+            if (!isfinite(pc[k][s])) *bad = 1;
+    // column 0 is owned by wave 0, slot 0
+    if (wave == 0) {
+        double q = 0.0;
+#pragma unroll
+        for (int s = 0; s < MR; s++) {
+            const int r = lane + 64 * s;
+            if (r < m) xcol[r] = pc[0][s];
+            if (r > 0 && r < m) q += pc[0][s] * pc[0][s];
+        }
+        q = wave_sum(q);
+        if (lane == 0) {
+            s_nrm[0] = q;
+            s_alpha[0] = pc[0][0];
+        }
+    }
+    __syncthreads();
+    for (int c = 0; c < nb; c++) {
+        const double *xc = xcol + (c & 1) * m;
+        double *xn = xcol + ((c + 1) & 1) * m;
+        const double alpha = s_alpha[c], xn2 = s_nrm[c];
+        double tau = 0.0, sc = 0.0, beta = alpha;
+        if (xn2 != 0.0) {
+            beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+            tau = (beta - alpha) / beta;
+            sc = 1.0 / (alpha - beta);
+        }
+        if (threadIdx.x == 0) {
+            if (xn2 != 0.0) (*ntau)++;
+            if (!isfinite(tau) || !isfinite(sc)) *bad = 1;
+            Tl[c + c * QR_NB] = tau;
+            s_sc[c] = sc;
+            s_beta[c] = beta;
+        }
+        double x[MR];
+#pragma unroll
+        for (int s = 0; s < MR; s++) {
+            const int r = lane + 64 * s;
+            x[s] = (r > c && r < m) ? xc[r] : 0.0;
+        }
+        double d[NS];
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int s = 0; s < MR; s++) acc += x[s] * pc[k][s];
+            d[k] = acc;
+        }
+#pragma unroll
+        for (int k = 0; k < NS; k++) d[k] = wave_sum(d[k]);
+        // the pivot row c < QR_NB <= 64 always sits in register slot 0
+        const int cl = c;
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            const int c2 = QR_WAVES * k + wave;
+            if (c2 > c && c2 < nb && tau != 0.0) {
+                const double pcc = bcast_lane(pc[k][0], cl);
+                const double w = tau * (pcc + sc * d[k]);
+                const double ws = w * sc;
+#pragma unroll
+                for (int s = 0; s < MR; s++) pc[k][s] -= ws * x[s];
+                if (lane == cl) pc[k][0] -= w;
+            }
+        }
+        // the owner of column c+1 publishes it (unscaled), its diagonal and
+        // the squared norm below the diagonal
+        if (c + 1 < nb && ((c + 1) % QR_WAVES) == wave) {
+            const int k1 = (c + 1) / QR_WAVES;
+            double q = 0.0;
+#pragma unroll
+            for (int k = 0; k < NS; k++) {
+                if (k == k1) {
+#pragma unroll
+                    for (int s = 0; s < MR; s++) {
+                        const int r = lane + 64 * s;
+                        if (r < m) xn[r] = pc[k][s];
+                        if (r > c + 1 && r < m) q += pc[k][s] * pc[k][s];
+                    }
+                }
+            }
+            q = wave_sum(q);
+            if (lane == 0) s_nrm[c + 1] = q;
+            if (lane == ((c + 1) & 63)) {
+                double av = 0.0;
+#pragma unroll
+                for (int k = 0; k < NS; k++)
+                    if (k == k1) av = pc[k][0];
+                s_alpha[c + 1] = av;
+            }
+        }
+        __syncthreads();
+    }
+    // final panel: R above and on the diagonal, scaled reflectors below;
+    // into LDS (for T and the trailing update) and back to A
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+        const int c2 = QR_WAVES * k + wave;
+        if (c2 < nb) {
+            const double sc = s_sc[c2], beta = s_beta[c2];
+#pragma unroll
+            for (int s = 0; s < MR; s++) {
+                const int r = lane + 64 * s;
+                if (r < m) {
+                    double v = pc[k][s];
+                    if (r == c2) v = beta;
+                    else if (r > c2 && sc != 0.0) v *= sc;
+                    P[c2 * ld + r] = v;
+                    A[(p0 + r) + (long long)(p0 + c2) * n] = v;
+                }
             }
         }
     }
+    __syncthreads();
 }
 
-__global__ __launch_bounds__(QR_THREADS) void k_qr_udv(QrArgs a)
+// T of the compact WY form Q = I - V T V^T of the panel.  For Householder
+// reflectors tau_i = 2 / v_i^T v_i, which gives T^-1 = diag(1/tau) +
+// triu(V^T V, 1) (reflectors with tau = 0 are the identity: their row and
+// column of T vanish).  V^T V on MFMA (one 16x16 tile per wave), then lane c
+// of wave 0 solves for column c of T.
+__device__ void qr_tfactor(const double *P, int ld, int nb, int m, double *Tl, double *Y)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lc = lane & 15, lr = lane >> 4;
+    if (wave < 4) {
+        const int ti = wave >> 1, tj = wave & 1;
+        f64x4 acc = {0, 0, 0, 0};
+        for (int k0 = 0; k0 < m; k0 += 4) {
+            const int k = k0 + lr;
+            const int ci = ti * 16 + lc, cj = tj * 16 + lc;
+            const double va = (ci < nb && k < m) ? panel_v(P, ld, ci, k) : 0.0;
+            const double vb = (cj < nb && k < m) ? panel_v(P, ld, cj, k) : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(va, vb, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = ti * 16 + Mfma<double>::row(lane, v), c = tj * 16 + lc;
+            Y[i + c * QR_NB] = acc[v];
+        }
+    }
+    __syncthreads();
+    if (wave == 0 && lane < QR_NB) {
+        // lane c: back substitution for column c of T, kept in LDS (own column)
+        const int c = lane;
+        double *t = Y + QR_NB * QR_NB + c * QR_NB;
+        const double tc = Tl[c + c * QR_NB];
+        for (int i = QR_NB - 1; i >= 0; i--) {
+            const double ti = Tl[i + i * QR_NB];
+            double s = (i == c) ? 1.0 : 0.0;
+            for (int k = i + 1; k <= c; k++) s -= Y[i + k * QR_NB] * t[k];
+            t[i] = (i <= c && c < nb && tc != 0.0) ? s * ti : 0.0;
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < QR_NB * QR_NB; idx += blockDim.x) Tl[idx] = Y[idx + QR_NB * QR_NB];
+    __syncthreads();
+}
+
+template <int MR>
+__global__ __launch_bounds__(QR_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_qr_udv(QrArgs a)
 {
     const int b = blockIdx.x;
     if (b >= a.batch) return;
     if (a.status && a.status[b]) return;
     const int n = a.n, tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
+    const int wave = tid >> 6;
     double *A = a.A + (long long)b * a.sA;
     double *Tw = a.Tws + (long long)b * a.sT;
 
     extern __shared__ double lds[];
-    double *P = lds;                     // QR_NB x n panel, P[c*n + r]
-    double *Tl = P + (size_t)QR_NB * n;  // QR_NB x QR_NB
-    double *Wl = Tl + QR_NB * QR_NB;     // QR_WAVES x (QR_NB x 16)
-    double *nrm = Wl + QR_WAVES * QR_NB * 16; // QR_NB column norms^2 (below diag)
-    double *scl = nrm + QR_NB;           // QR_NB 1/(alpha - beta)
+    const int ld = n + 1;                      // odd leading dimension: conflict-free column reads
+    double *P = lds;                           // QR_NB x n panel, P[c*ld + r]
+    double *Tl = P + (size_t)QR_NB * ld;       // QR_NB x QR_NB
+    double *Wl = Tl + QR_NB * QR_NB;           // QR_WAVES x (QR_NB x 16); also Y and T scratch
+    double *xcol = Wl + QR_WAVES * QR_NB * 16; // 2 x n pivot columns
+    double *s_alpha = xcol + 2 * n;
+    double *s_nrm = s_alpha + QR_NB;
+    double *s_sc = s_nrm + QR_NB;
+    double *s_beta = s_sc + QR_NB;
     __shared__ int s_bad, s_ntau;
     if (tid == 0) { s_bad = 0; s_ntau = 0; }
+    __syncthreads();
 
     const int npan = (n + QR_NB - 1) / QR_NB;
     for (int pi = 0; pi < npan; pi++) {
         const int p0 = pi * QR_NB;
         const int nb = min(QR_NB, n - p0);
         const int m = n - p0;
-        __syncthreads();
-        panel_load(P, n, A + p0 + (long long)p0 * n, n, nb, m, &s_bad);
         for (int idx = tid; idx < QR_NB * QR_NB; idx += blockDim.x) Tl[idx] = 0.0;
         __syncthreads();
-        if (wave == 0) {
-            double s = 0.0;
-            for (int r = 1 + lane; r < m; r += 64) s += P[r] * P[r];
-            s = wave_sum(s);
-            if (lane == 0) nrm[0] = s;
-        }
-        __syncthreads();
-        // ---- panel: one barrier per column
-        for (int c = 0; c < nb; c++) {
-            const double alpha = P[c * n + c];
-            const double xn2 = nrm[c];
-            double tau = 0.0, sc = 0.0, beta = alpha;
-            if (xn2 != 0.0) {
-                beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
-                tau = (beta - alpha) / beta;
-                sc = 1.0 / (alpha - beta);
-            }
-            if (tid == 0) {
-                if (xn2 != 0.0) s_ntau++;
-                if (!isfinite(tau) || !isfinite(sc)) s_bad = 1;
-                Tl[c + c * QR_NB] = tau;
-                scl[c] = sc;
-            }
-            // reflector v = [1; sc * x]; columns c2 > c owned by this wave
-            for (int c2 = c + 1; c2 < nb; c2++) {
-                if ((c2 % QR_WAVES) != wave) continue;
-                double d = 0.0;
-                if (tau != 0.0) {
-                    for (int r = c + 1 + lane; r < m; r += 64) d += P[c * n + r] * P[c2 * n + r];
-                    d = wave_sum(d);
-                    const double w = tau * (P[c2 * n + c] + sc * d);
-                    const double ws = w * sc;
-                    for (int r = c + 1 + lane; r < m; r += 64) P[c2 * n + r] -= ws * P[c * n + r];
-                    if (lane == 0) P[c2 * n + c] -= w;
-                }
-                if (c2 == c + 1) {
-                    double s = 0.0;
-                    for (int r = c2 + 1 + lane; r < m; r += 64) s += P[c2 * n + r] * P[c2 * n + r];
-                    s = wave_sum(s);
-                    if (lane == 0) nrm[c2] = s;
-                }
-            }
-            __syncthreads();
-            if (tid == 0 && xn2 != 0.0) P[c * n + c] = beta;
-        }
-        __syncthreads();
-        // scale the stored reflectors: v = sc * x below the diagonal
-        for (int idx = tid; idx < nb * m; idx += blockDim.x) {
-            const int c = idx / m, r = idx % m;
-            if (r > c && scl[c] != 0.0) P[c * n + r] *= scl[c];
-        }
-        __syncthreads();
-        // ---- T factor: Y(i,c) = v_i^T v_c for i < c into Wl scratch, then
-        // T(0:c, c) = -tau_c T(0:c, 0:c) Y(0:c, c)
-        double *Y = Wl;
-        for (int pr = wave; pr < nb * nb; pr += QR_WAVES) {
-            const int i = pr % nb, c = pr / nb;
-            if (i >= c) continue;
-            double s = 0.0;
-            for (int r = c + lane; r < m; r += 64) s += panel_v(P, n, i, r) * panel_v(P, n, c, r);
-            s = wave_sum(s);
-            if (lane == 0) Y[i + c * QR_NB] = s;
-        }
-        __syncthreads();
-        if (wave == 0) {
-            for (int c = 1; c < nb; c++) {
-                if (lane < c) {
-                    double s = 0.0;
-                    for (int k = lane; k < c; k++) s += Tl[lane + k * QR_NB] * Y[k + c * QR_NB];
-                    Tl[lane + c * QR_NB] = -Tl[c + c * QR_NB] * s;
-                }
-                __builtin_amdgcn_s_waitcnt(0xc07f);
-                __builtin_amdgcn_wave_barrier();
-            }
-        }
-        __syncthreads();
+        qr_panel<MR>(A, n, p0, nb, m, P, ld, xcol, s_alpha, s_nrm, s_sc, s_beta, Tl, &s_bad, &s_ntau);
+        qr_tfactor(P, ld, nb, m, Tl, Wl);
         for (int idx = tid; idx < QR_NB * QR_NB; idx += blockDim.x) Tw[pi * QR_NB * QR_NB + idx] = Tl[idx];
-        for (int idx = tid; idx < nb * m; idx += blockDim.x) {
-            const int c = idx / m, r = idx % m;
-            A[(p0 + r) + (long long)(p0 + c) * n] = P[c * n + r];
-        }
-        __syncthreads();
-        // ---- trailing update A[p0:, p0+nb:] <- (I - V T^T V^T) A
+        // trailing update A[p0:, p0+nb:] <- (I - V T^T V^T) A
         for (int j0 = p0 + nb + 16 * wave; j0 < n; j0 += 16 * QR_WAVES)
-            wy_apply_block(P, n, Tl, nb, m, A, n, p0, j0, n, 1, Wl + wave * QR_NB * 16);
+            wy_apply_block(P, ld, Tl, nb, m, A, n, p0, j0, n, 1, Wl + wave * QR_NB * 16);
+        __syncthreads();
     }
-    __syncthreads();
 
     // D = diag R, Rn = D^-1 R
     double *Dd = a.D + (long long)b * a.sD;
@@ -306,11 +425,23 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_udv(QrArgs a)
             const int nb = min(QR_NB, n - p0);
             const int m = n - p0;
             __syncthreads();
-            panel_load(P, n, A + p0 + (long long)p0 * n, n, nb, m, nullptr);
+            for (int base = tid; base < nb * m; base += 8 * blockDim.x) {
+                double v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int idx = base + u * blockDim.x;
+                    v[u] = idx < nb * m ? A[(p0 + idx % m) + (long long)(p0 + idx / m) * n] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int idx = base + u * blockDim.x;
+                    if (idx < nb * m) P[(idx / m) * ld + idx % m] = v[u];
+                }
+            }
             for (int idx = tid; idx < QR_NB * QR_NB; idx += blockDim.x) Tl[idx] = Tw[pi * QR_NB * QR_NB + idx];
             __syncthreads();
             for (int j0 = p0 + 16 * wave; j0 < n; j0 += 16 * QR_WAVES)
-                wy_apply_block(P, n, Tl, nb, m, Q, n, p0, j0, n, 0, Wl + wave * QR_NB * 16);
+                wy_apply_block(P, ld, Tl, nb, m, Q, n, p0, j0, n, 0, Wl + wave * QR_NB * 16);
         }
     }
     __syncthreads();
@@ -325,7 +456,8 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_udv(QrArgs a)
 
 static inline size_t qr_lds_bytes(int n)
 {
-    return sizeof(double) * ((size_t)QR_NB * n + QR_NB * QR_NB + QR_WAVES * QR_NB * 16 + 2 * QR_NB);
+    return sizeof(double) * ((size_t)QR_NB * (n + 1) + QR_NB * QR_NB + QR_WAVES * QR_NB * 16 + 2 * (size_t)n +
+                             4 * QR_NB);
 }
 
 // X = T^-1 for unit upper-triangular T (the diagonal of T is not read).
