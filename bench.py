@@ -7,12 +7,15 @@ Ising fields come from field_init's RNG stream, tau=0 measurement with
 Szz/Sperp at all 256 momenta after every timed sweep.
 
 A step is one forward sweep of all 128 replicas plus its measurement.  The
-mixed-precision engine (FP32 wrap / delayed updates / B-chains on MFMA, FP64
-QR, rebuild and measurement) gives `value`; the FP64 engine is timed on the
-same workload for the mixed/FP64 ratio.  Inputs are resident on the GPU
-before the timed region, which is bracketed by HIP events.
+FP64 engine gives `value`: it is the engine whose wrap-vs-rebuild drift
+stays at the reference's own level on this workload.  The two mixed
+FP32/FP64 engines (FP32 wrap / delayed updates / B-chains on MFMA, FP64 QR,
+rebuild and measurement) are timed on the same workload and reported beside
+it with their drift.  Inputs are resident on the GPU before the timed
+region, which is bracketed by HIP events.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                    [--precision all|fp64|mixed|mixed_fp32_chain] [--calibrate-copy]
 """
 import argparse
 import json
@@ -69,7 +72,25 @@ def run_engine(dq, precision, steps, warmup, drift):
     return out
 
 
+KERNEL_OF_STAGE = {"metro": "k_metro_slice", "wrap": "k_gemm (wrap)", "stack": "suffix stack",
+                   "lmul": "left UDV chain", "combine": "udv_combine", "inv_one_plus": "udv_inv_one_plus"}
+
+
+def traffic_from_profile(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy kernel)."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_16x16_beta8_128rep_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch"):
+            return float(line.split()[-1])
+    return None
+
+
 def roofline(prof, acc, mixed):
+    """Roofline of the stage with the largest time per sweep (one kernel
+    launch per invocation for the Metropolis and wrap stages)."""
     n, L, s = LX * LY, int(round(BETA / DTAU)), STAB
     fl = stage_flops(n, L, s, acc)
     best = None
@@ -86,9 +107,12 @@ def roofline(prof, acc, mixed):
     t_roof = lo * NREP / p_lo + hi * NREP / P64
     peak = flops / t_roof
     ach = flops / t
-    return {"bound": "mfma", "kernel": k, "achieved": round(ach / 1e12, 3), "peak": round(peak / 1e12, 3),
-            "unit": "TFLOP/s", "frac": round(ach / peak, 4), "traffic": None,
-            "avg_launch_ms": round(t * 1e3, 4), "launches_per_sweep": cnt}
+    kern = KERNEL_OF_STAGE[k]
+    return {"bound": "mfma", "kernel": kern, "achieved": round(ach / 1e12, 3), "peak": round(peak / 1e12, 3),
+            "unit": "TFLOP/s", "frac": round(ach / peak, 4),
+            "traffic": traffic_from_profile(kern) if (k == "metro" and not mixed) else None,
+            "avg_launch_ms": round(t * 1e3, 4), "launches_per_sweep": cnt,
+            "flops_per_launch": flops}
 
 
 def cpu_baseline():
@@ -119,12 +143,26 @@ def cpu_baseline():
                       f"OMP_NUM_THREADS={cores}, sequential OpenBLAS"}
 
 
+def calibrate_copy():
+    """A plain device copy of known size (256 MiB read, 256 MiB written) for
+    scaling the PMC byte counters."""
+    import torch
+    a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"# calibration copy: {a.numel() * 4} bytes read and written per dispatch", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", default="all", choices=["all", "fp64", "mixed", "mixed_fp32_chain"])
+    ap.add_argument("--calibrate-copy", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: replicas are independent Markov chains and this engine runs on a "
@@ -132,45 +170,49 @@ def main():
         sys.exit(2)
     import dqmc_gpu as dq
 
+    if a.calibrate_copy:
+        calibrate_copy()
+    modes = {"fp64": dq.FP64, "mixed": dq.MIXED, "mixed_fp32_chain": dq.MIXED_FP32_CHAIN}
+    names = list(modes) if a.precision == "all" else [a.precision]
     runs = {}
-    for name, prec in (("mixed", dq.MIXED), ("fp64", dq.FP64), ("mixed_fp32_chain", dq.MIXED_FP32_CHAIN)):
-        runs[name] = run_engine(dq, prec, a.steps, a.warmup, drift=True)
+    for name in names:
+        runs[name] = run_engine(dq, modes[name], a.steps, a.warmup, drift=True)
         r = runs[name]
         print(f"# {name}: {r['value']:.1f} replica-sweeps/s, {r['ms'] / a.steps:.2f} ms/sweep, "
               f"drift max {r['drift_max']:.3e} mean {r['drift_mean']:.3e}", file=sys.stderr, flush=True)
-    mixed, fp64 = runs["mixed"], runs["fp64"]
+    main_name = names[0]
+    main_run = runs[main_name]
     cpu = None if a.no_cpu else cpu_baseline()
     line = {
         "metric": METRIC,
-        "value": round(mixed["value"], 2),
+        "value": round(main_run["value"], 2),
         "unit": "replica-sweeps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(mixed["ms"] / a.steps, 3),
+        "ms_per_step": round(main_run["ms"] / a.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32+fp64",
+        "dtype": "f64" if main_name == "fp64" else "f32+f64",
         "data": "synthetic: Ising fields drawn from seeds 1..128 by field_init's RNG stream",
-        "config": {"workload": "16x16 periodic square, t=-1, U=4, dtau=0.1, beta=8 (80 slices), stab=8, "
+        "config": {"workload": "16x16 periodic square, t=-1, U=4, mu=U/2, dtau=0.1, beta=8 (80 slices), stab=8, "
                                "delay depth 32, 128 replicas, Szz/Sperp at all 256 q",
-                   "replicas": NREP, "precision": "mixed FP32/FP64"},
-        "fp64": {"value": round(fp64["value"], 2), "ms_per_step": round(fp64["ms"] / a.steps, 3),
-                 "drift_max": fp64["drift_max"], "drift_mean": fp64["drift_mean"]},
-        "mixed_over_fp64": round(mixed["value"] / fp64["value"], 3),
-        "mixed_fp32_chain": {"value": round(runs["mixed_fp32_chain"]["value"], 2),
-                             "drift_max": runs["mixed_fp32_chain"]["drift_max"],
-                             "drift_mean": runs["mixed_fp32_chain"]["drift_mean"]},
-        "drift_max": mixed["drift_max"],
-        "drift_mean": mixed["drift_mean"],
-        "acceptance": round(mixed["acc"], 4),
-        "roofline": roofline(mixed["profile"], mixed["acc"], True),
-        "roofline_fp64": roofline(fp64["profile"], fp64["acc"], False),
-        "profile_ms_per_sweep": {k: round(v[0], 3) for k, v in mixed["profile"].items()},
-        "profile_fp64_ms_per_sweep": {k: round(v[0], 3) for k, v in fp64["profile"].items()},
-        "cpu_baseline": cpu,
+                   "replicas": NREP, "precision": main_name},
+        "drift_max": main_run["drift_max"],
+        "drift_mean": main_run["drift_mean"],
+        "acceptance": round(main_run["acc"], 4),
+        "roofline": roofline(main_run["profile"], main_run["acc"], main_name != "fp64"),
+        "profile_ms_per_sweep": {k: round(v[0], 3) for k, v in main_run["profile"].items()},
     }
+    for name in names[1:]:
+        r = runs[name]
+        line[name] = {"value": round(r["value"], 2), "ms_per_step": round(r["ms"] / a.steps, 3),
+                      "over_fp64": round(r["value"] / main_run["value"], 3),
+                      "drift_max": r["drift_max"], "drift_mean": r["drift_mean"],
+                      "roofline": roofline(r["profile"], r["acc"], True),
+                      "profile_ms_per_sweep": {k: round(v[0], 3) for k, v in r["profile"].items()}}
+    line["cpu_baseline"] = cpu
     print(json.dumps(line))
 
 

@@ -615,7 +615,9 @@ static bool chain_f32(const dqmc_gpu_ctx *ctx) { return ctx->opts.precision == D
 
 template <typename T, int W> static void launch_metro(const MetroArgs<T> &a, hipStream_t s)
 {
-    const int threads = std::min(METRO_THREADS, std::max(64, ((a.n + 63) / 64) * 64));
+    // at least 512 threads: sites beyond n idle in the proposals but add
+    // loads in flight to the rank-p flushes
+    const int threads = std::min(METRO_THREADS, std::max(512, ((a.n + 63) / 64) * 64));
     hipLaunchKernelGGL((k_metro_slice<T, W>), dim3(a.batch), dim3(threads), metro_lds_bytes<T>(a.n, W), s, a);
 }
 

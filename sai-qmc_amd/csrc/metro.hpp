@@ -51,6 +51,10 @@ static inline size_t metro_lds_bytes(int n, int W)
 }
 
 // G[n x n] -= Cc[0:p]^T Vv[0:p] with Cc, Vv stored q-major (Cc[q*n + j]).
+// Each MFMA tile holds a 16x16 block of G^T (output row = G column, output
+// column = G row), so the 16 lanes of an accumulator row cover 16
+// consecutive rows of one column of G: every load and store instruction
+// touches four 128-byte segments.  B tiles per wave are kept in flight.
 template <typename T>
 __device__ void flush_rank_p(T *G, int n, const T *Cc, const T *Vv, int p)
 {
@@ -59,17 +63,17 @@ __device__ void flush_rank_p(T *G, int n, const T *Cc, const T *Vv, int p)
     const int lc = lane & 15, lr = lane >> 4;
     const int nt = (n + 15) >> 4;
     const int ntiles = nt * nt;
-    constexpr int B = 8; // tiles in flight per wave
+    constexpr int B = sizeof(T) == 8 ? 8 : 16;
     for (int t0 = wave * B; t0 < ntiles; t0 += nw * B) {
         acc_t acc[B];
 #pragma unroll
         for (int u = 0; u < B; u++) {
             const int t = t0 + u;
             const int r0 = (t % nt) * 16, c0 = (t / nt) * 16;
-            const int col = c0 + lc;
+            const int row = r0 + lc;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int row = r0 + Mfma<T>::row(lane, v);
+                const int col = c0 + Mfma<T>::row(lane, v);
                 acc[u][v] = (t < ntiles && row < n && col < n) ? G[row + (long long)col * n] : T(0);
             }
         }
@@ -79,8 +83,8 @@ __device__ void flush_rank_p(T *G, int n, const T *Cc, const T *Vv, int p)
             for (int u = 0; u < B; u++) {
                 const int t = t0 + u;
                 const int r0 = (t % nt) * 16, c0 = (t / nt) * 16;
-                const T a = (q < p && r0 + lc < n) ? -Cc[q * n + r0 + lc] : T(0);
-                const T b = (q < p && c0 + lc < n) ? Vv[q * n + c0 + lc] : T(0);
+                const T a = (q < p && c0 + lc < n) ? -Vv[q * n + c0 + lc] : T(0);
+                const T b = (q < p && r0 + lc < n) ? Cc[q * n + r0 + lc] : T(0);
                 acc[u] = Mfma<T>::mma(a, b, acc[u]);
             }
         }
@@ -88,10 +92,10 @@ __device__ void flush_rank_p(T *G, int n, const T *Cc, const T *Vv, int p)
         for (int u = 0; u < B; u++) {
             const int t = t0 + u;
             const int r0 = (t % nt) * 16, c0 = (t / nt) * 16;
-            const int col = c0 + lc;
+            const int row = r0 + lc;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int row = r0 + Mfma<T>::row(lane, v);
+                const int col = c0 + Mfma<T>::row(lane, v);
                 if (t < ntiles && row < n && col < n) G[row + (long long)col * n] = acc[u][v];
             }
         }
