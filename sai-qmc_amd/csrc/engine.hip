@@ -891,7 +891,11 @@ static int set_lds_limits()
                              (const void *)k_metro_slice<double, 8>,  (const void *)k_metro_slice<float, 32>,
                              (const void *)k_metro_slice<float, 16>,  (const void *)k_metro_slice<float, 8>,
                              (const void *)k_qr_udv<1>, (const void *)k_qr_udv<2>, (const void *)k_qr_udv<4>, (const void *)k_qr_udv<8>,
-                             (const void *)k_trtri_unit, (const void *)k_measure};
+                             (const void *)k_trtri_unit, (const void *)k_measure,
+                             (const void *)k_gemm_big<double, 0, 0>, (const void *)k_gemm_big<double, 1, 0>,
+                             (const void *)k_gemm_big<double, 0, 1>, (const void *)k_gemm_big<double, 1, 1>,
+                             (const void *)k_gemm_big<float, 0, 0>, (const void *)k_gemm_big<float, 1, 0>,
+                             (const void *)k_gemm_big<float, 0, 1>, (const void *)k_gemm_big<float, 1, 1>};
     for (const void *k : kernels) {
         hipFuncAttributes at;
         HIPCHK(hipFuncGetAttributes(&at, k));

@@ -127,6 +127,200 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs<T> g)
             }
 }
 
+// Large-tile variant for M, N >= 128: 128x128x16 workgroup tile, each of
+// the 4 waves a 64x64 block of 4x4 MFMA tiles.  Tiles are computed as C^T
+// (the MFMA A operand is op(B), the B operand op(A)), so the 16 lanes of an
+// accumulator row hold 16 consecutive rows of one column of C and the
+// epilogue stores 128-byte segments.  The next k-tile is loaded into
+// registers (two elements per load) while the current one is multiplied
+// out of the other half of a double-buffered LDS; rows of the k-major LDS
+// tiles are padded so a 4-row fragment read covers all 64 banks once.
+#define GB_M 128
+#define GB_K 16
+#define GB_LD (GB_M + 16)
+
+template <typename T>
+static inline size_t gemm_big_lds_bytes()
+{
+    return sizeof(T) * 2 * 2 * GB_K * GB_LD;
+}
+
+template <typename T> struct Two { T x, y; };
+
+template <typename T>
+__device__ __forceinline__ Two<T> ld_two(const T *p, bool ok0, bool ok1)
+{
+    Two<T> r;
+    if (ok0 && ok1 && ((reinterpret_cast<uintptr_t>(p) & (2 * sizeof(T) - 1)) == 0)) {
+        if constexpr (sizeof(T) == 8) {
+            const double2 v = *reinterpret_cast<const double2 *>(p);
+            r.x = v.x; r.y = v.y;
+        } else {
+            const float2 v = *reinterpret_cast<const float2 *>(p);
+            r.x = v.x; r.y = v.y;
+        }
+    } else {
+        r.x = ok0 ? p[0] : T(0);
+        r.y = ok1 ? p[1] : T(0);
+    }
+    return r;
+}
+
+template <typename T, int TA, int TB>
+__global__ __launch_bounds__(256, 2) void k_gemm_big(GemmArgs<T> g)
+{
+    const int tiles_m = (g.M + GB_M - 1) / GB_M;
+    const int tiles_n = (g.N + GB_M - 1) / GB_M;
+    const int per = tiles_m * tiles_n;
+    const int logical = xcd_remap(blockIdx.x, gridDim.x);
+    const int b = logical / per;
+    const int t = logical % per;
+    const int m0 = (t % tiles_m) * GB_M;
+    const int n0 = (t / tiles_m) * GB_M;
+    if (b >= g.batch) return;
+    if (g.status && g.status[b]) return;
+
+    extern __shared__ __align__(16) unsigned char gb_raw[];
+    T *lds = reinterpret_cast<T *>(gb_raw);
+    // buffer u: As = lds + u * 2 * GB_K * GB_LD, Bs = As + GB_K * GB_LD
+    const T *A = g.A + (long long)b * g.sA;
+    const T *B = g.B + (long long)b * g.sB;
+    T *C = g.C + (long long)b * g.sC;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lc = lane & 15, lr = lane >> 4;
+    const int wr = wave >> 1, wc = wave & 1;
+
+    // fixed per-thread load coordinates (4 pairs per operand per k-tile)
+    //   A, not transposed: pair (2*i2, 2*i2+1) x k, i2 = tid % 64, k = tid / 64 + 4 r
+    //   A, transposed:     pair i x (2*k2, 2*k2+1), k2 = tid % 8, i = tid / 8 + 32 r
+    //   B, not transposed: pair (2*k2, 2*k2+1) x j, k2 = tid % 8, j = tid / 8 + 32 r
+    //   B, transposed:     pair k x (2*j2, 2*j2+1), j2 = tid % 64, k = tid / 64 + 4 r
+    Two<T> ra[4], rb[4];
+    T sa_fixed[2] = {T(1), T(1)}, sb_fixed[2] = {T(1), T(1)};
+    if (!TA) {
+        const int i = m0 + 2 * (tid & 63);
+        sa_fixed[0] = (i < g.M) ? g.ar.at(b, i) : T(0);
+        sa_fixed[1] = (i + 1 < g.M) ? g.ar.at(b, i + 1) : T(0);
+    }
+    if (TB) {
+        const int j = n0 + 2 * (tid & 63);
+        sb_fixed[0] = (j < g.N) ? g.bc.at(b, j) : T(0);
+        sb_fixed[1] = (j + 1 < g.N) ? g.bc.at(b, j + 1) : T(0);
+    }
+
+    auto load_tile = [&](int k0) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            if (!TA) {
+                const int i = m0 + 2 * (tid & 63), k = k0 + (tid >> 6) + 4 * r;
+                const bool kok = k < g.K;
+                ra[r] = ld_two(A + i + (long long)k * g.lda, kok && i < g.M, kok && i + 1 < g.M);
+                const T s = kok ? g.ac.at(b, k) : T(0);
+                ra[r].x *= sa_fixed[0] * s;
+                ra[r].y *= sa_fixed[1] * s;
+            } else {
+                const int i = m0 + (tid >> 3) + 32 * r, k = k0 + 2 * (tid & 7);
+                const bool iok = i < g.M;
+                ra[r] = ld_two(A + k + (long long)i * g.lda, iok && k < g.K, iok && k + 1 < g.K);
+                const T s = iok ? g.ar.at(b, i) : T(0);
+                ra[r].x *= s * (k < g.K ? g.ac.at(b, k) : T(0));
+                ra[r].y *= s * (k + 1 < g.K ? g.ac.at(b, k + 1) : T(0));
+            }
+            if (!TB) {
+                const int j = n0 + (tid >> 3) + 32 * r, k = k0 + 2 * (tid & 7);
+                const bool jok = j < g.N;
+                rb[r] = ld_two(B + k + (long long)j * g.ldb, jok && k < g.K, jok && k + 1 < g.K);
+                const T s = jok ? g.bc.at(b, j) : T(0);
+                rb[r].x *= (k < g.K ? g.br.at(b, k) : T(0)) * s;
+                rb[r].y *= (k + 1 < g.K ? g.br.at(b, k + 1) : T(0)) * s;
+            } else {
+                const int j = n0 + 2 * (tid & 63), k = k0 + (tid >> 6) + 4 * r;
+                const bool kok = k < g.K;
+                rb[r] = ld_two(B + j + (long long)k * g.ldb, kok && j < g.N, kok && j + 1 < g.N);
+                const T s = kok ? g.br.at(b, k) : T(0);
+                rb[r].x *= s * sb_fixed[0];
+                rb[r].y *= s * sb_fixed[1];
+            }
+        }
+    };
+    auto store_tile = [&](int u) {
+        T *As = lds + u * 2 * GB_K * GB_LD;
+        T *Bs = As + GB_K * GB_LD;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            if (!TA) {
+                T *d = As + ((tid >> 6) + 4 * r) * GB_LD + 2 * (tid & 63);
+                d[0] = ra[r].x; d[1] = ra[r].y;
+            } else {
+                const int i = (tid >> 3) + 32 * r, k = 2 * (tid & 7);
+                As[k * GB_LD + i] = ra[r].x;
+                As[(k + 1) * GB_LD + i] = ra[r].y;
+            }
+            if (!TB) {
+                const int j = (tid >> 3) + 32 * r, k = 2 * (tid & 7);
+                Bs[k * GB_LD + j] = rb[r].x;
+                Bs[(k + 1) * GB_LD + j] = rb[r].y;
+            } else {
+                T *d = Bs + ((tid >> 6) + 4 * r) * GB_LD + 2 * (tid & 63);
+                d[0] = rb[r].x; d[1] = rb[r].y;
+            }
+        }
+    };
+
+    typedef typename Mfma<T>::acc_t acc_t;
+    acc_t acc[4][4];
+#pragma unroll
+    for (int it = 0; it < 4; it++)
+#pragma unroll
+        for (int jt = 0; jt < 4; jt++) acc[it][jt] = acc_t{0, 0, 0, 0};
+
+    load_tile(0);
+    store_tile(0);
+    __syncthreads();
+    int u = 0;
+    for (int k0 = 0; k0 < g.K; k0 += GB_K) {
+        const bool more = k0 + GB_K < g.K;
+        if (more) load_tile(k0 + GB_K);
+        const T *As = lds + u * 2 * GB_K * GB_LD;
+        const T *Bs = As + GB_K * GB_LD;
+#pragma unroll
+        for (int kk = 0; kk < GB_K; kk += 4) {
+            T fa[4], fb[4];
+#pragma unroll
+            for (int jt = 0; jt < 4; jt++) fa[jt] = Bs[(kk + lr) * GB_LD + wc * 64 + jt * 16 + lc];
+#pragma unroll
+            for (int it = 0; it < 4; it++) fb[it] = As[(kk + lr) * GB_LD + wr * 64 + it * 16 + lc];
+#pragma unroll
+            for (int it = 0; it < 4; it++)
+#pragma unroll
+                for (int jt = 0; jt < 4; jt++) acc[it][jt] = Mfma<T>::mma(fa[jt], fb[it], acc[it][jt]);
+        }
+        if (more) store_tile(u ^ 1);
+        __syncthreads();
+        u ^= 1;
+    }
+
+#pragma unroll
+    for (int it = 0; it < 4; it++) {
+        const int i = m0 + wr * 64 + it * 16 + lc;
+        const bool iok = i < g.M;
+        const T sr = iok ? g.cr.at(b, i) : T(0);
+#pragma unroll
+        for (int jt = 0; jt < 4; jt++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int j = n0 + wc * 64 + jt * 16 + Mfma<T>::row(lane, v);
+                if (iok && j < g.N) {
+                    T r = g.alpha * acc[it][jt][v] * sr * g.cc.at(b, j);
+                    if (g.dg.p && i == j) r += g.dg.at(b, i);
+                    T *cp = &C[i + (long long)j * g.ldc];
+                    if (g.beta != T(0)) r += g.beta * *cp;
+                    *cp = r;
+                }
+            }
+    }
+}
+
 template <typename T>
 static inline GemmArgs<T> gemm_args(int M, int N, int K, int batch, const T *A, long long lda,
                                     long long sA, int transA, const T *B, long long ldb,
@@ -146,6 +340,16 @@ static inline GemmArgs<T> gemm_args(int M, int N, int K, int batch, const T *A, 
 template <typename T>
 static inline void gemm_launch(const GemmArgs<T> &g, hipStream_t s)
 {
+    if (g.M >= GB_M && g.N >= GB_M) {
+        const int tb = ((g.M + GB_M - 1) / GB_M) * ((g.N + GB_M - 1) / GB_M) * g.batch;
+        if (tb <= 0) return;
+        const size_t lds = gemm_big_lds_bytes<T>();
+        if (!g.transA && !g.transB) hipLaunchKernelGGL((k_gemm_big<T, 0, 0>), dim3(tb), dim3(256), lds, s, g);
+        else if (g.transA && !g.transB) hipLaunchKernelGGL((k_gemm_big<T, 1, 0>), dim3(tb), dim3(256), lds, s, g);
+        else if (!g.transA && g.transB) hipLaunchKernelGGL((k_gemm_big<T, 0, 1>), dim3(tb), dim3(256), lds, s, g);
+        else hipLaunchKernelGGL((k_gemm_big<T, 1, 1>), dim3(tb), dim3(256), lds, s, g);
+        return;
+    }
     const int tiles = ((g.M + GEMM_BM - 1) / GEMM_BM) * ((g.N + GEMM_BN - 1) / GEMM_BN);
     const int total = tiles * g.batch;
     if (total <= 0) return;

@@ -25,7 +25,7 @@
 #include "common.hpp"
 
 #define QR_NB 32
-#define QR_THREADS 256
+#define QR_THREADS 512
 #define QR_WAVES (QR_THREADS / 64)
 
 struct QrArgs {
@@ -319,48 +319,68 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
 // T of the compact WY form Q = I - V T V^T of the panel.  For Householder
 // reflectors tau_i = 2 / v_i^T v_i, which gives T^-1 = diag(1/tau) +
 // triu(V^T V, 1) (reflectors with tau = 0 are the identity: their row and
-// column of T vanish).  V^T V on MFMA (one 16x16 tile per wave), then lane c
-// of wave 0 solves for column c of T.
+// column of T vanish).  V^T V on MFMA: waves w and w+4 take one 16x16 tile
+// over the two halves of the rows, each with two interleaved accumulators;
+// then lane c of wave 0 solves T^-1 t = e_c for column c of T in column
+// order, so the dependent chain is 32 steps long.
 __device__ void qr_tfactor(const double *P, int ld, int nb, int m, double *Tl, double *Y)
 {
+    static_assert(QR_WAVES == 8, "two waves per Y tile");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lc = lane & 15, lr = lane >> 4;
-    if (wave < 4) {
-        const int ti = wave >> 1, tj = wave & 1;
-        f64x4 acc = {0, 0, 0, 0};
-        for (int k0 = 0; k0 < m; k0 += 4) {
-            const int k = k0 + lr;
-            const int ci = ti * 16 + lc, cj = tj * 16 + lc;
+    double *Y2 = Y + QR_NB * QR_NB;     // partial sums of the second half
+    double *Ts = Y2 + QR_NB * QR_NB;    // columns of T
+    {
+        const int tile = wave & 3, half = wave >> 2;
+        const int ti = tile >> 1, tj = tile & 1;
+        const int ci = ti * 16 + lc, cj = tj * 16 + lc;
+        const int steps = (m + 3) >> 2, hs = (steps + 1) >> 1;
+        const int s0 = half ? hs : 0, s1 = half ? steps : hs;
+        f64x4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+        for (int st = s0; st < s1; st += 2) {
+            const int k = 4 * st + lr, k2 = k + 4;
             const double va = (ci < nb && k < m) ? panel_v(P, ld, ci, k) : 0.0;
             const double vb = (cj < nb && k < m) ? panel_v(P, ld, cj, k) : 0.0;
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(va, vb, acc, 0, 0, 0);
+            const bool two = st + 1 < s1;
+            const double va2 = (two && ci < nb && k2 < m) ? panel_v(P, ld, ci, k2) : 0.0;
+            const double vb2 = (two && cj < nb && k2 < m) ? panel_v(P, ld, cj, k2) : 0.0;
+            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(va, vb, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(va2, vb2, acc1, 0, 0, 0);
         }
+        double *dst = half ? Y2 : Y;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int i = ti * 16 + Mfma<double>::row(lane, v), c = tj * 16 + lc;
-            Y[i + c * QR_NB] = acc[v];
+            dst[i + c * QR_NB] = acc0[v] + acc1[v];
         }
     }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < QR_NB * QR_NB; idx += blockDim.x) Y[idx] += Y2[idx];
     __syncthreads();
     if (wave == 0 && lane < QR_NB) {
-        // lane c: back substitution for column c of T, kept in LDS (own column)
+        // lane c: back substitution for column c of T in registers, fully
+        // unrolled (t[k] = 0 for k > c keeps the sums exact); the Y and tau
+        // reads are lane-uniform and independent of the chain
         const int c = lane;
-        double *t = Y + QR_NB * QR_NB + c * QR_NB;
-        const double tc = Tl[c + c * QR_NB];
+        const bool live = c < nb && Tl[c + c * QR_NB] != 0.0;
+        double t[QR_NB];
+#pragma unroll
         for (int i = QR_NB - 1; i >= 0; i--) {
-            const double ti = Tl[i + i * QR_NB];
             double s = (i == c) ? 1.0 : 0.0;
-            for (int k = i + 1; k <= c; k++) s -= Y[i + k * QR_NB] * t[k];
-            t[i] = (i <= c && c < nb && tc != 0.0) ? s * ti : 0.0;
+#pragma unroll
+            for (int k = i + 1; k < QR_NB; k++) s -= Y[i + k * QR_NB] * t[k];
+            t[i] = (i <= c && live) ? s * Tl[i + i * QR_NB] : 0.0;
         }
+#pragma unroll
+        for (int i = 0; i < QR_NB; i++) Ts[i + c * QR_NB] = t[i];
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < QR_NB * QR_NB; idx += blockDim.x) Tl[idx] = Y[idx + QR_NB * QR_NB];
+    for (int idx = threadIdx.x; idx < QR_NB * QR_NB; idx += blockDim.x) Tl[idx] = Ts[idx];
     __syncthreads();
 }
 
 template <int MR>
-__global__ __launch_bounds__(QR_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_qr_udv(QrArgs a)
+__global__ __launch_bounds__(QR_THREADS) void k_qr_udv(QrArgs a)
 {
     const int b = blockIdx.x;
     if (b >= a.batch) return;
