@@ -343,7 +343,11 @@ static int qr(Work &w, double *A, double *Q, long long sQ, double *Rn, long long
         hipLaunchKernelGGL(k_qr_udv<4>, dim3(w.nb), dim3(QR_THREADS), qr_lds_bytes(w.n), w.s, a);
     else
         hipLaunchKernelGGL(k_qr_udv<8>, dim3(w.nb), dim3(QR_THREADS), qr_lds_bytes(w.n), w.s, a);
-    return launch_check("k_qr_udv");
+    if (launch_check("k_qr_udv")) return 1;
+    if (!Q) return 0;
+    hipLaunchKernelGGL(k_qr_formq, dim3(qr_formq_blocks(w.n, w.nb)), dim3(FQ_WAVES * 64), qr_formq_lds_bytes(w.n),
+                       w.s, a);
+    return launch_check("k_qr_formq");
 }
 
 static int trtri(Work &w, const double *T, long long sT, double *X)
@@ -891,7 +895,7 @@ static int set_lds_limits()
                              (const void *)k_metro_slice<double, 8>,  (const void *)k_metro_slice<float, 32>,
                              (const void *)k_metro_slice<float, 16>,  (const void *)k_metro_slice<float, 8>,
                              (const void *)k_qr_udv<1>, (const void *)k_qr_udv<2>, (const void *)k_qr_udv<4>, (const void *)k_qr_udv<8>,
-                             (const void *)k_trtri_unit, (const void *)k_measure,
+                             (const void *)k_trtri_unit, (const void *)k_measure, (const void *)k_qr_formq,
                              (const void *)k_gemm_big<double, 0, 0>, (const void *)k_gemm_big<double, 1, 0>,
                              (const void *)k_gemm_big<double, 0, 1>, (const void *)k_gemm_big<double, 1, 1>,
                              (const void *)k_gemm_big<float, 0, 0>, (const void *)k_gemm_big<float, 1, 0>,

@@ -18,6 +18,8 @@
 //     MFMA 16x16x4: each wave takes 16-column blocks, W = V^T A (2 row tiles,
 //     K = panel height), W' = op(T) W, A -= V W'.
 //
+// k_qr_formq: explicit Q from the reflectors, one wave per 16-column block.
+//
 // k_trtri_unit: inverse of a unit upper-triangular matrix
 // (dtrtri('U','U'), src/linalg.c:992-1004), one wave per 32-column block
 // column, back substitution over 32-row blocks with MFMA products.
@@ -434,36 +436,6 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_udv(QrArgs a)
             Rn[idx] = (i <= j) ? A[idx] / A[i + (long long)i * n] : 0.0;
         }
     }
-    if (a.Q) {
-        double *Q = a.Q + (long long)b * a.sQ;
-        for (long long idx = tid; idx < (long long)n * n; idx += blockDim.x) {
-            const int i = (int)(idx % n), j = (int)(idx / n);
-            Q[idx] = (i == j) ? 1.0 : 0.0;
-        }
-        for (int pi = npan - 1; pi >= 0; pi--) {
-            const int p0 = pi * QR_NB;
-            const int nb = min(QR_NB, n - p0);
-            const int m = n - p0;
-            __syncthreads();
-            for (int base = tid; base < nb * m; base += 8 * blockDim.x) {
-                double v[8];
-#pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    const int idx = base + u * blockDim.x;
-                    v[u] = idx < nb * m ? A[(p0 + idx % m) + (long long)(p0 + idx / m) * n] : 0.0;
-                }
-#pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    const int idx = base + u * blockDim.x;
-                    if (idx < nb * m) P[(idx / m) * ld + idx % m] = v[u];
-                }
-            }
-            for (int idx = tid; idx < QR_NB * QR_NB; idx += blockDim.x) Tl[idx] = Tw[pi * QR_NB * QR_NB + idx];
-            __syncthreads();
-            for (int j0 = p0 + 16 * wave; j0 < n; j0 += 16 * QR_WAVES)
-                wy_apply_block(P, ld, Tl, nb, m, Q, n, p0, j0, n, 0, Wl + wave * QR_NB * 16);
-        }
-    }
     __syncthreads();
     if (tid == 0) {
         if (a.qsign) a.qsign[(long long)b * a.sq] = (s_ntau & 1) ? -1 : 1;
@@ -478,6 +450,78 @@ static inline size_t qr_lds_bytes(int n)
 {
     return sizeof(double) * ((size_t)QR_NB * (n + 1) + QR_NB * QR_NB + QR_WAVES * QR_NB * 16 + 2 * (size_t)n +
                              4 * QR_NB);
+}
+
+// Explicit Q = H_0 H_1 ... H_{npan-1} (dorgqr) from the reflectors and T
+// factors left by k_qr_udv.  Column blocks of Q are independent: a block
+// starts as columns of I and receives the block reflectors from the last
+// panel back to the first, so each wave owns one 16-column block and the
+// launch has n/16 waves per matrix (all CUs busy for a batch of 128).
+// Columns j < p0 of panel p are still e_j below row p0 and are skipped.
+#define FQ_WAVES 8
+
+__global__ __launch_bounds__(FQ_WAVES * 64) void k_qr_formq(QrArgs a)
+{
+    const int n = a.n;
+    const int nblk = (n + 15) / 16;
+    const int wg_per = (nblk + FQ_WAVES - 1) / FQ_WAVES;
+    const int b = blockIdx.x / wg_per, g = blockIdx.x % wg_per;
+    if (b >= a.batch) return;
+    if (a.status && a.status[b]) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const double *A = a.A + (long long)b * a.sA;
+    const double *Tw = a.Tws + (long long)b * a.sT;
+    double *Q = a.Q + (long long)b * a.sQ;
+    extern __shared__ double lds[];
+    const int ld = n + 1;
+    double *P = lds;
+    double *Tl = P + (size_t)QR_NB * ld;
+    double *Wl = Tl + QR_NB * QR_NB;
+    const int jb = g * FQ_WAVES + wave;
+    const int j0 = 16 * jb;
+    const int jlast = min(n, 16 * (g + 1) * FQ_WAVES) - 1; // last column of this workgroup
+    if (jb < nblk) {
+        for (int idx = lane; idx < n * 16; idx += 64) {
+            const int r = idx % n, c = j0 + idx / n;
+            if (c < n) Q[r + (long long)c * n] = (r == c) ? 1.0 : 0.0;
+        }
+    }
+    const int npan = (n + QR_NB - 1) / QR_NB;
+    for (int pi = npan - 1; pi >= 0; pi--) {
+        const int p0 = pi * QR_NB;
+        if (jlast < p0) continue; // uniform over the workgroup
+        const int nb = min(QR_NB, n - p0);
+        const int m = n - p0;
+        __syncthreads();
+        for (int base = tid; base < nb * m; base += 8 * blockDim.x) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int idx = base + u * blockDim.x;
+                v[u] = idx < nb * m ? A[(p0 + idx % m) + (long long)(p0 + idx / m) * n] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int idx = base + u * blockDim.x;
+                if (idx < nb * m) P[(idx / m) * ld + idx % m] = v[u];
+            }
+        }
+        for (int idx = tid; idx < QR_NB * QR_NB; idx += blockDim.x) Tl[idx] = Tw[pi * QR_NB * QR_NB + idx];
+        __syncthreads();
+        if (jb < nblk && j0 + 15 >= p0)
+            wy_apply_block(P, ld, Tl, nb, m, Q, n, p0, j0, n, 0, Wl + wave * QR_NB * 16);
+    }
+}
+
+static inline size_t qr_formq_lds_bytes(int n)
+{
+    return sizeof(double) * ((size_t)QR_NB * (n + 1) + QR_NB * QR_NB + FQ_WAVES * QR_NB * 16);
+}
+
+static inline int qr_formq_blocks(int n, int batch)
+{
+    const int nblk = (n + 15) / 16;
+    return ((nblk + FQ_WAVES - 1) / FQ_WAVES) * batch;
 }
 
 // X = T^-1 for unit upper-triangular T (the diagonal of T is not read).
