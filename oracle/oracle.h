@@ -10,7 +10,7 @@
  * dense linear algebra (no LAPACK).  Each function cites the reference
  * file:line it restates.  How each function is pinned (golden fixtures,
  * the compiled reference, closed forms, invariants) is listed in
- * tests/test_oracle_*.py; every function below has at least one such check.
+ * tests/test_oracle.py; every function below has at least one such check.
  */
 #ifndef SAIQMC_ORACLE_H
 #define SAIQMC_ORACLE_H

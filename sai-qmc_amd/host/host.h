@@ -28,7 +28,7 @@ typedef struct {
     char stab_drift_file[256];
     unsigned long long seed;
     /* engine keys without a counterpart in the reference */
-    char precision[16]; /* fp64 | mixed */
+    char precision[24]; /* fp64 | mixed | mixed_fp32_chain */
     int delay;          /* delayed-update depth, 0 = engine default */
     int device;
 } Params;
@@ -47,9 +47,9 @@ typedef struct {
 int lattice_build(Lattice *L, const Params *p);
 void lattice_release(Lattice *L);
 
-/* exp(scale * A) of a symmetric matrix through a cyclic Jacobi
-   eigendecomposition (the reference uses LAPACK dsyev in la_expm_sym,
-   src/linalg.c:319-346). */
+/* exp(scale * A) of a symmetric matrix by Taylor series with scaling and
+   squaring (the reference uses LAPACK dsyev in la_expm_sym,
+   src/linalg.c:319-346). Returns 0 on success. */
 int expm_sym(int n, const double *A, double scale, double *out);
 
 /* replica_seed (src/replica.c:17-34). */

@@ -8,7 +8,7 @@
  * (sweep_order=alternating, green_rebuild=two_sided|centered, lattice=file,
  * profile=1, the UDV diagnostics files, replica_log,
  * spin_consistency_file) are accepted only at their default values.
- * Additional keys: precision=fp64|mixed, delay=<k>, device=<id>.
+ * Additional keys: precision=fp64|mixed|mixed_fp32_chain, delay=<k>, device=<id>.
  */
 #include "host.h"
 
@@ -257,7 +257,8 @@ int params_read(Params *p, const char *path)
     else if (p->nbin < 2) err = "nbin must be >= 2";
     else if (p->nmeas < p->nbin) err = "nmeas must be >= nbin";
     else if (p->nmeas % p->nbin != 0) err = "nmeas must be divisible by nbin";
-    else if (strcmp(p->precision, "fp64") && strcmp(p->precision, "mixed")) err = "precision must be fp64 or mixed";
+    else if (strcmp(p->precision, "fp64") && strcmp(p->precision, "mixed") && strcmp(p->precision, "mixed_fp32_chain"))
+        err = "precision must be fp64, mixed or mixed_fp32_chain";
     else if (p->delay < 0) err = "delay must be >= 0";
     if (err) {
         fprintf(stderr, "ERROR: %s\n", err);

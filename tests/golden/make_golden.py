@@ -22,6 +22,7 @@ Fixtures (all numpy .npz, no pickles):
   ed_l4.npz         exact-diagonalisation rows of the 4-site chain at U=4
                     (data/benchmark_L468_U4_full_diag_20260626/reference/).
   c1_observables.txt, c1_szz.txt, c1_sperp.txt
+  c2_observables.txt, c2_szz.txt, c2_sperp.txt (8x8, 64 replicas)
                     the reference's output files for the 4-site chain run
                     (nwarm=300 nmeas=3000 nbin=30 seed=1, szz_q=all, sperp_q=all).
 """
@@ -189,6 +190,22 @@ def c1_outputs():
                 fo.write(fi.read())
 
 
+def c2_outputs():
+    """8x8, beta=4, stab=10, 64 replicas with seeds replica_seed(1, 0, r):
+    the reference's OpenMP driver on the host cores."""
+    with tempfile.TemporaryDirectory() as d:
+        inp = os.path.join(d, "c2.in")
+        with open(inp, "w") as f:
+            f.write("lattice=square\nLx=8\nLy=8\nU=4\ndtau=0.1\nbeta_list=4\nstab=10\nnwarm=200\n"
+                    "nmeas=2000\nnbin=20\nseed=1\nparallel=omp\nnrep=64\nszz_q=all\nsperp_q=all\n")
+        subprocess.run([pyoracle.REF_OMP, inp], cwd=d, check=True, stdout=subprocess.DEVNULL,
+                       env=dict(os.environ, OMP_NUM_THREADS=str(os.cpu_count() or 1), OPENBLAS_NUM_THREADS="1"))
+        for src, dst in (("observables.dat", "c2_observables.txt"), ("szz.dat", "c2_szz.txt"),
+                         ("sperp.dat", "c2_sperp.txt")):
+            with open(os.path.join(d, src)) as fi, open(os.path.join(HERE, dst), "w") as fo:
+                fo.write(fi.read())
+
+
 if __name__ == "__main__":
     if not pyoracle.ref_available():
         sys.exit("oracle/_ref not built: make -C oracle ref")
@@ -200,4 +217,5 @@ if __name__ == "__main__":
     ref_fixtures()
     ed_l4()
     c1_outputs()
+    c2_outputs()
     print("golden fixtures written to", HERE)
