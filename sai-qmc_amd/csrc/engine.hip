@@ -76,6 +76,17 @@ __global__ void k_copy_d(const double *src, long long sS, double *dst, long long
         dst[b * sD + idx] = src[b * sS + idx];
 }
 
+// acceptance counters at the start of the next measured sweep: a bin adds
+// only the accepts of its own sweeps (src/replica_run.c:293-295, 449-451)
+__global__ void k_acc_mark(const unsigned long long *acc, const unsigned long long *att,
+                           unsigned long long *acc_prev, unsigned long long *att_prev, int batch)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch) return;
+    acc_prev[b] = acc[b];
+    att_prev[b] = att[b];
+}
+
 __global__ void k_d2f(const double *src, long long sS, float *dst, long long sD, long long len, int batch)
 {
     const int b = blockIdx.y;
@@ -1141,6 +1152,13 @@ int dqmc_gpu_sweep(dqmc_gpu_ctx *ctx, int nsweep, int bin)
             if (bin >= 0 && stage_measure(*m, bin)) return 1;
         }
     }
+    if (bin < 0)
+        for (Model *m : ctx->models) {
+            if (m->nrep == 0) continue;
+            hipLaunchKernelGGL(k_acc_mark, dim3((m->nrep + 255) / 256), dim3(256), 0, m->w.s, m->acc, m->att,
+                               m->acc_prev, m->att_prev, m->nrep);
+            if (launch_check("k_acc_mark")) return 1;
+        }
     for (size_t k = 1; k < ctx->models.size(); k++) {
         hipEvent_t e;
         HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
