@@ -48,12 +48,16 @@ def stage_flops(n, L, s, acc):
     }
 
 
-def run_engine(dq, precision, steps, warmup, drift):
+def run_engine(dq, precision, steps, warmup, drift, groups=1, delay=DELAY):
+    """The 128 replicas as `groups` equal models: each model has its own
+    stream and sweep graph, so the groups' sweeps run concurrently."""
     eng = dq.Engine(precision, drift=drift)
     L = int(round(BETA / DTAU))
-    mid = eng.add_model("square", LX, LY, U, DTAU, L, STAB, delay_k=DELAY)
-    eng.add_replicas(mid, list(range(1, NREP + 1)))
-    eng.set_measure(mid, nbin=1, q="all")
+    per = NREP // groups
+    for g in range(groups):
+        mid = eng.add_model("square", LX, LY, U, DTAU, L, STAB, delay_k=delay)
+        eng.add_replicas(mid, list(range(1 + g * per, 1 + (g + 1) * per)))
+        eng.set_measure(mid, nbin=1, q="all")
     eng.init()
     eng.sweep(warmup)
     eng.sweep(steps, 0)
@@ -88,9 +92,10 @@ def traffic_from_profile(kernel):
     return None
 
 
-def roofline(prof, acc, mixed):
+def roofline(prof, acc, mixed, nrep_launch=NREP):
     """Roofline of the stage with the largest time per sweep (one kernel
-    launch per invocation for the Metropolis and wrap stages)."""
+    launch per invocation for the Metropolis and wrap stages); a launch
+    covers nrep_launch replicas."""
     n, L, s = LX * LY, int(round(BETA / DTAU)), STAB
     fl = stage_flops(n, L, s, acc)
     best = None
@@ -102,9 +107,9 @@ def roofline(prof, acc, mixed):
             best = (k, ms, cnt, lo, hi)
     k, ms, cnt, lo, hi = best
     t = ms / cnt / 1e3
-    flops = (lo + hi) * NREP
+    flops = (lo + hi) * nrep_launch
     p_lo = P32 if mixed else P64
-    t_roof = lo * NREP / p_lo + hi * NREP / P64
+    t_roof = lo * nrep_launch / p_lo + hi * nrep_launch / P64
     peak = flops / t_roof
     ach = flops / t
     kern = KERNEL_OF_STAGE[k]
@@ -163,6 +168,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="all", choices=["all", "fp64", "mixed", "mixed_fp32_chain"])
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--groups", type=int, default=1, help="replica groups swept concurrently")
+    ap.add_argument("--delay", type=int, default=DELAY, help="delayed-update window (sites)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: replicas are independent Markov chains and this engine runs on a "
@@ -176,7 +183,7 @@ def main():
     names = list(modes) if a.precision == "all" else [a.precision]
     runs = {}
     for name in names:
-        runs[name] = run_engine(dq, modes[name], a.steps, a.warmup, drift=True)
+        runs[name] = run_engine(dq, modes[name], a.steps, a.warmup, drift=True, groups=a.groups, delay=a.delay)
         r = runs[name]
         print(f"# {name}: {r['value']:.1f} replica-sweeps/s, {r['ms'] / a.steps:.2f} ms/sweep, "
               f"drift max {r['drift_max']:.3e} mean {r['drift_mean']:.3e}", file=sys.stderr, flush=True)
@@ -202,7 +209,7 @@ def main():
         "drift_max": main_run["drift_max"],
         "drift_mean": main_run["drift_mean"],
         "acceptance": round(main_run["acc"], 4),
-        "roofline": roofline(main_run["profile"], main_run["acc"], main_name != "fp64"),
+        "roofline": roofline(main_run["profile"], main_run["acc"], main_name != "fp64", NREP // a.groups),
         "profile_ms_per_sweep": {k: round(v[0], 3) for k, v in main_run["profile"].items()},
     }
     for name in names[1:]:
@@ -210,7 +217,7 @@ def main():
         line[name] = {"value": round(r["value"], 2), "ms_per_step": round(r["ms"] / a.steps, 3),
                       "over_fp64": round(r["value"] / main_run["value"], 3),
                       "drift_max": r["drift_max"], "drift_mean": r["drift_mean"],
-                      "roofline": roofline(r["profile"], r["acc"], True),
+                      "roofline": roofline(r["profile"], r["acc"], True, NREP // a.groups),
                       "profile_ms_per_sweep": {k: round(v[0], 3) for k, v in r["profile"].items()}}
     line["cpu_baseline"] = cpu
     print(json.dumps(line))

@@ -462,6 +462,7 @@ struct Model {
     double *CU = nullptr, *CD = nullptr, *CT = nullptr;
     int *Cq = nullptr;
     unsigned long long *acc = nullptr, *att = nullptr, *acc_prev = nullptr, *att_prev = nullptr;
+    int *pcount = nullptr; // pending delayed-update pairs per replica after a slice
     double *sign = nullptr;
     double *dmax = nullptr, *dsum = nullptr;
     unsigned long long *dcnt = nullptr;
@@ -514,13 +515,18 @@ static int model_for_replica(dqmc_gpu_ctx *ctx, int replica, Model **m, int *loc
 
 // Delay depth (window) for the Metropolis kernel: the largest of 32, 16, 8
 // not above the request whose two W x n pending-update tiles fit in LDS.
+// window of the Metropolis kernel: the largest of 32 / 16 / 8 sites not
+// above the requested delay whose LDS fits (FP64 at most 16: the 32-site
+// FP64 window does not fit the register file)
 static int metro_window(int n, int want, size_t elem)
 {
     const size_t budget = 150 * 1024;
     int W = want > 0 ? want : 32;
+    if (elem == sizeof(double)) W = std::min(W, 16);
     const int opts[] = {32, 16, 8};
     for (int o : opts)
-        if (o <= W && metro_lds_bytes<double>(n, o) * elem / 8 <= budget) return o;
+        if (o <= W && (elem == sizeof(double) ? metro_lds_bytes<double>(n, o) : metro_lds_bytes<float>(n, o)) <= budget)
+            return o;
     return 8;
 }
 
@@ -558,6 +564,8 @@ static int model_alloc(dqmc_gpu_ctx *ctx, Model &m)
         dalloc(&m.CT, nn * R) || dalloc(&m.Cq, R))
         return 1;
     own(m.LU_); own(m.LD); own(m.LT); own(m.Lq); own(m.CU); own(m.CD); own(m.CT); own(m.Cq);
+    if (dalloc(&m.pcount, R)) return 1;
+    own(m.pcount);
     if (dalloc(&m.acc, R) || dalloc(&m.att, R) || dalloc(&m.acc_prev, R) || dalloc(&m.att_prev, R) ||
         dalloc(&m.sign, R) || dalloc(&m.dmax, R) || dalloc(&m.dsum, R) || dalloc(&m.dcnt, R))
         return 1;
@@ -638,12 +646,20 @@ template <typename T, int W> static void launch_metro(const MetroArgs<T> &a, hip
 
 template <typename T> static void launch_metro_w(const MetroArgs<T> &a, hipStream_t s)
 {
-    if (a.W >= 32) launch_metro<T, 32>(a, s);
-    else if (a.W >= 16) launch_metro<T, 16>(a, s);
+    if constexpr (sizeof(T) == 4) {
+        if (a.W >= 32) {
+            launch_metro<T, 32>(a, s);
+            return;
+        }
+    }
+    if (a.W >= 16) launch_metro<T, 16>(a, s);
     else launch_metro<T, 8>(a, s);
 }
 
-// Metropolis over slice l (src/dqmc.c:656-679)
+// Metropolis over slice l (src/dqmc.c:656-679): the proposal kernel leaves
+// the slice's accepted pairs in C, V (workspace), then one batched GEMM
+// with per-replica K applies G -= C V^T (green_delay_flush,
+// src/green.c:462-475).
 static int stage_metro(dqmc_gpu_ctx *ctx, Model &m, int l)
 {
     ProfScope ps_(DQMC_GPU_PROF_METRO, m.w.s);
@@ -662,21 +678,33 @@ static int stage_metro(dqmc_gpu_ctx *ctx, Model &m, int l)
         a.rng = m.rng;
         a.nu[0] = m.nu[0]; a.nu[1] = m.nu[1]; a.nd[0] = m.nd[0]; a.nd[1] = m.nd[1];
         a.acc = m.acc; a.att = m.att; a.sign = m.sign; a.status = m.w.status;
+        a.Cp = m.w.F1; a.Vp = m.w.F2; a.sP = nn; a.pcount = m.pcount;
         launch_metro_w<float>(a, m.w.s);
-    } else {
-        MetroArgs<double> a;
-        a.n = n; a.L = m.L; a.W = m.W; a.l = l; a.batch = m.nrep;
-        a.G = m.G; a.sG = nn; a.field = m.field; a.sF = sV;
-        a.dvec = m.dvec; a.dinv = m.dinv; a.sV = sV;
-        a.e[0] = m.expv[0]; a.e[1] = m.expv[1];
-        a.einv[0] = m.expvinv[0]; a.einv[1] = m.expvinv[1];
-        a.dvec64 = nullptr; a.dinv64 = nullptr;
-        a.rng = m.rng;
-        a.nu[0] = m.nu[0]; a.nu[1] = m.nu[1]; a.nd[0] = m.nd[0]; a.nd[1] = m.nd[1];
-        a.acc = m.acc; a.att = m.att; a.sign = m.sign; a.status = m.w.status;
-        launch_metro_w<double>(a, m.w.s);
+        if (launch_check("k_metro_slice")) return 1;
+        GemmArgs<float> g = gemm_args<float>(n, n, n, m.nrep, m.w.F1, n, nn, 0, m.w.F2, n, nn, 1, m.Gf, n, nn);
+        g.alpha = -1.0f;
+        g.beta = 1.0f;
+        g.Kb = m.pcount;
+        return gemm(m.w, g);
     }
-    return launch_check("k_metro_slice");
+    MetroArgs<double> a;
+    a.n = n; a.L = m.L; a.W = m.W; a.l = l; a.batch = m.nrep;
+    a.G = m.G; a.sG = nn; a.field = m.field; a.sF = sV;
+    a.dvec = m.dvec; a.dinv = m.dinv; a.sV = sV;
+    a.e[0] = m.expv[0]; a.e[1] = m.expv[1];
+    a.einv[0] = m.expvinv[0]; a.einv[1] = m.expvinv[1];
+    a.dvec64 = nullptr; a.dinv64 = nullptr;
+    a.rng = m.rng;
+    a.nu[0] = m.nu[0]; a.nu[1] = m.nu[1]; a.nd[0] = m.nd[0]; a.nd[1] = m.nd[1];
+    a.acc = m.acc; a.att = m.att; a.sign = m.sign; a.status = m.w.status;
+    a.Cp = m.w.W3; a.Vp = m.w.W4; a.sP = nn; a.pcount = m.pcount;
+    launch_metro_w<double>(a, m.w.s);
+    if (launch_check("k_metro_slice")) return 1;
+    GemmArgs<double> g = gemm_args<double>(n, n, n, m.nrep, m.w.W3, n, nn, 0, m.w.W4, n, nn, 1, m.G, n, nn);
+    g.alpha = -1.0;
+    g.beta = 1.0;
+    g.Kb = m.pcount;
+    return gemm(m.w, g);
 }
 
 // G <- B_l G B_l^-1 = expK diag(d_l) G diag(1/d_l) expKinv (src/green.c:509-527)
@@ -902,7 +930,7 @@ static int set_lds_limits()
     static bool done = false;
     if (done) return 0;
     // 160 KiB of LDS per CU on gfx950, less each kernel's static LDS
-    const void *kernels[] = {(const void *)k_metro_slice<double, 32>, (const void *)k_metro_slice<double, 16>,
+    const void *kernels[] = {(const void *)k_metro_slice<double, 16>,
                              (const void *)k_metro_slice<double, 8>,  (const void *)k_metro_slice<float, 32>,
                              (const void *)k_metro_slice<float, 16>,  (const void *)k_metro_slice<float, 8>,
                              (const void *)k_qr_udv<1>, (const void *)k_qr_udv<2>, (const void *)k_qr_udv<4>, (const void *)k_qr_udv<8>,

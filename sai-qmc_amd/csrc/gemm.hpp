@@ -29,6 +29,7 @@ template <typename T> struct GemmArgs {
     T alpha, beta;
     Vec<T> ar, ac, br, bc, cr, cc, dg;
     const int *status; // skip batch entries with status != 0 (may be null)
+    const int *Kb;     // per-entry K <= K (may be null)
 };
 
 #define GEMM_BM 64
@@ -48,6 +49,8 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs<T> g)
     const int n0 = (t / tiles_m) * GEMM_BN;
     if (b >= g.batch) return;
     if (g.status && g.status[b]) return;
+    const int K = g.Kb ? min(g.K, g.Kb[b]) : g.K;
+    if (K <= 0 && g.beta == T(1) && !g.dg.p) return; // C unchanged
 
     __shared__ T As[GEMM_BK][GEMM_BM + 4];
     __shared__ T Bs[GEMM_BK][GEMM_BN + 4];
@@ -64,7 +67,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs<T> g)
 #pragma unroll
         for (int j = 0; j < 2; j++) acc[i][j] = acc_t{0, 0, 0, 0};
 
-    for (int k0 = 0; k0 < g.K; k0 += GEMM_BK) {
+    for (int k0 = 0; k0 < K; k0 += GEMM_BK) {
 #pragma unroll
         for (int r = 0; r < (GEMM_BM * GEMM_BK) / 256; r++) {
             const int e = tid + 256 * r;
@@ -73,7 +76,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs<T> g)
             else { k = e % GEMM_BK; i = e / GEMM_BK; }
             const int gi = m0 + i, gk = k0 + k;
             T v = 0;
-            if (gi < g.M && gk < g.K) {
+            if (gi < g.M && gk < K) {
                 v = g.transA ? A[gk + (long long)gi * g.lda] : A[gi + (long long)gk * g.lda];
                 v *= g.ar.at(b, gi) * g.ac.at(b, gk);
             }
@@ -87,7 +90,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs<T> g)
             else { j = e % GEMM_BN; k = e / GEMM_BN; }
             const int gk = k0 + k, gj = n0 + j;
             T v = 0;
-            if (gk < g.K && gj < g.N) {
+            if (gk < K && gj < g.N) {
                 v = g.transB ? B[gj + (long long)gk * g.ldb] : B[gk + (long long)gj * g.ldb];
                 v *= g.br.at(b, gk) * g.bc.at(b, gj);
             }
@@ -179,6 +182,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_big(GemmArgs<T> g)
     const int n0 = (t / tiles_m) * GB_M;
     if (b >= g.batch) return;
     if (g.status && g.status[b]) return;
+    const int K = g.Kb ? min(g.K, g.Kb[b]) : g.K;
+    if (K <= 0 && g.beta == T(1) && !g.dg.p) return; // C unchanged
 
     extern __shared__ __align__(16) unsigned char gb_raw[];
     T *lds = reinterpret_cast<T *>(gb_raw);
@@ -213,7 +218,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_big(GemmArgs<T> g)
         for (int r = 0; r < 4; r++) {
             if (!TA) {
                 const int i = m0 + 2 * (tid & 63), k = k0 + (tid >> 6) + 4 * r;
-                const bool kok = k < g.K;
+                const bool kok = k < K;
                 ra[r] = ld_two(A + i + (long long)k * g.lda, kok && i < g.M, kok && i + 1 < g.M);
                 const T s = kok ? g.ac.at(b, k) : T(0);
                 ra[r].x *= sa_fixed[0] * s;
@@ -221,21 +226,21 @@ __global__ __launch_bounds__(256, 2) void k_gemm_big(GemmArgs<T> g)
             } else {
                 const int i = m0 + (tid >> 3) + 32 * r, k = k0 + 2 * (tid & 7);
                 const bool iok = i < g.M;
-                ra[r] = ld_two(A + k + (long long)i * g.lda, iok && k < g.K, iok && k + 1 < g.K);
+                ra[r] = ld_two(A + k + (long long)i * g.lda, iok && k < K, iok && k + 1 < K);
                 const T s = iok ? g.ar.at(b, i) : T(0);
-                ra[r].x *= s * (k < g.K ? g.ac.at(b, k) : T(0));
-                ra[r].y *= s * (k + 1 < g.K ? g.ac.at(b, k + 1) : T(0));
+                ra[r].x *= s * (k < K ? g.ac.at(b, k) : T(0));
+                ra[r].y *= s * (k + 1 < K ? g.ac.at(b, k + 1) : T(0));
             }
             if (!TB) {
                 const int j = n0 + (tid >> 3) + 32 * r, k = k0 + 2 * (tid & 7);
                 const bool jok = j < g.N;
-                rb[r] = ld_two(B + k + (long long)j * g.ldb, jok && k < g.K, jok && k + 1 < g.K);
+                rb[r] = ld_two(B + k + (long long)j * g.ldb, jok && k < K, jok && k + 1 < K);
                 const T s = jok ? g.bc.at(b, j) : T(0);
-                rb[r].x *= (k < g.K ? g.br.at(b, k) : T(0)) * s;
-                rb[r].y *= (k + 1 < g.K ? g.br.at(b, k + 1) : T(0)) * s;
+                rb[r].x *= (k < K ? g.br.at(b, k) : T(0)) * s;
+                rb[r].y *= (k + 1 < K ? g.br.at(b, k + 1) : T(0)) * s;
             } else {
                 const int j = n0 + 2 * (tid & 63), k = k0 + (tid >> 6) + 4 * r;
-                const bool kok = k < g.K;
+                const bool kok = k < K;
                 rb[r] = ld_two(B + j + (long long)k * g.ldb, kok && j < g.N, kok && j + 1 < g.N);
                 const T s = kok ? g.br.at(b, k) : T(0);
                 rb[r].x *= s * sb_fixed[0];
@@ -278,8 +283,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_big(GemmArgs<T> g)
     store_tile(0);
     __syncthreads();
     int u = 0;
-    for (int k0 = 0; k0 < g.K; k0 += GB_K) {
-        const bool more = k0 + GB_K < g.K;
+    for (int k0 = 0; k0 < K; k0 += GB_K) {
+        const bool more = k0 + GB_K < K;
         if (more) load_tile(k0 + GB_K);
         const T *As = lds + u * 2 * GB_K * GB_LD;
         const T *Bs = As + GB_K * GB_LD;
@@ -334,6 +339,7 @@ static inline GemmArgs<T> gemm_args(int M, int N, int K, int batch, const T *A, 
     g.alpha = T(1); g.beta = T(0);
     g.ar = g.ac = g.br = g.bc = g.cr = g.cc = g.dg = vnone<T>();
     g.status = nullptr;
+    g.Kb = nullptr;
     return g;
 }
 

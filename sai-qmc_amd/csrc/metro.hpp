@@ -1,20 +1,24 @@
 // Single-site Metropolis over one imaginary-time slice with delayed
-// updates, one workgroup per replica and one thread per site (forward PH
-// sweep, src/dqmc.c:656-679, src/green.c:384-475).
+// updates, one workgroup per replica (forward PH sweep, src/dqmc.c:656-679,
+// src/green.c:384-475).
 //
-// Sites are visited in windows of W consecutive sites.  At a window start
-// thread j holds in registers G~(j, i0+w) and G~(i0+w, j) for the window's W
-// sites (the window's columns and rows of the current Green's function), and
-// every thread holds the W diagonal entries G~(i0+w, i0+w).  An accepted flip
-// at site i = i0+w forms the reference's delayed-update pair
+// G itself is not touched during the slice.  Accepted flips become pairs
 // c = (N/R)(e_i - G~ e_i), v = G~^T e_i (green_delay_accept,
-// src/green.c:418-460) straight from those registers, publishes it to LDS,
-// and after one barrier every thread applies the rank-1 change to its
-// remaining window registers: no global traffic and no O(k) loop on the
-// sequential path.  At the window end the accumulated rank-p update
-// G -= C V^T (green_delay_flush / la_gemm_nt_rect, src/green.c:462-475,
-// src/linalg.c:209-219) is applied to G in global memory with MFMA 16x16x4
-// tiles.  The window size W is the delay depth k.
+// src/green.c:418-460) appended to per-replica pending buffers C, V (n x P,
+// q-major: C[q*n + j]); G~ = G - C V^T is never formed.  The sites are
+// visited in windows of W:
+//   1. the window's columns and rows of G~ are computed on MFMA from G and
+//      the pending pairs (n x W x P products, staged through LDS);
+//   2. wave 0 runs the window's W proposals alone on the W x W block of G~
+//      in its registers: each accept forms the pair restricted to the window
+//      and applies the rank-1 change to the block (no workgroup barrier per
+//      proposal);
+//   3. every thread j forms c_q(j), v_q(j) of the window's accepts from its
+//      column/row registers and the window-restricted pairs (O(p^2) per
+//      thread), and appends them to C, V.
+// The rank-P flush G -= C V^T (green_delay_flush / la_gemm_nt_rect,
+// src/green.c:462-475, src/linalg.c:209-219) is one batched GEMM after the
+// kernel, with P per replica in `pcount`.
 //
 // The ratio is R = (1 + (1 - G~_ii) N_up)(1 + G~_ii N_dn)
 // (src/green.c:401-416), always evaluated in FP64, accepted iff
@@ -25,7 +29,7 @@
 
 template <typename T> struct MetroArgs {
     int n, L, W, l, batch;
-    T *G;
+    const T *G;
     long long sG;
     signed char *field; // [L][n] per replica
     long long sF;
@@ -39,64 +43,117 @@ template <typename T> struct MetroArgs {
     unsigned long long *acc, *att;
     double *sign;
     int *status;
+    T *Cp, *Vp;         // pending pairs, n x n per replica (q-major)
+    long long sP;
+    int *pcount;        // pending pair count per replica (the flush GEMM's K)
 };
 
 #define METRO_THREADS 512
 #define METRO_MAX_N 512
 
+// LDS: corrected columns and rows (2 x W x (n+1)), the window slices of C
+// and V (2 x n x W, P <= n), the window block and the window-restricted
+// pairs (3 x W x W), N/R per site, the slice's draws, spins and flips.
 template <typename T>
 static inline size_t metro_lds_bytes(int n, int W)
 {
-    return sizeof(T) * ((size_t)2 * W * n + 4 * W) + sizeof(double) * n + 2 * (size_t)n + 64;
+    return sizeof(T) * (2 * (size_t)W * (n + 1) + 2 * (size_t)n * W + 3 * (size_t)W * W + W) + sizeof(double) * n +
+           2 * (size_t)n + 64;
 }
 
-// G[n x n] -= Cc[0:p]^T Vv[0:p] with Cc, Vv stored q-major (Cc[q*n + j]).
-// Each MFMA tile holds a 16x16 block of G^T (output row = G column, output
-// column = G row), so the 16 lanes of an accumulator row cover 16
-// consecutive rows of one column of G: every load and store instruction
-// touches four 128-byte segments.  B tiles per wave are kept in flight.
-template <typename T>
-__device__ void flush_rank_p(T *G, int n, const T *Cc, const T *Vv, int p)
+__device__ __forceinline__ float bcast_any(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+__device__ __forceinline__ double bcast_any(double v, int lane)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// Window corrections on MFMA, columns and rows in one pass:
+//   Sc[w][j] = G[j][i0+w] - sum_q C[q][j] V[q][i0+w]
+//   Sr[w][j] = G[i0+w][j] - sum_q C[q][i0+w] V[q][j]
+// Sc, Sr are W x (n+1) in LDS; Vw[q*W + w] = V[q][i0+w] and Cw[q*W + w] =
+// C[q][i0+w] are the window slices staged in LDS.  Each wave takes 16-row
+// blocks of j for both outputs at once (two accumulators per 16-wide w
+// block), streaming C and V from global memory with the next eight k-steps
+// in flight.  Each accumulator row of a tile covers 16 consecutive rows of
+// one column of G, so the G loads are 128-byte segments.
+template <typename T, int W>
+__device__ void window_correct(const T *G, int n, int i0, int wn, const T *C, const T *V, int P, const T *Cw,
+                               const T *Vw, T *Sc, T *Sr)
 {
     typedef typename Mfma<T>::acc_t acc_t;
+    constexpr int NTW = (W + 15) / 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int lc = lane & 15, lr = lane >> 4;
-    const int nt = (n + 15) >> 4;
-    const int ntiles = nt * nt;
-    constexpr int B = sizeof(T) == 8 ? 8 : 16;
-    for (int t0 = wave * B; t0 < ntiles; t0 += nw * B) {
-        acc_t acc[B];
+    const int ntj = (n + 15) >> 4;
+    const int ld = n + 1;
+    for (int tj = wave; tj < ntj; tj += nw) {
+        const int j0 = tj * 16;
+        const bool jok = j0 + lc < n;
+        acc_t ac[NTW], ar[NTW];
 #pragma unroll
-        for (int u = 0; u < B; u++) {
-            const int t = t0 + u;
-            const int r0 = (t % nt) * 16, c0 = (t / nt) * 16;
-            const int row = r0 + lc;
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int col = c0 + Mfma<T>::row(lane, v);
-                acc[u][v] = (t < ntiles && row < n && col < n) ? G[row + (long long)col * n] : T(0);
-            }
-        }
-        for (int q0 = 0; q0 < p; q0 += 4) {
-            const int q = q0 + lr;
-#pragma unroll
-            for (int u = 0; u < B; u++) {
-                const int t = t0 + u;
-                const int r0 = (t % nt) * 16, c0 = (t / nt) * 16;
-                const T a = (q < p && c0 + lc < n) ? -Vv[q * n + c0 + lc] : T(0);
-                const T b = (q < p && r0 + lc < n) ? Cc[q * n + r0 + lc] : T(0);
-                acc[u] = Mfma<T>::mma(a, b, acc[u]);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < B; u++) {
-            const int t = t0 + u;
-            const int r0 = (t % nt) * 16, c0 = (t / nt) * 16;
-            const int row = r0 + lc;
+        for (int tw = 0; tw < NTW; tw++) {
+            const int w0 = tw * 16;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int col = c0 + Mfma<T>::row(lane, v);
-                if (t < ntiles && row < n && col < n) G[row + (long long)col * n] = acc[u][v];
+                const int rr = Mfma<T>::row(lane, v);
+                // cols tile: rows w = w0 + rr, columns j = j0 + lc
+                ac[tw][v] = (w0 + rr < wn && jok) ? G[(j0 + lc) + (long long)(i0 + w0 + rr) * n] : T(0);
+                // rows tile: rows j = j0 + rr, columns w = w0 + lc
+                ar[tw][v] = (w0 + lc < wn && j0 + rr < n) ? G[(i0 + w0 + lc) + (long long)(j0 + rr) * n] : T(0);
+            }
+        }
+        // big operands: C[q][j0+lc] (cols) and V[q][j0+lc] (rows), 8 k-steps per batch
+        T nc[8], nv[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int q = 4 * u + lr;
+            nc[u] = (q < P && jok) ? C[(long long)q * n + j0 + lc] : T(0);
+            nv[u] = (q < P && jok) ? V[(long long)q * n + j0 + lc] : T(0);
+        }
+        for (int q0 = 0; q0 < P; q0 += 32) {
+            T cc[8], cv[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                cc[u] = nc[u];
+                cv[u] = nv[u];
+            }
+            if (q0 + 32 < P) {
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int q = q0 + 32 + 4 * u + lr;
+                    nc[u] = (q < P && jok) ? C[(long long)q * n + j0 + lc] : T(0);
+                    nv[u] = (q < P && jok) ? V[(long long)q * n + j0 + lc] : T(0);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int q = q0 + 4 * u + lr;
+#pragma unroll
+                for (int tw = 0; tw < NTW; tw++) {
+                    const int w = tw * 16 + lc;
+                    const T vw = (q < P && w < wn) ? Vw[q * W + w] : T(0);
+                    const T cw = (q < P && w < wn) ? Cw[q * W + w] : T(0);
+                    // cols: A[w][q] = -V[q][i0+w], B[q][j] = C[q][j]
+                    ac[tw] = Mfma<T>::mma(-vw, cc[u], ac[tw]);
+                    // rows: A[j][q] = -V[q][j], B[q][w] = C[q][i0+w]
+                    ar[tw] = Mfma<T>::mma(-cv[u], cw, ar[tw]);
+                }
+            }
+        }
+#pragma unroll
+        for (int tw = 0; tw < NTW; tw++) {
+            const int w0 = tw * 16;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int rr = Mfma<T>::row(lane, v);
+                if (w0 + rr < W && jok) Sc[(w0 + rr) * ld + j0 + lc] = ac[tw][v];
+                if (w0 + lc < W && j0 + rr < n) Sr[(w0 + lc) * ld + j0 + rr] = ar[tw][v];
             }
         }
     }
@@ -105,25 +162,35 @@ __device__ void flush_rank_p(T *G, int n, const T *Cc, const T *Vv, int p)
 template <typename T, int W>
 __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
 {
+    static_assert(W == 8 || W == 16 || W == 32, "window of 8, 16 or 32 sites");
+    constexpr int RPL = W * W / 64; // rows of the window block per lane of wave 0
     const int b = blockIdx.x;
     if (b >= a.batch) return;
-    if (a.status[b]) return;
+    if (a.status[b]) {
+        if (threadIdx.x == 0) a.pcount[b] = 0;
+        return;
+    }
     const int n = a.n, j = threadIdx.x;
+    const int lane = j & 63, wave = j >> 6;
     const bool act = j < n;
-    T *G = a.G + (long long)b * a.sG;
+    const T *G = a.G + (long long)b * a.sG;
+    T *Cp = a.Cp + (long long)b * a.sP;
+    T *Vp = a.Vp + (long long)b * a.sP;
     signed char *fld = a.field + (long long)b * a.sF + (long long)a.l * n;
 
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    T *Cc = (T *)lds_raw;        // W x n pending columns c_q (also the row staging area)
-    T *Vv = Cc + (size_t)W * n;  // W x n pending rows v_q
-    // the accepted pair at the window's sites, double-buffered by the parity
-    // of the accept count so a fast thread never overwrites values a slow one
-    // is still reading
-    T *cwin = Vv + (size_t)W * n;
-    T *vwin = cwin + 2 * W;
-    double *draws = (double *)(vwin + 2 * W);
+    T *Sc = (T *)lds_raw;             // W x (n+1) corrected window columns
+    T *Sr = Sc + (size_t)W * (n + 1); // W x (n+1) corrected window rows
+    T *Cw = Sr + (size_t)W * (n + 1); // n x W window slice of C (q-major)
+    T *Vw = Cw + (size_t)n * W;       // n x W window slice of V
+    T *Bw = Vw + (size_t)n * W;       // W x W window block, Bw[r*W + c]
+    T *cwin = Bw + W * W;            // cwin[u*W + r]: pair of the accept at window site u
+    T *vwin = cwin + W * W;          // vwin[u*W + c]
+    T *facw = vwin + W * W;          // N/R of the accept at window site u
+    double *draws = (double *)(facw + W);
     signed char *fs = (signed char *)(draws + n);
     signed char *flip = fs + n;
+    __shared__ unsigned s_mask;
 
     if (act) {
         fs[j] = fld[j];
@@ -138,78 +205,112 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
     }
     __syncthreads();
 
-    int naccept = 0, negflips = 0;
-    T gc[W], gr[W], dW[W];
+    int naccept = 0, negflips = 0, P = 0;
+    T gc[W], gr[W];
     for (int i0 = 0; i0 < n; i0 += W) {
         const int wn = min(W, n - i0);
-        // window columns (coalesced over j) and rows (staged through LDS)
-#pragma unroll
-        for (int w = 0; w < W; w++) gc[w] = (act && w < wn) ? G[j + (long long)(i0 + w) * n] : T(0);
-        for (int base = threadIdx.x; base < wn * n; base += 8 * blockDim.x) {
-            T v[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int idx = base + u * blockDim.x;
-                v[u] = idx < wn * n ? G[(i0 + idx % wn) + (long long)(idx / wn) * n] : T(0);
-            }
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int idx = base + u * blockDim.x;
-                if (idx < wn * n) Cc[(idx % wn) * n + idx / wn] = v[u];
-            }
+        // 1. corrected window columns -> gc, rows -> gr, the window block -> Bw
+        for (int idx = threadIdx.x; idx < P * W; idx += blockDim.x) {
+            const int q = idx / W, w = idx % W;
+            Cw[idx] = w < wn ? Cp[(long long)q * n + i0 + w] : T(0);
+            Vw[idx] = w < wn ? Vp[(long long)q * n + i0 + w] : T(0);
         }
+        __syncthreads();
+        window_correct<T, W>(G, n, i0, wn, Cp, Vp, P, Cw, Vw, Sc, Sr);
         __syncthreads();
 #pragma unroll
         for (int w = 0; w < W; w++) {
-            gr[w] = (act && w < wn) ? Cc[w * n + j] : T(0);
-            dW[w] = (w < wn) ? Cc[w * n + i0 + w] : T(0);
+            gc[w] = (act && w < wn) ? Sc[w * (n + 1) + j] : T(0);
+            gr[w] = (act && w < wn) ? Sr[w * (n + 1) + j] : T(0);
+        }
+        if (act && j >= i0 && j < i0 + wn) {
+#pragma unroll
+            for (int w = 0; w < W; w++) Bw[(j - i0) * W + w] = gc[w];
         }
         __syncthreads();
-        int p = 0;
+
+        // 2. proposals on the window block, wave 0 alone
+        if (wave == 0) {
+            const int c = lane % W, h = lane / W;
+            T bw[RPL];
 #pragma unroll
-        for (int w = 0; w < W; w++) {
-            if (w < wn) {
-                const int i = i0 + w;
-                const double gii = (double)dW[w];
-                const int sp = fs[i] > 0;
-                const double Nu = a.nu[sp], Nd = a.nd[sp];
-                const double Ru = 1.0 + (1.0 - gii) * Nu;
-                const double Rd = 1.0 + gii * Nd;
-                const double R = Ru * Rd;
-                if (draws[i] < fabs(R)) {
-                    const T fac = (T)(Nu / Ru);
-                    const T cj = fac * ((j == i ? T(1) : T(0)) - gc[w]);
-                    const T vj = gr[w];
-                    if (act) {
-                        Cc[p * n + j] = cj;
-                        Vv[p * n + j] = vj;
-                        if (j >= i0 && j < i0 + wn) {
-                            cwin[(p & 1) * W + j - i0] = cj;
-                            vwin[(p & 1) * W + j - i0] = vj;
+            for (int k = 0; k < RPL; k++) {
+                const int r = h * RPL + k;
+                bw[k] = (r < wn && c < wn) ? Bw[r * W + c] : T(0);
+            }
+            // site data of the window in lanes 0..wn-1, broadcast per proposal
+            double dr_l = 0.0, nu_l = 0.0, nd_l = 0.0;
+            if (lane < wn) {
+                const int sp = fs[i0 + lane] > 0;
+                dr_l = draws[i0 + lane];
+                nu_l = a.nu[sp];
+                nd_l = a.nd[sp];
+            }
+            unsigned mask = 0;
+#pragma unroll
+            for (int w = 0; w < W; w++) {
+                if (w < wn) {
+                    const int i = i0 + w;
+                    const double gii = (double)bcast_any(bw[w % RPL], (w / RPL) * W + w);
+                    const double Nu = bcast_any(nu_l, w), Nd = bcast_any(nd_l, w);
+                    const double Ru = 1.0 + (1.0 - gii) * Nu;
+                    const double Rd = 1.0 + gii * Nd;
+                    const double R = Ru * Rd;
+                    if (bcast_any(dr_l, w) < fabs(R)) {
+                        const T fac = (T)(Nu / Ru);
+                        // column w of the block -> c, row w -> v (values before the update)
+                        if (c == w) {
+#pragma unroll
+                            for (int k = 0; k < RPL; k++) {
+                                const int r = h * RPL + k;
+                                cwin[w * W + r] = fac * ((r == w ? T(1) : T(0)) - bw[k]);
+                            }
+                        }
+                        if (h == w / RPL) vwin[w * W + c] = bw[w % RPL];
+                        if (lane == 0) {
+                            facw[w] = fac;
+                            flip[i] = 1;
+                        }
+                        mask |= 1u << w;
+                        naccept++;
+                        if (R < 0.0) negflips ^= 1;
+                        __builtin_amdgcn_s_waitcnt(0xc07f);
+                        __builtin_amdgcn_wave_barrier();
+                        const T vc = vwin[w * W + c];
+#pragma unroll
+                        for (int k = 0; k < RPL; k++) bw[k] -= cwin[w * W + h * RPL + k] * vc;
+                        __builtin_amdgcn_s_waitcnt(0xc07f);
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                }
+            }
+            if (lane == 0) s_mask = mask;
+        }
+        __syncthreads();
+
+        // 3. full pairs of the window's accepts, appended to C, V
+        const unsigned mask = s_mask;
+        if (act) {
+#pragma unroll
+            for (int w = 0; w < W; w++) {
+                if ((mask >> w) & 1u) {
+                    T s = gc[w], t = gr[w];
+#pragma unroll
+                    for (int u = 0; u < w; u++) {
+                        if ((mask >> u) & 1u) {
+                            s -= gc[u] * vwin[u * W + w];
+                            t -= cwin[u * W + w] * gr[u];
                         }
                     }
-                    if (j == 0) flip[i] = 1;
-                    naccept++;
-                    if (R < 0.0) negflips ^= 1;
-                    __syncthreads();
-#pragma unroll
-                    for (int w2 = w + 1; w2 < W; w2++) {
-                        if (w2 < wn) {
-                            const T cw = cwin[(p & 1) * W + w2];
-                            const T vw = vwin[(p & 1) * W + w2];
-                            gc[w2] -= cj * vw;
-                            gr[w2] -= cw * vj;
-                            dW[w2] -= cw * vw;
-                        }
-                    }
-                    p++;
+                    gc[w] = facw[w] * ((j == i0 + w ? T(1) : T(0)) - s);
+                    gr[w] = t;
+                    const int q = P + __builtin_popcount(mask & ((1u << w) - 1u));
+                    Cp[(long long)q * n + j] = gc[w];
+                    Vp[(long long)q * n + j] = gr[w];
                 }
             }
         }
-        if (p > 0) {
-            __syncthreads();
-            flush_rank_p<T>(G, n, Cc, Vv, p);
-        }
+        P += __builtin_popcount(mask);
         __syncthreads();
     }
     T *dv = a.dvec + (long long)b * a.sV + (long long)a.l * n;
@@ -226,6 +327,7 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
         }
     }
     if (j == 0) {
+        a.pcount[b] = P;
         a.acc[b] += (unsigned long long)naccept;
         a.att[b] += (unsigned long long)n;
         if (negflips) a.sign[b] = -a.sign[b];
