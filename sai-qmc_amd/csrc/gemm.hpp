@@ -139,8 +139,13 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs<T> g)
 // out of the other half of a double-buffered LDS; rows of the k-major LDS
 // tiles are padded so a 4-row fragment read covers all 64 banks once.
 #define GB_M 128
+#ifndef GB_K
 #define GB_K 16
+#endif
 #define GB_LD (GB_M + 16)
+#define GB_PPT (GB_K / 4)       // element pairs per thread per operand and k-tile
+#define GB_KH (GB_K / 2)        // pairs along k
+#define GB_IS (256 / GB_KH)     // rows between a thread's pairs when pairs run along k
 
 template <typename T>
 static inline size_t gemm_big_lds_bytes()
@@ -195,12 +200,12 @@ __global__ __launch_bounds__(256, 2) void k_gemm_big(GemmArgs<T> g)
     const int lc = lane & 15, lr = lane >> 4;
     const int wr = wave >> 1, wc = wave & 1;
 
-    // fixed per-thread load coordinates (4 pairs per operand per k-tile)
+    // fixed per-thread load coordinates (GB_PPT pairs per operand per k-tile)
     //   A, not transposed: pair (2*i2, 2*i2+1) x k, i2 = tid % 64, k = tid / 64 + 4 r
-    //   A, transposed:     pair i x (2*k2, 2*k2+1), k2 = tid % 8, i = tid / 8 + 32 r
-    //   B, not transposed: pair (2*k2, 2*k2+1) x j, k2 = tid % 8, j = tid / 8 + 32 r
+    //   A, transposed:     pair i x (2*k2, 2*k2+1), k2 = tid % GB_KH, i = tid / GB_KH + GB_IS r
+    //   B, not transposed: pair (2*k2, 2*k2+1) x j, k2 = tid % GB_KH, j = tid / GB_KH + GB_IS r
     //   B, transposed:     pair k x (2*j2, 2*j2+1), j2 = tid % 64, k = tid / 64 + 4 r
-    Two<T> ra[4], rb[4];
+    Two<T> ra[GB_PPT], rb[GB_PPT];
     T sa_fixed[2] = {T(1), T(1)}, sb_fixed[2] = {T(1), T(1)};
     if (!TA) {
         const int i = m0 + 2 * (tid & 63);
@@ -215,7 +220,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_big(GemmArgs<T> g)
 
     auto load_tile = [&](int k0) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
+        for (int r = 0; r < GB_PPT; r++) {
             if (!TA) {
                 const int i = m0 + 2 * (tid & 63), k = k0 + (tid >> 6) + 4 * r;
                 const bool kok = k < K;
@@ -224,7 +229,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_big(GemmArgs<T> g)
                 ra[r].x *= sa_fixed[0] * s;
                 ra[r].y *= sa_fixed[1] * s;
             } else {
-                const int i = m0 + (tid >> 3) + 32 * r, k = k0 + 2 * (tid & 7);
+                const int i = m0 + tid / GB_KH + GB_IS * r, k = k0 + 2 * (tid % GB_KH);
                 const bool iok = i < g.M;
                 ra[r] = ld_two(A + k + (long long)i * g.lda, iok && k < K, iok && k + 1 < K);
                 const T s = iok ? g.ar.at(b, i) : T(0);
@@ -232,7 +237,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_big(GemmArgs<T> g)
                 ra[r].y *= s * (k + 1 < K ? g.ac.at(b, k + 1) : T(0));
             }
             if (!TB) {
-                const int j = n0 + (tid >> 3) + 32 * r, k = k0 + 2 * (tid & 7);
+                const int j = n0 + tid / GB_KH + GB_IS * r, k = k0 + 2 * (tid % GB_KH);
                 const bool jok = j < g.N;
                 rb[r] = ld_two(B + k + (long long)j * g.ldb, jok && k < K, jok && k + 1 < K);
                 const T s = jok ? g.bc.at(b, j) : T(0);
@@ -252,17 +257,17 @@ __global__ __launch_bounds__(256, 2) void k_gemm_big(GemmArgs<T> g)
         T *As = lds + u * 2 * GB_K * GB_LD;
         T *Bs = As + GB_K * GB_LD;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
+        for (int r = 0; r < GB_PPT; r++) {
             if (!TA) {
                 T *d = As + ((tid >> 6) + 4 * r) * GB_LD + 2 * (tid & 63);
                 d[0] = ra[r].x; d[1] = ra[r].y;
             } else {
-                const int i = (tid >> 3) + 32 * r, k = 2 * (tid & 7);
+                const int i = tid / GB_KH + GB_IS * r, k = 2 * (tid % GB_KH);
                 As[k * GB_LD + i] = ra[r].x;
                 As[(k + 1) * GB_LD + i] = ra[r].y;
             }
             if (!TB) {
-                const int j = (tid >> 3) + 32 * r, k = 2 * (tid & 7);
+                const int j = tid / GB_KH + GB_IS * r, k = 2 * (tid % GB_KH);
                 Bs[k * GB_LD + j] = rb[r].x;
                 Bs[(k + 1) * GB_LD + j] = rb[r].y;
             } else {

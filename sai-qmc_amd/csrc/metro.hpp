@@ -77,58 +77,59 @@ __device__ __forceinline__ double bcast_any(double v, int lane)
 //   Sc[w][j] = G[j][i0+w] - sum_q C[q][j] V[q][i0+w]
 //   Sr[w][j] = G[i0+w][j] - sum_q C[q][i0+w] V[q][j]
 // Sc, Sr are W x (n+1) in LDS; Vw[q*W + w] = V[q][i0+w] and Cw[q*W + w] =
-// C[q][i0+w] are the window slices staged in LDS.  Each wave takes 16-row
-// blocks of j for both outputs at once (two accumulators per 16-wide w
-// block), streaming C and V from global memory with the next eight k-steps
-// in flight.  Each accumulator row of a tile covers 16 consecutive rows of
-// one column of G, so the G loads are 128-byte segments.
+// C[q][i0+w] are the window slices staged in LDS.  Each wave takes two
+// 16-row blocks of j for both outputs at once, streaming C and V from
+// global memory eight k-steps (32 loads per lane) at a time.  Each
+// accumulator row of a tile
+// covers 16 consecutive rows of one column of G, so the G loads are
+// 128-byte segments.
 template <typename T, int W>
 __device__ void window_correct(const T *G, int n, int i0, int wn, const T *C, const T *V, int P, const T *Cw,
                                const T *Vw, T *Sc, T *Sr)
 {
     typedef typename Mfma<T>::acc_t acc_t;
     constexpr int NTW = (W + 15) / 16;
+    constexpr int JT = 2; // j blocks per wave per pass
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int lc = lane & 15, lr = lane >> 4;
     const int ntj = (n + 15) >> 4;
     const int ld = n + 1;
-    for (int tj = wave; tj < ntj; tj += nw) {
-        const int j0 = tj * 16;
-        const bool jok = j0 + lc < n;
-        acc_t ac[NTW], ar[NTW];
+    for (int tb = wave; tb < ntj; tb += JT * nw) {
+        int j0[JT];
+        bool jok[JT];
 #pragma unroll
-        for (int tw = 0; tw < NTW; tw++) {
-            const int w0 = tw * 16;
+        for (int t = 0; t < JT; t++) {
+            j0[t] = (tb + t * nw) * 16;
+            jok[t] = j0[t] + lc < n;
+        }
+        acc_t ac[JT][NTW], ar[JT][NTW];
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int rr = Mfma<T>::row(lane, v);
-                // cols tile: rows w = w0 + rr, columns j = j0 + lc
-                ac[tw][v] = (w0 + rr < wn && jok) ? G[(j0 + lc) + (long long)(i0 + w0 + rr) * n] : T(0);
-                // rows tile: rows j = j0 + rr, columns w = w0 + lc
-                ar[tw][v] = (w0 + lc < wn && j0 + rr < n) ? G[(i0 + w0 + lc) + (long long)(j0 + rr) * n] : T(0);
+        for (int t = 0; t < JT; t++)
+#pragma unroll
+            for (int tw = 0; tw < NTW; tw++) {
+                const int w0 = tw * 16;
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int rr = Mfma<T>::row(lane, v);
+                    // cols tile: rows w = w0 + rr, columns j = j0 + lc
+                    ac[t][tw][v] = (w0 + rr < wn && jok[t]) ? G[(j0[t] + lc) + (long long)(i0 + w0 + rr) * n] : T(0);
+                    // rows tile: rows j = j0 + rr, columns w = w0 + lc
+                    ar[t][tw][v] =
+                        (w0 + lc < wn && j0[t] + rr < n) ? G[(i0 + w0 + lc) + (long long)(j0[t] + rr) * n] : T(0);
+                }
             }
-        }
-        // big operands: C[q][j0+lc] (cols) and V[q][j0+lc] (rows), 8 k-steps per batch
-        T nc[8], nv[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const int q = 4 * u + lr;
-            nc[u] = (q < P && jok) ? C[(long long)q * n + j0 + lc] : T(0);
-            nv[u] = (q < P && jok) ? V[(long long)q * n + j0 + lc] : T(0);
-        }
+        // big operands C[q][j0+lc] (cols) and V[q][j0+lc] (rows): 8 k-steps of
+        // both j blocks (32 loads per lane) in flight per batch; the second
+        // wave on the SIMD computes while this one waits
         for (int q0 = 0; q0 < P; q0 += 32) {
-            T cc[8], cv[8];
+            T cc[JT][8], cv[JT][8];
 #pragma unroll
             for (int u = 0; u < 8; u++) {
-                cc[u] = nc[u];
-                cv[u] = nv[u];
-            }
-            if (q0 + 32 < P) {
+                const int q = q0 + 4 * u + lr;
 #pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    const int q = q0 + 32 + 4 * u + lr;
-                    nc[u] = (q < P && jok) ? C[(long long)q * n + j0 + lc] : T(0);
-                    nv[u] = (q < P && jok) ? V[(long long)q * n + j0 + lc] : T(0);
+                for (int t = 0; t < JT; t++) {
+                    cc[t][u] = (q < P && jok[t]) ? C[(long long)q * n + j0[t] + lc] : T(0);
+                    cv[t][u] = (q < P && jok[t]) ? V[(long long)q * n + j0[t] + lc] : T(0);
                 }
             }
 #pragma unroll
@@ -139,23 +140,28 @@ __device__ void window_correct(const T *G, int n, int i0, int wn, const T *C, co
                     const int w = tw * 16 + lc;
                     const T vw = (q < P && w < wn) ? Vw[q * W + w] : T(0);
                     const T cw = (q < P && w < wn) ? Cw[q * W + w] : T(0);
-                    // cols: A[w][q] = -V[q][i0+w], B[q][j] = C[q][j]
-                    ac[tw] = Mfma<T>::mma(-vw, cc[u], ac[tw]);
-                    // rows: A[j][q] = -V[q][j], B[q][w] = C[q][i0+w]
-                    ar[tw] = Mfma<T>::mma(-cv[u], cw, ar[tw]);
+#pragma unroll
+                    for (int t = 0; t < JT; t++) {
+                        // cols: A[w][q] = -V[q][i0+w], B[q][j] = C[q][j]
+                        ac[t][tw] = Mfma<T>::mma(-vw, cc[t][u], ac[t][tw]);
+                        // rows: A[j][q] = -V[q][j], B[q][w] = C[q][i0+w]
+                        ar[t][tw] = Mfma<T>::mma(-cv[t][u], cw, ar[t][tw]);
+                    }
                 }
             }
         }
 #pragma unroll
-        for (int tw = 0; tw < NTW; tw++) {
-            const int w0 = tw * 16;
+        for (int t = 0; t < JT; t++)
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int rr = Mfma<T>::row(lane, v);
-                if (w0 + rr < W && jok) Sc[(w0 + rr) * ld + j0 + lc] = ac[tw][v];
-                if (w0 + lc < W && j0 + rr < n) Sr[(w0 + lc) * ld + j0 + rr] = ar[tw][v];
+            for (int tw = 0; tw < NTW; tw++) {
+                const int w0 = tw * 16;
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int rr = Mfma<T>::row(lane, v);
+                    if (w0 + rr < W && jok[t]) Sc[(w0 + rr) * ld + j0[t] + lc] = ac[t][tw][v];
+                    if (w0 + lc < W && j0[t] + rr < n) Sr[(w0 + lc) * ld + j0[t] + rr] = ar[t][tw][v];
+                }
             }
-        }
     }
 }
 
@@ -209,7 +215,7 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
     T gc[W], gr[W];
     for (int i0 = 0; i0 < n; i0 += W) {
         const int wn = min(W, n - i0);
-        // 1. corrected window columns -> gc, rows -> gr, the window block -> Bw
+        // 1. corrected window columns -> Sc, rows -> Sr, the window block -> Bw
         for (int idx = threadIdx.x; idx < P * W; idx += blockDim.x) {
             const int q = idx / W, w = idx % W;
             Cw[idx] = w < wn ? Cp[(long long)q * n + i0 + w] : T(0);
@@ -218,14 +224,9 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
         __syncthreads();
         window_correct<T, W>(G, n, i0, wn, Cp, Vp, P, Cw, Vw, Sc, Sr);
         __syncthreads();
-#pragma unroll
-        for (int w = 0; w < W; w++) {
-            gc[w] = (act && w < wn) ? Sc[w * (n + 1) + j] : T(0);
-            gr[w] = (act && w < wn) ? Sr[w * (n + 1) + j] : T(0);
-        }
-        if (act && j >= i0 && j < i0 + wn) {
-#pragma unroll
-            for (int w = 0; w < W; w++) Bw[(j - i0) * W + w] = gc[w];
+        for (int idx = threadIdx.x; idx < W * W; idx += blockDim.x) {
+            const int r = idx / W, c = idx % W;
+            Bw[idx] = (r < wn && c < wn) ? Sc[c * (n + 1) + i0 + r] : T(0);
         }
         __syncthreads();
 
@@ -291,6 +292,11 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
         // 3. full pairs of the window's accepts, appended to C, V
         const unsigned mask = s_mask;
         if (act) {
+#pragma unroll
+            for (int w = 0; w < W; w++) {
+                gc[w] = (w < wn) ? Sc[w * (n + 1) + j] : T(0);
+                gr[w] = (w < wn) ? Sr[w * (n + 1) + j] : T(0);
+            }
 #pragma unroll
             for (int w = 0; w < W; w++) {
                 if ((mask >> w) & 1u) {
