@@ -106,6 +106,18 @@ __global__ void k_f2d(const float *src, long long sS, double *dst, long long sD,
 }
 
 // out = expK diag(d) (the single-slice B matrix, src/green.c:126-137), FP32
+// out = expK diag(d): the single-slice B matrix (src/green.c:126-137)
+__global__ void k_scale_cols_d(const double *expK, const double *d, long long sdv, double *out, long long sO, int n,
+                               int batch)
+{
+    const int b = blockIdx.y;
+    if (b >= batch) return;
+    const long long nn = (long long)n * n;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < nn;
+         idx += (long long)gridDim.x * blockDim.x)
+        out[b * sO + idx] = expK[idx] * d[b * sdv + idx / n];
+}
+
 __global__ void k_scale_cols_f(const float *expK, const float *d, long long sdv, float *out,
                                long long sO, int n, int batch)
 {
@@ -463,6 +475,11 @@ struct Model {
     int *Cq = nullptr;
     unsigned long long *acc = nullptr, *att = nullptr, *acc_prev = nullptr, *att_prev = nullptr;
     int *pcount = nullptr; // pending delayed-update pairs per replica after a slice
+    // block products Bblock_j = B_{(j+1)s-1} ... B_{js} (green_build_Bblock,
+    // src/green.c:151-166), formed by the left chain after block j's
+    // Metropolis and read by the next sweep's suffix stack: M x R x n^2
+    double *BB = nullptr;
+    bool bb_valid = false;
     double *sign = nullptr;
     double *dmax = nullptr, *dsum = nullptr;
     unsigned long long *dcnt = nullptr;
@@ -566,6 +583,10 @@ static int model_alloc(dqmc_gpu_ctx *ctx, Model &m)
     own(m.LU_); own(m.LD); own(m.LT); own(m.Lq); own(m.CU); own(m.CD); own(m.CT); own(m.Cq);
     if (dalloc(&m.pcount, R)) return 1;
     own(m.pcount);
+    if (ctx->opts.precision != DQMC_GPU_MIXED_FP32_CHAIN) {
+        if (dalloc(&m.BB, (size_t)m.M * R * m.n * m.n)) return 1;
+        own(m.BB);
+    }
     if (dalloc(&m.acc, R) || dalloc(&m.att, R) || dalloc(&m.acc_prev, R) || dalloc(&m.att_prev, R) ||
         dalloc(&m.sign, R) || dalloc(&m.dmax, R) || dalloc(&m.dsum, R) || dalloc(&m.dcnt, R))
         return 1;
@@ -748,6 +769,41 @@ static float *bblock_f32(Model &m, int lb, int len)
     return cur;
 }
 
+// out = B_{lb+len-1} ... B_{lb} (FP64, left-accumulated as green_build_Bblock,
+// src/green.c:151-166): B_lb = expK diag(d_lb), then X <- expK diag(d_l) X.
+// The product alternates between `out` and W2 so that it ends in `out`.
+static int bblock_build(Model &m, int lb, int len, double *out)
+{
+    const int n = m.n, R = m.nrep;
+    const long long nn = (long long)n * n, sV = (long long)m.L * n;
+    double *bufs[2] = {out, m.w.W2};
+    int cur = (len - 1) % 2; // the first buffer so that the last product lands in out
+    hipLaunchKernelGGL(k_scale_cols_d, grid2(nn, R), dim3(256), 0, m.w.s, m.expK, m.dvec + (long long)lb * n, sV,
+                       bufs[cur], nn, n, R);
+    if (launch_check("k_scale_cols_d")) return 1;
+    for (int l = lb + 1; l < lb + len; l++) {
+        GemmArgs<double> g = gemm_args<double>(n, n, n, R, m.expK, n, 0, 0, bufs[cur], n, nn, 0, bufs[cur ^ 1], n, nn);
+        g.br = vec<double>(m.dvec + (long long)l * n, sV);
+        if (gemm(m.w, g)) return 1;
+        cur ^= 1;
+    }
+    return 0;
+}
+
+// every block product from the current fields (after the host wrote fields
+// or a single step changed them)
+static int bblock_refresh(Model &m)
+{
+    if (!m.BB) return 0;
+    const long long nn = (long long)m.n * m.n;
+    for (int j = 0; j < m.M; j++) {
+        const int lb = j * m.stab;
+        if (bblock_build(m, lb, std::min(m.stab, m.L - lb), m.BB + (long long)j * m.nrep * nn)) return 1;
+    }
+    m.bb_valid = true;
+    return 0;
+}
+
 // left <- B_{lb+len-1} ... B_{lb} left   (udv_lmul_work, src/linalg.c:515-642)
 static int stage_lmul(dqmc_gpu_ctx *ctx, Model &m, const Udv &lf, int lb, int len)
 {
@@ -759,6 +815,16 @@ static int stage_lmul(dqmc_gpu_ctx *ctx, Model &m, const Udv &lf, int lb, int le
         float *Bf = bblock_f32(m, lb, len);
         hipLaunchKernelGGL(k_f2d, grid2(nn, R), dim3(256), 0, m.w.s, Bf, nn, m.w.W3, nn, nn, R);
         GemmArgs<double> g = gemm_args<double>(n, n, n, R, m.w.W3, n, nn, 0, lf.U, n, lf.sU, 0, m.w.W1, n, nn);
+        g.bc = vec(lf.D, lf.sD);
+        if (gemm(m.w, g)) return 1;
+        cur = m.w.W1;
+    } else if (m.BB && lb % m.stab == 0 && len == std::min(m.stab, m.L - lb)) {
+        // Bblock_j = B_{lb+len-1} ... B_{lb}, left-accumulated (src/green.c:151-166),
+        // kept for the next sweep's suffix stack; then (Bblock U) diag(D)
+        const int j = lb / m.stab;
+        double *Bj = m.BB + (long long)j * R * nn;
+        if (bblock_build(m, lb, len, Bj)) return 1;
+        GemmArgs<double> g = gemm_args<double>(n, n, n, R, Bj, n, nn, 0, lf.U, n, lf.sU, 0, m.w.W1, n, nn);
         g.bc = vec(lf.D, lf.sD);
         if (gemm(m.w, g)) return 1;
         cur = m.w.W1;
@@ -798,6 +864,13 @@ static int stage_rmul(dqmc_gpu_ctx *ctx, Model &m, int j)
         float *Bf = bblock_f32(m, bj, bj1 - bj);
         hipLaunchKernelGGL(k_f2d, grid2(nn, R), dim3(256), 0, m.w.s, Bf, nn, m.w.W3, nn, nn, R);
         GemmArgs<double> g = gemm_args<double>(n, n, n, R, s1.T, n, s1.sU, 0, m.w.W3, n, nn, 0, m.w.W1, n, nn);
+        g.ar = vec(s1.D, s1.sD);
+        if (gemm(m.w, g)) return 1;
+        cur = m.w.W1;
+    } else if (m.BB && m.bb_valid) {
+        // diag(D) T Bblock_j with the block product the left chain formed
+        GemmArgs<double> g = gemm_args<double>(n, n, n, R, s1.T, n, s1.sU, 0, m.BB + (long long)j * R * nn, n, nn, 0,
+                                               m.w.W1, n, nn);
         g.ar = vec(s1.D, s1.sD);
         if (gemm(m.w, g)) return 1;
         cur = m.w.W1;
@@ -887,6 +960,7 @@ static int sweep_once(dqmc_gpu_ctx *ctx, Model &m)
 
 static int launch_sweep(dqmc_gpu_ctx *ctx, Model &m)
 {
+    if (m.BB && !m.bb_valid && bblock_refresh(m)) return 1;
     if (!ctx->opts.use_graph) return sweep_once(ctx, m);
     if (!m.gexec) {
         HIPCHK(hipStreamBeginCapture(m.w.s, hipStreamCaptureModeThreadLocal));
@@ -1142,7 +1216,8 @@ int dqmc_gpu_init(dqmc_gpu_ctx *ctx)
     for (Model *m : ctx->models) {
         if (m->nrep == 0) continue;
         if (!m->allocated && model_alloc(ctx, *m)) return 1;
-        if (stage_from_scratch(ctx, *m, 0)) return 1;
+        if (stage_from_scratch(ctx, *m, 0)) return 1; // also forms every block product
+        m->bb_valid = m->BB != nullptr;
     }
     if (sync_all(ctx)) return 1;
     ctx->initialised = true;
@@ -1343,6 +1418,7 @@ int dqmc_gpu_write_field(dqmc_gpu_ctx *ctx, int replica, const signed char *s)
         di[k] = m->expvinv[s[k] > 0];
     }
     HIPCHK(hipMemcpy(m->field + Ln * r, s, Ln, hipMemcpyHostToDevice));
+    m->bb_valid = false;
     HIPCHK(hipMemcpy(m->dvec + Ln * r, dv.data(), Ln * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(m->dinv + Ln * r, di.data(), Ln * 8, hipMemcpyHostToDevice));
     if (m->dvecf) {
@@ -1375,7 +1451,10 @@ int dqmc_gpu_step(dqmc_gpu_ctx *ctx, int step, int arg)
         }
         int rc;
         switch (step) {
-        case DQMC_GPU_STEP_PROPOSE_SLICE: rc = stage_metro(ctx, *m, arg); break;
+        case DQMC_GPU_STEP_PROPOSE_SLICE:
+            rc = stage_metro(ctx, *m, arg);
+            m->bb_valid = false;
+            break;
         case DQMC_GPU_STEP_WRAP: rc = stage_wrap(ctx, *m, arg); break;
         case DQMC_GPU_STEP_REBUILD:
             if (arg < 0 || arg >= m->L) {
@@ -1405,7 +1484,8 @@ int dqmc_gpu_profile_sweep(dqmc_gpu_ctx *ctx, int nsweep, double *ms, int *launc
     for (int s = 0; s < nsweep && !rc; s++)
         for (Model *m : ctx->models) {
             if (m->nrep == 0) continue;
-            rc = sweep_once(ctx, *m);
+            rc = (m->BB && !m->bb_valid) ? bblock_refresh(*m) : 0;
+            if (!rc) rc = sweep_once(ctx, *m);
             if (!rc && m->nbin > 0) rc = stage_measure(*m, 0);
             if (rc) break;
         }
