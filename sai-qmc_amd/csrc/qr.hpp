@@ -360,21 +360,24 @@ __device__ void qr_tfactor(const double *P, int ld, int nb, int m, double *Tl, d
     for (int idx = threadIdx.x; idx < QR_NB * QR_NB; idx += blockDim.x) Y[idx] += Y2[idx];
     __syncthreads();
     if (wave == 0 && lane < QR_NB) {
-        // lane c: back substitution for column c of T in registers, fully
-        // unrolled (t[k] = 0 for k > c keeps the sums exact); the Y and tau
-        // reads are lane-uniform and independent of the chain
-        const int c = lane;
-        const bool live = c < nb && Tl[c + c * QR_NB] != 0.0;
-        double t[QR_NB];
+        // lane r builds row r of T column by column (dlarft, forward,
+        // columnwise): T(r, i) = -tau_i sum_{k<i} T(r, k) Y(k, i), T(i, i) =
+        // tau_i; column i of Y is one broadcast read per pair of entries and
+        // the sum runs as four independent chains
+        const int r = lane;
+        double trow[QR_NB];
 #pragma unroll
-        for (int i = QR_NB - 1; i >= 0; i--) {
-            double s = (i == c) ? 1.0 : 0.0;
+        for (int i = 0; i < QR_NB; i++) {
+            const double tau = Tl[i + i * QR_NB];
+            double s4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int k = i + 1; k < QR_NB; k++) s -= Y[i + k * QR_NB] * t[k];
-            t[i] = (i <= c && live) ? s * Tl[i + i * QR_NB] : 0.0;
+            for (int k = 0; k < i; k++) s4[k & 3] += trow[k] * Y[k + i * QR_NB];
+            const double s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+            trow[i] = (r < i) ? -tau * s : (r == i ? tau : 0.0);
+            __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
-        for (int i = 0; i < QR_NB; i++) Ts[i + c * QR_NB] = t[i];
+        for (int i = 0; i < QR_NB; i++) Ts[r + i * QR_NB] = trow[i];
     }
     __syncthreads();
     for (int idx = threadIdx.x; idx < QR_NB * QR_NB; idx += blockDim.x) Tl[idx] = Ts[idx];
@@ -430,10 +433,24 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_udv(QrArgs a)
         if (!isfinite(d) || d == 0.0) s_bad = (s_bad == 1) ? 1 : 2;
     }
     if (a.Rn) {
+        // Rn = D^-1 R, eight elements per thread in flight
         double *Rn = a.Rn + (long long)b * a.sR;
-        for (long long idx = tid; idx < (long long)n * n; idx += blockDim.x) {
-            const int i = (int)(idx % n), j = (int)(idx / n);
-            Rn[idx] = (i <= j) ? A[idx] / A[i + (long long)i * n] : 0.0;
+        const long long nn = (long long)n * n;
+        for (long long base = tid; base < nn; base += 8LL * blockDim.x) {
+            double v[8], d[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const long long idx = base + (long long)u * blockDim.x;
+                const int i = (int)(idx % n), j = (int)(idx / n);
+                const bool up = idx < nn && i <= j;
+                v[u] = up ? A[idx] : 0.0;
+                d[u] = up ? A[i + (long long)i * n] : 1.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const long long idx = base + (long long)u * blockDim.x;
+                if (idx < nn) Rn[idx] = v[u] / d[u];
+            }
         }
     }
     __syncthreads();
