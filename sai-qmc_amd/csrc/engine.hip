@@ -449,6 +449,7 @@ static int udv_inv_one_plus(Work &w, const Udv &u, double *Gout, long long sG)
 
 struct Model {
     int n = 0, L = 0, stab = 0, M = 0, W = 0, Wf = 0, nrep = 0, first = 0, cap = 0;
+    int msplit = 1; // Metropolis launches (site segments, each with its flush) per slice
     int Lx = 1, Ly = 1, nq = 0, nbin = 0;
     double U = 0, dtau = 0, mu = 0, lambda = 0;
     double expv[2], expvinv[2], nu[2], nd[2];
@@ -686,46 +687,56 @@ static int stage_metro(dqmc_gpu_ctx *ctx, Model &m, int l)
     ProfScope ps_(DQMC_GPU_PROF_METRO, m.w.s);
     const int n = m.n;
     const long long nn = (long long)n * n, sV = (long long)m.L * n;
-    if (is_mixed(ctx)) {
-        MetroArgs<float> a;
-        a.n = n; a.L = m.L; a.W = m.Wf; a.l = l; a.batch = m.nrep;
-        a.G = m.Gf; a.sG = nn; a.field = m.field; a.sF = sV;
-        a.dvec = m.dvecf; a.dinv = m.dinvf; a.sV = sV;
-        a.e[0] = (float)m.expv[0]; a.e[1] = (float)m.expv[1];
-        a.einv[0] = (float)m.expvinv[0]; a.einv[1] = (float)m.expvinv[1];
-        a.dvec64 = m.dvec; a.dinv64 = m.dinv;
-        a.e64[0] = m.expv[0]; a.e64[1] = m.expv[1];
-        a.einv64[0] = m.expvinv[0]; a.einv64[1] = m.expvinv[1];
-        a.rng = m.rng;
-        a.nu[0] = m.nu[0]; a.nu[1] = m.nu[1]; a.nd[0] = m.nd[0]; a.nd[1] = m.nd[1];
-        a.acc = m.acc; a.att = m.att; a.sign = m.sign; a.status = m.w.status;
-        a.Cp = m.w.F1; a.Vp = m.w.F2; a.sP = nn; a.pcount = m.pcount;
-        launch_metro_w<float>(a, m.w.s);
-        if (launch_check("k_metro_slice")) return 1;
-        GemmArgs<float> g = gemm_args<float>(n, n, n, m.nrep, m.w.F1, n, nn, 0, m.w.F2, n, nn, 1, m.Gf, n, nn);
-        g.alpha = -1.0f;
-        g.beta = 1.0f;
-        g.Kb = m.pcount;
-        return gemm(m.w, g);
+    const int W = is_mixed(ctx) ? m.Wf : m.W;
+    // segment boundaries on window multiples
+    const int nwin = (n + W - 1) / W, per = (nwin + m.msplit - 1) / m.msplit;
+    for (int s0 = 0; s0 < n; s0 += per * W) {
+        const int s1 = std::min(n, s0 + per * W);
+        if (is_mixed(ctx)) {
+            MetroArgs<float> a;
+            a.n = n; a.L = m.L; a.W = m.Wf; a.l = l; a.batch = m.nrep; a.s0 = s0; a.s1 = s1;
+            a.G = m.Gf; a.sG = nn; a.field = m.field; a.sF = sV;
+            a.dvec = m.dvecf; a.dinv = m.dinvf; a.sV = sV;
+            a.e[0] = (float)m.expv[0]; a.e[1] = (float)m.expv[1];
+            a.einv[0] = (float)m.expvinv[0]; a.einv[1] = (float)m.expvinv[1];
+            a.dvec64 = m.dvec; a.dinv64 = m.dinv;
+            a.e64[0] = m.expv[0]; a.e64[1] = m.expv[1];
+            a.einv64[0] = m.expvinv[0]; a.einv64[1] = m.expvinv[1];
+            a.rng = m.rng;
+            a.nu[0] = m.nu[0]; a.nu[1] = m.nu[1]; a.nd[0] = m.nd[0]; a.nd[1] = m.nd[1];
+            a.acc = m.acc; a.att = m.att; a.sign = m.sign; a.status = m.w.status;
+            a.Cp = m.w.F1; a.Vp = m.w.F2; a.sP = nn; a.pcount = m.pcount;
+            launch_metro_w<float>(a, m.w.s);
+            if (launch_check("k_metro_slice")) return 1;
+            GemmArgs<float> g =
+                gemm_args<float>(n, n, n, m.nrep, m.w.F1, n, nn, 0, m.w.F2, n, nn, 1, m.Gf, n, nn);
+            g.alpha = -1.0f;
+            g.beta = 1.0f;
+            g.Kb = m.pcount;
+            if (gemm(m.w, g)) return 1;
+        } else {
+            MetroArgs<double> a;
+            a.n = n; a.L = m.L; a.W = m.W; a.l = l; a.batch = m.nrep; a.s0 = s0; a.s1 = s1;
+            a.G = m.G; a.sG = nn; a.field = m.field; a.sF = sV;
+            a.dvec = m.dvec; a.dinv = m.dinv; a.sV = sV;
+            a.e[0] = m.expv[0]; a.e[1] = m.expv[1];
+            a.einv[0] = m.expvinv[0]; a.einv[1] = m.expvinv[1];
+            a.dvec64 = nullptr; a.dinv64 = nullptr;
+            a.rng = m.rng;
+            a.nu[0] = m.nu[0]; a.nu[1] = m.nu[1]; a.nd[0] = m.nd[0]; a.nd[1] = m.nd[1];
+            a.acc = m.acc; a.att = m.att; a.sign = m.sign; a.status = m.w.status;
+            a.Cp = m.w.W3; a.Vp = m.w.W4; a.sP = nn; a.pcount = m.pcount;
+            launch_metro_w<double>(a, m.w.s);
+            if (launch_check("k_metro_slice")) return 1;
+            GemmArgs<double> g =
+                gemm_args<double>(n, n, n, m.nrep, m.w.W3, n, nn, 0, m.w.W4, n, nn, 1, m.G, n, nn);
+            g.alpha = -1.0;
+            g.beta = 1.0;
+            g.Kb = m.pcount;
+            if (gemm(m.w, g)) return 1;
+        }
     }
-    MetroArgs<double> a;
-    a.n = n; a.L = m.L; a.W = m.W; a.l = l; a.batch = m.nrep;
-    a.G = m.G; a.sG = nn; a.field = m.field; a.sF = sV;
-    a.dvec = m.dvec; a.dinv = m.dinv; a.sV = sV;
-    a.e[0] = m.expv[0]; a.e[1] = m.expv[1];
-    a.einv[0] = m.expvinv[0]; a.einv[1] = m.expvinv[1];
-    a.dvec64 = nullptr; a.dinv64 = nullptr;
-    a.rng = m.rng;
-    a.nu[0] = m.nu[0]; a.nu[1] = m.nu[1]; a.nd[0] = m.nd[0]; a.nd[1] = m.nd[1];
-    a.acc = m.acc; a.att = m.att; a.sign = m.sign; a.status = m.w.status;
-    a.Cp = m.w.W3; a.Vp = m.w.W4; a.sP = nn; a.pcount = m.pcount;
-    launch_metro_w<double>(a, m.w.s);
-    if (launch_check("k_metro_slice")) return 1;
-    GemmArgs<double> g = gemm_args<double>(n, n, n, m.nrep, m.w.W3, n, nn, 0, m.w.W4, n, nn, 1, m.G, n, nn);
-    g.alpha = -1.0;
-    g.beta = 1.0;
-    g.Kb = m.pcount;
-    return gemm(m.w, g);
+    return 0;
 }
 
 // G <- B_l G B_l^-1 = expK diag(d_l) G diag(1/d_l) expKinv (src/green.c:509-527)
@@ -1113,6 +1124,8 @@ int dqmc_gpu_add_model(dqmc_gpu_ctx *ctx, int n, int L, int stab, int delay_k, c
     m->Lx = Lx; m->Ly = Ly; m->U = U; m->dtau = dtau; m->mu = mu; m->lambda = lambda;
     m->cap = delay_k;
     m->W = metro_window(n, delay_k > 0 ? delay_k : 16, sizeof(double));
+    m->msplit = n >= 128 ? 2 : 1;
+    if (const char *e = getenv("DQMC_GPU_METRO_SPLIT")) m->msplit = std::max(1, std::min(8, atoi(e)));
     m->Wf = metro_window(n, delay_k > 0 ? delay_k : 32, sizeof(float));
     // expv for sigma = +1 (src/green.c:63-67), N = exp(-2 lambda sigma s) - 1 (src/field.c:10-17)
     const double c = -dtau * U / 2.0;

@@ -18,7 +18,9 @@
 //      thread), and appends them to C, V.
 // The rank-P flush G -= C V^T (green_delay_flush / la_gemm_nt_rect,
 // src/green.c:462-475, src/linalg.c:209-219) is one batched GEMM after the
-// kernel, with P per replica in `pcount`.
+// kernel, with P per replica in `pcount`.  A launch covers the sites
+// [s0, s1) of the slice: splitting a slice into segments, each followed by
+// its flush, bounds P and with it the correction work of later windows.
 //
 // The ratio is R = (1 + (1 - G~_ii) N_up)(1 + G~_ii N_dn)
 // (src/green.c:401-416), always evaluated in FP64, accepted iff
@@ -29,6 +31,7 @@
 
 template <typename T> struct MetroArgs {
     int n, L, W, l, batch;
+    int s0, s1;         // sites [s0, s1) of the slice in this launch
     const T *G;
     long long sG;
     signed char *field; // [L][n] per replica
@@ -206,15 +209,15 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
         uint64_t st[4];
         uint64_t *rs = a.rng + 4 * (long long)b;
         for (int q = 0; q < 4; q++) st[q] = rs[q];
-        for (int i = 0; i < n; i++) draws[i] = xoshiro_double(st);
+        for (int i = a.s0; i < a.s1; i++) draws[i] = xoshiro_double(st);
         for (int q = 0; q < 4; q++) rs[q] = st[q];
     }
     __syncthreads();
 
     int naccept = 0, negflips = 0, P = 0;
     T gc[W], gr[W];
-    for (int i0 = 0; i0 < n; i0 += W) {
-        const int wn = min(W, n - i0);
+    for (int i0 = a.s0; i0 < a.s1; i0 += W) {
+        const int wn = min(W, a.s1 - i0);
         // 1. corrected window columns -> Sc, rows -> Sr, the window block -> Bw
         for (int idx = threadIdx.x; idx < P * W; idx += blockDim.x) {
             const int q = idx / W, w = idx % W;
@@ -335,7 +338,7 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
     if (j == 0) {
         a.pcount[b] = P;
         a.acc[b] += (unsigned long long)naccept;
-        a.att[b] += (unsigned long long)n;
+        a.att[b] += (unsigned long long)(a.s1 - a.s0);
         if (negflips) a.sign[b] = -a.sign[b];
     }
 }
