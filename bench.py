@@ -33,21 +33,6 @@ LX, LY, U, DTAU, BETA, STAB, DELAY, NREP = 16, 16, 4.0, 0.1, 8.0, 8, 32, 128
 P32, P64 = 157.3e12, 78.6e12  # MI355X dense FP32 / FP64 matrix peaks
 
 
-def stage_flops(n, L, s, acc):
-    """Algorithmic flops per replica of one invocation of each stage class,
-    split into (FP32-class in mixed, FP64-class) work.  Counts: GEMM 2n^3,
-    Householder QR + explicit Q 8/3 n^3, unit-triangular inverse n^3/3."""
-    n3 = float(n) ** 3
-    return {
-        "metro": (2.0 * acc * n3, 0.0),                 # rank-k flushes of a*n accepted flips
-        "wrap": (4.0 * n3, 0.0),                        # B G B^-1
-        "stack": (2.0 * n3 * (s - 1), 20.0 / 3 * n3),  # B-chain; T B, QR, U Q
-        "lmul": (2.0 * n3 * (s - 1), 20.0 / 3 * n3),   # B-chain; B U, QR, R T
-        "combine": (0.0, 26.0 / 3 * n3),               # Tl Ur, QR, Ul Q, R Tr
-        "inv_one_plus": (0.0, 29.0 / 3 * n3),          # trtri, U^T T^-1, QR(M), trtri, 3 GEMMs
-    }
-
-
 def run_engine(dq, precision, steps, warmup, drift, groups=1, delay=DELAY):
     """The 128 replicas as `groups` equal models: each model has its own
     stream and sweep graph, so the groups' sweeps run concurrently."""
@@ -76,13 +61,10 @@ def run_engine(dq, precision, steps, warmup, drift, groups=1, delay=DELAY):
     return out
 
 
-KERNEL_OF_STAGE = {"metro": "k_metro_slice", "wrap": "k_gemm (wrap)", "stack": "suffix stack",
-                   "lmul": "left UDV chain", "combine": "udv_combine", "inv_one_plus": "udv_inv_one_plus"}
-
-
 def traffic_from_profile(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy kernel)."""
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter reads
+    for a device copy of known size)."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_16x16_beta8_128rep_pmc.txt")
     if not os.path.exists(path):
         return None
@@ -92,32 +74,32 @@ def traffic_from_profile(kernel):
     return None
 
 
-def roofline(prof, acc, mixed, nrep_launch=NREP):
-    """Roofline of the stage with the largest time per sweep (one kernel
-    launch per invocation for the Metropolis and wrap stages); a launch
-    covers nrep_launch replicas."""
-    n, L, s = LX * LY, int(round(BETA / DTAU)), STAB
-    fl = stage_flops(n, L, s, acc)
-    best = None
-    for k, (lo, hi) in fl.items():
-        ms, cnt = prof[k]
-        if cnt == 0:
-            continue
-        if best is None or ms > best[1]:
-            best = (k, ms, cnt, lo, hi)
-    k, ms, cnt, lo, hi = best
+def roofline_gemm(prof, mixed, nrep_launch=NREP):
+    """Roofline of the dominant kernel by total time: the batched n x n x n
+    GEMM (k_gemm_big), 2 n^3 flops per replica per launch, average launch
+    duration from HIP events around every such launch of a profiled sweep."""
+    n = LX * LY
+    ms, cnt = prof["gemm"]
     t = ms / cnt / 1e3
-    flops = (lo + hi) * nrep_launch
-    p_lo = P32 if mixed else P64
-    t_roof = lo * nrep_launch / p_lo + hi * nrep_launch / P64
-    peak = flops / t_roof
-    ach = flops / t
-    kern = KERNEL_OF_STAGE[k]
-    return {"bound": "mfma", "kernel": kern, "achieved": round(ach / 1e12, 3), "peak": round(peak / 1e12, 3),
-            "unit": "TFLOP/s", "frac": round(ach / peak, 4),
-            "traffic": traffic_from_profile(kern) if (k == "metro" and not mixed) else None,
-            "avg_launch_ms": round(t * 1e3, 4), "launches_per_sweep": cnt,
-            "flops_per_launch": flops}
+    flops = 2.0 * n ** 3 * nrep_launch
+    peak = P64
+    return {"bound": "mfma", "kernel": "k_gemm_big<double,0,0>", "achieved": round(flops / t / 1e12, 3),
+            "peak": round(peak / 1e12, 3), "unit": "TFLOP/s", "frac": round(flops / t / peak, 4),
+            "traffic": None if mixed else traffic_from_profile("k_gemm_big"),
+            "avg_launch_ms": round(t * 1e3, 4), "launches_per_sweep": cnt, "flops_per_launch": flops}
+
+
+def roofline_metro(prof, acc, nrep_launch=NREP):
+    """The Metropolis stage of one slice (proposal kernels + flush GEMMs)
+    against the delayed-update work the reference does for it: 2 a n^3
+    flops per replica (a = acceptance)."""
+    n = LX * LY
+    ms, cnt = prof["metro"]
+    t = ms / cnt / 1e3
+    flops = 2.0 * acc * n ** 3 * nrep_launch
+    return {"bound": "mfma", "stage": "metro (k_metro_slice + flush k_gemm_big<double,0,1>)",
+            "achieved": round(flops / t / 1e12, 3), "peak": round(P64 / 1e12, 3), "unit": "TFLOP/s",
+            "frac": round(flops / t / P64, 4), "avg_ms_per_slice": round(t * 1e3, 4)}
 
 
 def cpu_baseline():
@@ -209,7 +191,8 @@ def main():
         "drift_max": main_run["drift_max"],
         "drift_mean": main_run["drift_mean"],
         "acceptance": round(main_run["acc"], 4),
-        "roofline": roofline(main_run["profile"], main_run["acc"], main_name != "fp64", NREP // a.groups),
+        "roofline": roofline_gemm(main_run["profile"], main_name != "fp64", NREP // a.groups),
+        "roofline_metro": roofline_metro(main_run["profile"], main_run["acc"], NREP // a.groups),
         "profile_ms_per_sweep": {k: round(v[0], 3) for k, v in main_run["profile"].items()},
     }
     for name in names[1:]:
@@ -217,7 +200,7 @@ def main():
         line[name] = {"value": round(r["value"], 2), "ms_per_step": round(r["ms"] / a.steps, 3),
                       "over_fp64": round(r["value"] / main_run["value"], 3),
                       "drift_max": r["drift_max"], "drift_mean": r["drift_mean"],
-                      "roofline": roofline(r["profile"], r["acc"], True, NREP // a.groups),
+                      "roofline_metro": roofline_metro(r["profile"], r["acc"], NREP // a.groups),
                       "profile_ms_per_sweep": {k: round(v[0], 3) for k, v in r["profile"].items()}}
     line["cpu_baseline"] = cpu
     print(json.dumps(line))

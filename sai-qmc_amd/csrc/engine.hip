@@ -314,7 +314,7 @@ struct ProfScope {
     ProfRec r;
     hipStream_t s = nullptr;
     bool on;
-    ProfScope(int cls, hipStream_t st) : on(g_prof != nullptr)
+    ProfScope(int cls, hipStream_t st, bool enable = true) : on(enable && g_prof != nullptr)
     {
         if (!on) return;
         r.cls = cls;
@@ -388,6 +388,8 @@ static int trtri(Work &w, const double *T, long long sT, double *X)
 template <typename T> static int gemm(Work &w, GemmArgs<T> g)
 {
     g.status = w.status;
+    const bool full = !g.Kb && g.M == w.n && g.N == w.n && g.K == w.n;
+    ProfScope ps_(g.Kb ? DQMC_GPU_PROF_FLUSH : DQMC_GPU_PROF_GEMM, w.s, g.Kb || full);
     gemm_launch(g, w.s);
     return launch_check("k_gemm");
 }
