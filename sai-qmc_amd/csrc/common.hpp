@@ -118,6 +118,10 @@ template <typename T> struct Vec {
         const T v = p[(long long)b * stride + i];
         return inv ? T(1) / v : v;
     }
+    // the stored value (1 without a vector) and its use as a factor: at()
+    // split in two so the load can be issued long before the value is used
+    __device__ __forceinline__ T raw(int b, int i) const { return p ? p[(long long)b * stride + i] : T(1); }
+    __device__ __forceinline__ T fix(T v) const { return (p && inv) ? T(1) / v : v; }
 };
 
 template <typename T> static inline Vec<T> vnone() { return Vec<T>{nullptr, 0, 0}; }

@@ -30,6 +30,7 @@ template <typename T> struct GemmArgs {
     Vec<T> ar, ac, br, bc, cr, cc, dg;
     const int *status; // skip batch entries with status != 0 (may be null)
     const int *Kb;     // per-entry K <= K (may be null)
+    int vec2;          // set by gemm_launch: element pairs may be loaded as one aligned vector
 };
 
 #define GEMM_BM 64
@@ -146,30 +147,31 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs<T> g)
 #define GB_PPT (GB_K / 4)       // element pairs per thread per operand and k-tile
 #define GB_KH (GB_K / 2)        // pairs along k
 #define GB_IS (256 / GB_KH)     // rows between a thread's pairs when pairs run along k
+#define GB_LDK (GB_K + 4)       // j-major B tile (B not transposed): Bs[j*GB_LDK + k]
+#define GB_BUF (GB_K * GB_LD + GB_M * GB_LDK) // one LDS buffer: A tile + B tile
 
 template <typename T>
 static inline size_t gemm_big_lds_bytes()
 {
-    return sizeof(T) * 2 * 2 * GB_K * GB_LD;
+    return sizeof(T) * 2 * GB_BUF;
 }
 
 template <typename T> struct Two { T x, y; };
 
+// Loads elements off and off + 1 of base (16-byte aligned pair) as one
+// vector without a divergent branch: a pair outside the matrix is read from
+// base[0] instead (always valid) and zeroed later.
 template <typename T>
-__device__ __forceinline__ Two<T> ld_two(const T *p, bool ok0, bool ok1)
+__device__ __forceinline__ Two<T> ld_pair(const T *base, long long off, bool ok)
 {
     Two<T> r;
-    if (ok0 && ok1 && ((reinterpret_cast<uintptr_t>(p) & (2 * sizeof(T) - 1)) == 0)) {
-        if constexpr (sizeof(T) == 8) {
-            const double2 v = *reinterpret_cast<const double2 *>(p);
-            r.x = v.x; r.y = v.y;
-        } else {
-            const float2 v = *reinterpret_cast<const float2 *>(p);
-            r.x = v.x; r.y = v.y;
-        }
+    const T *p = base + (ok ? off : 0);
+    if constexpr (sizeof(T) == 8) {
+        const double2 v = *reinterpret_cast<const double2 *>(p);
+        r.x = v.x; r.y = v.y;
     } else {
-        r.x = ok0 ? p[0] : T(0);
-        r.y = ok1 ? p[1] : T(0);
+        const float2 v = *reinterpret_cast<const float2 *>(p);
+        r.x = v.x; r.y = v.y;
     }
     return r;
 }
@@ -192,7 +194,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm_big(GemmArgs<T> g)
 
     extern __shared__ __align__(16) unsigned char gb_raw[];
     T *lds = reinterpret_cast<T *>(gb_raw);
-    // buffer u: As = lds + u * 2 * GB_K * GB_LD, Bs = As + GB_K * GB_LD
+    // buffer u: As = lds + u * GB_BUF (k-major), Bs = As + GB_K * GB_LD (k-major
+    // when B is transposed; j-major otherwise, so that a thread's two
+    // consecutive k of one column are one 16-byte store and the fragment
+    // reads of 16 columns x 4 k spread over all banks)
     const T *A = g.A + (long long)b * g.sA;
     const T *B = g.B + (long long)b * g.sB;
     T *C = g.C + (long long)b * g.sC;
@@ -205,74 +210,102 @@ __global__ __launch_bounds__(256, 2) void k_gemm_big(GemmArgs<T> g)
     //   A, transposed:     pair i x (2*k2, 2*k2+1), k2 = tid % GB_KH, i = tid / GB_KH + GB_IS r
     //   B, not transposed: pair (2*k2, 2*k2+1) x j, k2 = tid % GB_KH, j = tid / GB_KH + GB_IS r
     //   B, transposed:     pair k x (2*j2, 2*j2+1), j2 = tid % 64, k = tid / 64 + 4 r
+    // A k-tile is read in two phases: load_tile only issues the loads (from
+    // clamped, always valid addresses, no divergent branches), so all of
+    // them are in flight during the multiplication of the previous tile;
+    // store_tile then zeroes what lies outside the matrix, applies the
+    // diagonal scalings and writes LDS.  Pairs are aligned vectors: the
+    // launcher sends a GEMM here only when every pair is (vec2).
+    const int ia = 2 * (tid & 63);             // TA = 0: row pair; TB = 1: column pair
+    const int kq = tid >> 6;                    // TA = 0 / TB = 1: k row of pair r is kq + 4 r
+    const int kh = 2 * (tid % GB_KH);           // TA = 1 / TB = 0: k pair
+    const int ih = tid / GB_KH;                 // TA = 1 / TB = 0: row / column of pair r is ih + GB_IS r
     Two<T> ra[GB_PPT], rb[GB_PPT];
+    T ska[GB_PPT], skb[GB_PPT], skp[4];         // raw k-dependent scales of the tile in flight
+    // k-independent scales, fixed per thread
     T sa_fixed[2] = {T(1), T(1)}, sb_fixed[2] = {T(1), T(1)};
+    T sa_row[GB_PPT], sb_col[GB_PPT];
     if (!TA) {
-        const int i = m0 + 2 * (tid & 63);
-        sa_fixed[0] = (i < g.M) ? g.ar.at(b, i) : T(0);
-        sa_fixed[1] = (i + 1 < g.M) ? g.ar.at(b, i + 1) : T(0);
+        sa_fixed[0] = (m0 + ia < g.M) ? g.ar.at(b, m0 + ia) : T(0);
+        sa_fixed[1] = (m0 + ia + 1 < g.M) ? g.ar.at(b, m0 + ia + 1) : T(0);
     }
     if (TB) {
-        const int j = n0 + 2 * (tid & 63);
-        sb_fixed[0] = (j < g.N) ? g.bc.at(b, j) : T(0);
-        sb_fixed[1] = (j + 1 < g.N) ? g.bc.at(b, j + 1) : T(0);
+        sb_fixed[0] = (n0 + ia < g.N) ? g.bc.at(b, n0 + ia) : T(0);
+        sb_fixed[1] = (n0 + ia + 1 < g.N) ? g.bc.at(b, n0 + ia + 1) : T(0);
     }
+#pragma unroll
+    for (int r = 0; r < GB_PPT; r++) {
+        const int i = m0 + ih + GB_IS * r, j = n0 + ih + GB_IS * r;
+        sa_row[r] = (TA && i < g.M) ? g.ar.at(b, i) : T(1);
+        sb_col[r] = (!TB && j < g.N) ? g.bc.at(b, j) : T(1);
+    }
+    const int Kc = K > 0 ? K - 1 : 0; // clamp for scale reads
 
     auto load_tile = [&](int k0) {
 #pragma unroll
         for (int r = 0; r < GB_PPT; r++) {
             if (!TA) {
-                const int i = m0 + 2 * (tid & 63), k = k0 + (tid >> 6) + 4 * r;
-                const bool kok = k < K;
-                ra[r] = ld_two(A + i + (long long)k * g.lda, kok && i < g.M, kok && i + 1 < g.M);
-                const T s = kok ? g.ac.at(b, k) : T(0);
-                ra[r].x *= sa_fixed[0] * s;
-                ra[r].y *= sa_fixed[1] * s;
+                const int i = m0 + ia, k = k0 + kq + 4 * r;
+                const bool ok0 = k < K && i < g.M, ok1 = k < K && i + 1 < g.M;
+                ra[r] = ld_pair(A, i + (long long)k * g.lda, ok0 || ok1);
+                ska[r] = g.ac.raw(b, min(k, Kc));
             } else {
-                const int i = m0 + tid / GB_KH + GB_IS * r, k = k0 + 2 * (tid % GB_KH);
-                const bool iok = i < g.M;
-                ra[r] = ld_two(A + k + (long long)i * g.lda, iok && k < K, iok && k + 1 < K);
-                const T s = iok ? g.ar.at(b, i) : T(0);
-                ra[r].x *= s * (k < K ? g.ac.at(b, k) : T(0));
-                ra[r].y *= s * (k + 1 < K ? g.ac.at(b, k + 1) : T(0));
+                const int i = m0 + ih + GB_IS * r, k = k0 + kh;
+                const bool ok0 = i < g.M && k < K, ok1 = i < g.M && k + 1 < K;
+                ra[r] = ld_pair(A, k + (long long)i * g.lda, ok0 || ok1);
             }
             if (!TB) {
-                const int j = n0 + tid / GB_KH + GB_IS * r, k = k0 + 2 * (tid % GB_KH);
-                const bool jok = j < g.N;
-                rb[r] = ld_two(B + k + (long long)j * g.ldb, jok && k < K, jok && k + 1 < K);
-                const T s = jok ? g.bc.at(b, j) : T(0);
-                rb[r].x *= (k < K ? g.br.at(b, k) : T(0)) * s;
-                rb[r].y *= (k + 1 < K ? g.br.at(b, k + 1) : T(0)) * s;
+                const int j = n0 + ih + GB_IS * r, k = k0 + kh;
+                const bool ok0 = j < g.N && k < K, ok1 = j < g.N && k + 1 < K;
+                rb[r] = ld_pair(B, k + (long long)j * g.ldb, ok0 || ok1);
             } else {
-                const int j = n0 + 2 * (tid & 63), k = k0 + (tid >> 6) + 4 * r;
-                const bool kok = k < K;
-                rb[r] = ld_two(B + j + (long long)k * g.ldb, kok && j < g.N, kok && j + 1 < g.N);
-                const T s = kok ? g.br.at(b, k) : T(0);
-                rb[r].x *= s * sb_fixed[0];
-                rb[r].y *= s * sb_fixed[1];
+                const int j = n0 + ia, k = k0 + kq + 4 * r;
+                const bool ok0 = k < K && j < g.N, ok1 = k < K && j + 1 < g.N;
+                rb[r] = ld_pair(B, j + (long long)k * g.ldb, ok0 || ok1);
+                skb[r] = g.br.raw(b, min(k, Kc));
             }
         }
+        if (TA) {
+            skp[0] = g.ac.raw(b, min(k0 + kh, Kc));
+            skp[1] = g.ac.raw(b, min(k0 + kh + 1, Kc));
+        }
+        if (!TB) {
+            skp[2] = g.br.raw(b, min(k0 + kh, Kc));
+            skp[3] = g.br.raw(b, min(k0 + kh + 1, Kc));
+        }
     };
-    auto store_tile = [&](int u) {
-        T *As = lds + u * 2 * GB_K * GB_LD;
+    auto store_tile = [&](int u, int k0) {
+        T *As = lds + u * GB_BUF;
         T *Bs = As + GB_K * GB_LD;
 #pragma unroll
         for (int r = 0; r < GB_PPT; r++) {
             if (!TA) {
-                T *d = As + ((tid >> 6) + 4 * r) * GB_LD + 2 * (tid & 63);
-                d[0] = ra[r].x; d[1] = ra[r].y;
+                const int i = m0 + ia, k = k0 + kq + 4 * r;
+                const bool kok = k < K;
+                const T s = g.ac.fix(ska[r]);
+                T *d = As + (kq + 4 * r) * GB_LD + ia;
+                d[0] = (kok && i < g.M) ? ra[r].x * (sa_fixed[0] * s) : T(0);
+                d[1] = (kok && i + 1 < g.M) ? ra[r].y * (sa_fixed[1] * s) : T(0);
             } else {
-                const int i = tid / GB_KH + GB_IS * r, k = 2 * (tid % GB_KH);
-                As[k * GB_LD + i] = ra[r].x;
-                As[(k + 1) * GB_LD + i] = ra[r].y;
+                const int i = m0 + ih + GB_IS * r, k = k0 + kh;
+                const bool iok = i < g.M;
+                As[kh * GB_LD + ih + GB_IS * r] = (iok && k < K) ? ra[r].x * (sa_row[r] * g.ac.fix(skp[0])) : T(0);
+                As[(kh + 1) * GB_LD + ih + GB_IS * r] =
+                    (iok && k + 1 < K) ? ra[r].y * (sa_row[r] * g.ac.fix(skp[1])) : T(0);
             }
             if (!TB) {
-                const int j = tid / GB_KH + GB_IS * r, k = 2 * (tid % GB_KH);
-                Bs[k * GB_LD + j] = rb[r].x;
-                Bs[(k + 1) * GB_LD + j] = rb[r].y;
+                const int j = n0 + ih + GB_IS * r, k = k0 + kh;
+                const bool jok = j < g.N;
+                T *d = Bs + (ih + GB_IS * r) * GB_LDK + kh;
+                d[0] = (jok && k < K) ? rb[r].x * (g.br.fix(skp[2]) * sb_col[r]) : T(0);
+                d[1] = (jok && k + 1 < K) ? rb[r].y * (g.br.fix(skp[3]) * sb_col[r]) : T(0);
             } else {
-                T *d = Bs + ((tid >> 6) + 4 * r) * GB_LD + 2 * (tid & 63);
-                d[0] = rb[r].x; d[1] = rb[r].y;
+                const int j = n0 + ia, k = k0 + kq + 4 * r;
+                const bool kok = k < K;
+                const T s = g.br.fix(skb[r]);
+                T *d = Bs + (kq + 4 * r) * GB_LD + ia;
+                d[0] = (kok && j < g.N) ? rb[r].x * (s * sb_fixed[0]) : T(0);
+                d[1] = (kok && j + 1 < g.N) ? rb[r].y * (s * sb_fixed[1]) : T(0);
             }
         }
     };
@@ -285,19 +318,21 @@ __global__ __launch_bounds__(256, 2) void k_gemm_big(GemmArgs<T> g)
         for (int jt = 0; jt < 4; jt++) acc[it][jt] = acc_t{0, 0, 0, 0};
 
     load_tile(0);
-    store_tile(0);
+    store_tile(0, 0);
     __syncthreads();
     int u = 0;
     for (int k0 = 0; k0 < K; k0 += GB_K) {
         const bool more = k0 + GB_K < K;
         if (more) load_tile(k0 + GB_K);
-        const T *As = lds + u * 2 * GB_K * GB_LD;
+        const T *As = lds + u * GB_BUF;
         const T *Bs = As + GB_K * GB_LD;
 #pragma unroll
         for (int kk = 0; kk < GB_K; kk += 4) {
             T fa[4], fb[4];
 #pragma unroll
-            for (int jt = 0; jt < 4; jt++) fa[jt] = Bs[(kk + lr) * GB_LD + wc * 64 + jt * 16 + lc];
+            for (int jt = 0; jt < 4; jt++)
+                fa[jt] = TB ? Bs[(kk + lr) * GB_LD + wc * 64 + jt * 16 + lc]
+                            : Bs[(wc * 64 + jt * 16 + lc) * GB_LDK + kk + lr];
 #pragma unroll
             for (int it = 0; it < 4; it++) fb[it] = As[(kk + lr) * GB_LD + wr * 64 + it * 16 + lc];
 #pragma unroll
@@ -305,7 +340,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_big(GemmArgs<T> g)
 #pragma unroll
                 for (int jt = 0; jt < 4; jt++) acc[it][jt] = Mfma<T>::mma(fa[jt], fb[it], acc[it][jt]);
         }
-        if (more) store_tile(u ^ 1);
+        if (more) store_tile(u ^ 1, k0 + GB_K);
         __syncthreads();
         u ^= 1;
     }
@@ -345,13 +380,22 @@ static inline GemmArgs<T> gemm_args(int M, int N, int K, int batch, const T *A, 
     g.ar = g.ac = g.br = g.bc = g.cr = g.cc = g.dg = vnone<T>();
     g.status = nullptr;
     g.Kb = nullptr;
+    g.vec2 = 0;
     return g;
 }
 
 template <typename T>
-static inline void gemm_launch(const GemmArgs<T> &g, hipStream_t s)
+static inline void gemm_launch(const GemmArgs<T> &g0, hipStream_t s)
 {
-    if (g.M >= GB_M && g.N >= GB_M) {
+    GemmArgs<T> g = g0;
+    {
+        const uintptr_t al = 2 * sizeof(T) - 1;
+        const bool even_k = (g.K % 2 == 0) && !g.Kb;
+        g.vec2 = ((reinterpret_cast<uintptr_t>(g.A) & al) == 0) && ((reinterpret_cast<uintptr_t>(g.B) & al) == 0) &&
+                 g.lda % 2 == 0 && g.ldb % 2 == 0 && g.sA % 2 == 0 && g.sB % 2 == 0 &&
+                 (g.transA ? even_k : g.M % 2 == 0) && (g.transB ? g.N % 2 == 0 : even_k);
+    }
+    if (g.M >= GB_M && g.N >= GB_M && g.vec2) {
         const int tb = ((g.M + GB_M - 1) / GB_M) * ((g.N + GB_M - 1) / GB_M) * g.batch;
         if (tb <= 0) return;
         const size_t lds = gemm_big_lds_bytes<T>();
