@@ -368,8 +368,12 @@ static int qr(Work &w, double *A, double *Q, long long sQ, double *Rn, long long
         hipLaunchKernelGGL(k_qr_udv<8>, dim3(w.nb), dim3(QR_THREADS), qr_lds_bytes(w.n), w.s, a);
     if (launch_check("k_qr_udv")) return 1;
     if (!Q) return 0;
-    hipLaunchKernelGGL(k_qr_formq, dim3(qr_formq_blocks(w.n, w.nb)), dim3(FQ_WAVES * 64), qr_formq_lds_bytes(w.n),
-                       w.s, a);
+    const dim3 fg(qr_formq_blocks(w.n, w.nb)), fb(FQ_WAVES * 64);
+    const size_t fl = qr_formq_lds_bytes(w.n);
+    if (w.n <= 64) hipLaunchKernelGGL(k_qr_formq_reg<16>, fg, fb, fl, w.s, a);
+    else if (w.n <= 128) hipLaunchKernelGGL(k_qr_formq_reg<32>, fg, fb, fl, w.s, a);
+    else if (w.n <= 256) hipLaunchKernelGGL(k_qr_formq_reg<64>, fg, fb, fl, w.s, a);
+    else hipLaunchKernelGGL(k_qr_formq, fg, fb, fl, w.s, a);
     return launch_check("k_qr_formq");
 }
 
@@ -1022,6 +1026,8 @@ static int set_lds_limits()
                              (const void *)k_metro_slice<float, 16>,  (const void *)k_metro_slice<float, 8>,
                              (const void *)k_qr_udv<1>, (const void *)k_qr_udv<2>, (const void *)k_qr_udv<4>, (const void *)k_qr_udv<8>,
                              (const void *)k_trtri_unit, (const void *)k_measure, (const void *)k_qr_formq,
+                             (const void *)k_qr_formq_reg<16>, (const void *)k_qr_formq_reg<32>,
+                             (const void *)k_qr_formq_reg<64>,
                              (const void *)k_gemm_big<double, 0, 0>, (const void *)k_gemm_big<double, 1, 0>,
                              (const void *)k_gemm_big<double, 0, 1>, (const void *)k_gemm_big<double, 1, 1>,
                              (const void *)k_gemm_big<float, 0, 0>, (const void *)k_gemm_big<float, 1, 0>,

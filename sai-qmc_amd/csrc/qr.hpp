@@ -530,6 +530,130 @@ __global__ __launch_bounds__(FQ_WAVES * 64) void k_qr_formq(QrArgs a)
     }
 }
 
+// Register-resident variant for n <= 4*NU: each wave keeps its 16-column
+// block of Q in registers for all panels (lane (lr, lc) holds rows 4u + lr
+// of column j0 + lc, u < NU: the MFMA B layout, which for FP64 16x16x4 is
+// also the accumulator layout of 16-row tiles), starting from the columns
+// of I, so Q is read from nowhere and written once.  Per panel: W = V^T X
+// on MFMA, W' = T W, X -= V W'.
+template <int NU>
+__global__ __launch_bounds__(FQ_WAVES * 64) void k_qr_formq_reg(QrArgs a)
+{
+    const int n = a.n;
+    const int nblk = (n + 15) / 16;
+    const int wg_per = (nblk + FQ_WAVES - 1) / FQ_WAVES;
+    const int b = blockIdx.x / wg_per, g = blockIdx.x % wg_per;
+    if (b >= a.batch) return;
+    if (a.status && a.status[b]) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lc = lane & 15, lr = lane >> 4;
+    const double *A = a.A + (long long)b * a.sA;
+    const double *Tw = a.Tws + (long long)b * a.sT;
+    double *Q = a.Q + (long long)b * a.sQ;
+    extern __shared__ double lds[];
+    const int ld = n + 1;
+    double *P = lds;
+    double *Tl = P + (size_t)QR_NB * ld;
+    double *Wl = Tl + QR_NB * QR_NB + wave * QR_NB * 16;
+    const int jb = g * FQ_WAVES + wave;
+    const int j0 = 16 * jb;
+    const int j = j0 + lc;
+    const int jlast = min(n, 16 * (g + 1) * FQ_WAVES) - 1;
+    double xb[NU];
+#pragma unroll
+    for (int u = 0; u < NU; u++) xb[u] = (4 * u + lr == j) ? 1.0 : 0.0;
+    const int npan = (n + QR_NB - 1) / QR_NB;
+    for (int pi = npan - 1; pi >= 0; pi--) {
+        const int p0 = pi * QR_NB;
+        if (jlast < p0) continue; // uniform over the workgroup
+        const int nb = min(QR_NB, n - p0);
+        const int m = n - p0;
+        __syncthreads();
+        for (int base = tid; base < nb * m; base += 8 * blockDim.x) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int idx = base + u * blockDim.x;
+                v[u] = idx < nb * m ? A[(p0 + idx % m) + (long long)(p0 + idx / m) * n] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int idx = base + u * blockDim.x;
+                if (idx < nb * m) P[(idx / m) * ld + idx % m] = v[u];
+            }
+        }
+        for (int idx = tid; idx < QR_NB * QR_NB; idx += blockDim.x) Tl[idx] = Tw[pi * QR_NB * QR_NB + idx];
+        __syncthreads();
+        if (jb >= nblk || j0 < p0) continue; // columns j < p0 are still e_j below row p0
+        // W = V^T X[p0:, block] (rows k >= p0 of X; V(k, c) = panel column c, row k - p0)
+        f64x4 w0 = {0, 0, 0, 0}, w1 = {0, 0, 0, 0};
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+            if (4 * u >= p0 && 4 * u < n) {
+                const int k = 4 * u + lr - p0;
+                const double a0 = (lc < nb && k < m) ? panel_v(P, ld, lc, k) : 0.0;
+                const double a1 = (16 + lc < nb && k < m) ? panel_v(P, ld, 16 + lc, k) : 0.0;
+                w0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, xb[u], w0, 0, 0, 0);
+                w1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, xb[u], w1, 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int r = Mfma<double>::row(lane, v);
+            Wl[r * 16 + lc] = w0[v];
+            Wl[(16 + r) * 16 + lc] = w1[v];
+        }
+        __builtin_amdgcn_s_waitcnt(0xc07f);
+        __builtin_amdgcn_wave_barrier();
+        // W' = T W
+        f64x4 u0 = {0, 0, 0, 0}, u1 = {0, 0, 0, 0};
+#pragma unroll
+        for (int k0 = 0; k0 < QR_NB; k0 += 4) {
+            const int k = k0 + lr;
+            const double bw = Wl[k * 16 + lc];
+            u0 = __builtin_amdgcn_mfma_f64_16x16x4f64(Tl[lc + k * QR_NB], bw, u0, 0, 0, 0);
+            u1 = __builtin_amdgcn_mfma_f64_16x16x4f64(Tl[(16 + lc) + k * QR_NB], bw, u1, 0, 0, 0);
+        }
+        __builtin_amdgcn_s_waitcnt(0xc07f);
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int r = Mfma<double>::row(lane, v);
+            Wl[r * 16 + lc] = u0[v];
+            Wl[(16 + r) * 16 + lc] = u1[v];
+        }
+        __builtin_amdgcn_s_waitcnt(0xc07f);
+        __builtin_amdgcn_wave_barrier();
+        double wv[QR_NB / 4];
+#pragma unroll
+        for (int kk = 0; kk < QR_NB / 4; kk++) wv[kk] = Wl[(4 * kk + lr) * 16 + lc];
+        // X[16t : 16t+16] -= V W' for the row tiles t at or below p0
+#pragma unroll
+        for (int t = 0; t < NU / 4; t++) {
+            if (16 * t >= p0 && 16 * t < n) {
+                f64x4 c = {xb[4 * t], xb[4 * t + 1], xb[4 * t + 2], xb[4 * t + 3]};
+                const int r = 16 * t + lc - p0;
+#pragma unroll
+                for (int kk = 0; kk < QR_NB / 4; kk++) {
+                    const int k = 4 * kk + lr;
+                    const double av = (k < nb && r < m) ? -panel_v(P, ld, k, r) : 0.0;
+                    c = __builtin_amdgcn_mfma_f64_16x16x4f64(av, wv[kk], c, 0, 0, 0);
+                }
+#pragma unroll
+                for (int v = 0; v < 4; v++) xb[4 * t + v] = c[v];
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (jb < nblk && j < n) {
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+            const int r = 4 * u + lr;
+            if (r < n) Q[r + (long long)j * n] = xb[u];
+        }
+    }
+}
+
 static inline size_t qr_formq_lds_bytes(int n)
 {
     return sizeof(double) * ((size_t)QR_NB * (n + 1) + QR_NB * QR_NB + FQ_WAVES * QR_NB * 16);
