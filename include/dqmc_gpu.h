@@ -149,7 +149,7 @@ int dqmc_gpu_last_elapsed_ms(dqmc_gpu_ctx *ctx, double *ms);
 #define DQMC_GPU_PROF_ACCEPT 6  /* drift bookkeeping and copy of the rebuilt G */
 #define DQMC_GPU_PROF_MEASURE 7 /* tau = 0 measurement into bin 0 */
 #define DQMC_GPU_PROF_QR 8      /* every QR (also counted in its stage) */
-#define DQMC_GPU_PROF_TRTRI 9   /* every triangular inverse (also in INV) */
+#define DQMC_GPU_PROF_TRSM 9    /* every triangular solve (also in INV) */
 #define DQMC_GPU_PROF_GEMM 10   /* every n x n x n GEMM launch (also in its stage) */
 #define DQMC_GPU_PROF_FLUSH 11  /* every delayed-update flush GEMM (also in METRO) */
 #define DQMC_GPU_PROF_N 12

@@ -137,25 +137,42 @@ __global__ void k_scale_cols_f(const float *expK, const float *d, long long sdv,
 __global__ void k_build_N(const double *U, const double *D, const double *T, long long sU, long long sD, double *N,
                           double *X, int *bigsign, int n, int batch, const int *status)
 {
-    const int b = blockIdx.x;
+    const int b = blockIdx.y;
     if (b >= batch || status[b]) return;
-    __shared__ int neg;
-    if (threadIdx.x == 0) neg = 0;
-    __syncthreads();
     const double *Ub = U + b * sU, *Tb = T + b * sU, *Db = D + b * sD;
     double *Nb = N + (long long)b * n * n, *Xb = X + (long long)b * n * n;
-    for (long long idx = threadIdx.x; idx < (long long)n * n; idx += blockDim.x) {
-        const int i = (int)(idx % n), j = (int)(idx / n);
-        const double d = Db[i];
-        const bool big = fabs(d) > 1.0;
-        const double x = big ? Ub[j + (long long)i * n] / d : Ub[j + (long long)i * n];
-        Xb[idx] = x;
-        Nb[idx] = x + (big ? 1.0 : d) * Tb[idx];
+    const long long nn = (long long)n * n, stride = (long long)gridDim.x * blockDim.x;
+    for (long long base = blockIdx.x * (long long)blockDim.x + threadIdx.x; base < nn; base += 4 * stride) {
+        double u[4], t[4], d[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const long long idx = base + k * stride;
+            const int i = (int)(idx % n), j = (int)(idx / n);
+            const bool ok = idx < nn;
+            u[k] = ok ? Ub[j + (long long)i * n] : 0.0;
+            t[k] = ok ? Tb[idx] : 0.0;
+            d[k] = ok ? Db[i] : 1.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const long long idx = base + k * stride;
+            if (idx < nn) {
+                const bool big = fabs(d[k]) > 1.0;
+                const double x = big ? u[k] / d[k] : u[k];
+                Xb[idx] = x;
+                Nb[idx] = x + (big ? 1.0 : d[k]) * t[k];
+            }
+        }
     }
-    for (int i = threadIdx.x; i < n; i += blockDim.x)
-        if (fabs(Db[i]) > 1.0 && Db[i] < 0.0) atomicXor(&neg, 1);
-    __syncthreads();
-    if (threadIdx.x == 0) bigsign[b] = neg ? -1 : 1;
+    if (blockIdx.x == 0) {
+        __shared__ int neg;
+        if (threadIdx.x == 0) neg = 0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += blockDim.x)
+            if (fabs(Db[i]) > 1.0 && Db[i] < 0.0) atomicXor(&neg, 1);
+        __syncthreads();
+        if (threadIdx.x == 0) bigsign[b] = neg ? -1 : 1;
+    }
 }
 
 // det sign of 1 + U D T = sgn(prod_big D) sgn det U sgn det M, with
@@ -197,47 +214,67 @@ __global__ void k_mul_q(const int *a, long long sa, const int *b_, long long sb,
 // Replace G by the rebuilt Gnew; optionally record max|G - Gnew| as the
 // stabilisation drift (dqmc_record_stab_drift, src/dqmc.c:64-98).  In mixed
 // precision the wrapped G lives in Gf and the FP32 copy is refreshed.
-__global__ void k_drift_copy(double *G, float *Gf, const double *Gnew, long long sG, int n, int batch,
-                             int drift, int from_f32, double *dmax, double *dsum,
-                             unsigned long long *dcnt, int *status, int *reason, double *sign)
+// G <- Gnew after a rebuild, with the drift max|G - Gnew| and a non-finite
+// check; many blocks per replica, results gathered with global atomics
+// (the max of non-negative doubles is the max of their bit patterns) and
+// finished by k_drift_finish.
+__global__ void k_drift_copy(double *G, float *Gf, const double *Gnew, long long sG, int n, int batch, int drift,
+                             int from_f32, unsigned long long *tmax, int *tbad, const int *status)
 {
-    const int b = blockIdx.x;
+    const int b = blockIdx.y;
     if (b >= batch || status[b]) return;
-    // PH path: the running sign is reset to the exact +1 at every rebuild
-    // (src/dqmc.c:740-742, 799-801)
-    if (threadIdx.x == 0 && sign) sign[b] = 1.0;
-    __shared__ double red[8];
-    __shared__ int bad;
-    if (threadIdx.x == 0) bad = 0;
-    __syncthreads();
+    const long long nn = (long long)n * n, stride = (long long)gridDim.x * blockDim.x;
     double m = 0.0;
-    for (long long idx = threadIdx.x; idx < (long long)n * n; idx += blockDim.x) {
-        const double v = Gnew[b * sG + idx];
-        if (!isfinite(v)) bad = 1;
-        if (drift) {
-            const double old = from_f32 ? (double)Gf[b * sG + idx] : G[b * sG + idx];
-            m = fmax(m, fabs(old - v));
+    int bad = 0;
+    for (long long base = blockIdx.x * (long long)blockDim.x + threadIdx.x; base < nn; base += 4 * stride) {
+        double v[4], o[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const long long idx = base + k * stride;
+            const bool ok = idx < nn;
+            v[k] = ok ? Gnew[b * sG + idx] : 0.0;
+            o[k] = (ok && drift) ? (from_f32 ? (double)Gf[b * sG + idx] : G[b * sG + idx]) : 0.0;
         }
-        G[b * sG + idx] = v;
-        if (Gf) Gf[b * sG + idx] = (float)v;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const long long idx = base + k * stride;
+            if (idx < nn) {
+                if (!isfinite(v[k])) bad = 1;
+                if (drift) m = fmax(m, fabs(o[k] - v[k]));
+                G[b * sG + idx] = v[k];
+                if (Gf) Gf[b * sG + idx] = (float)v[k];
+            }
+        }
     }
     if (drift) {
         m = wave_max(m);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+        if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(&tmax[b], (unsigned long long)__double_as_longlong(m));
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (bad) {
+    if (bad) atomicOr(&tbad[b], 1);
+}
+
+// per replica: the running sign is reset to the exact +1 at every rebuild
+// (PH path, src/dqmc.c:740-742, 799-801), failures are recorded, the drift
+// sample is accumulated, and the gather slots are cleared for the next call
+__global__ void k_drift_finish(int batch, int drift, unsigned long long *tmax, int *tbad, double *dmax, double *dsum,
+                               unsigned long long *dcnt, int *status, int *reason, double *sign)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch) return;
+    if (!status[b]) {
+        if (sign) sign[b] = 1.0;
+        if (tbad[b]) {
             status[b] = 1;
             reason[b] = DQMC_GPU_FAIL_NONFINITE;
         } else if (drift) {
-            double mm = 0.0;
-            for (int w = 0; w < (int)(blockDim.x >> 6); w++) mm = fmax(mm, red[w]);
+            const double mm = __longlong_as_double((long long)tmax[b]);
             dmax[b] = fmax(dmax[b], mm);
             dsum[b] += mm;
             dcnt[b] += 1;
         }
     }
+    tmax[b] = 0;
+    tbad[b] = 0;
 }
 
 // ---------------------------------------------------------------- buffers
@@ -377,16 +414,18 @@ static int qr(Work &w, double *A, double *Q, long long sQ, double *Rn, long long
     return launch_check("k_qr_formq");
 }
 
-static int trtri(Work &w, const double *T, long long sT, double *X)
+// X = U^-1 B, U unit upper triangular (dtrsm, left, upper, unit diagonal)
+static int trsm(Work &w, const double *U, long long sU, const double *B, long long sB, double *X, long long sX)
 {
-    ProfScope ps_(DQMC_GPU_PROF_TRTRI, w.s);
-    TrtriArgs a;
+    ProfScope ps_(DQMC_GPU_PROF_TRSM, w.s);
+    TrsmArgs a;
     a.n = w.n; a.batch = w.nb;
-    a.T = T; a.sT = sT;
-    a.X = X; a.sX = (long long)w.n * w.n;
+    a.U = U; a.sU = sU;
+    a.B = B; a.sB = sB;
+    a.X = X; a.sX = sX;
     a.status = w.status;
-    hipLaunchKernelGGL(k_trtri_unit, dim3(w.nb), dim3(256), trtri_lds_bytes(w.n), w.s, a);
-    return launch_check("k_trtri_unit");
+    hipLaunchKernelGGL(k_trsm_unit_upper, dim3(trsm_blocks(w.n, w.nb)), dim3(256), trsm_lds_bytes(w.n), w.s, a);
+    return launch_check("k_trsm_unit_upper");
 }
 
 template <typename T> static int gemm(Work &w, GemmArgs<T> g)
@@ -427,7 +466,8 @@ static int udv_combine(Work &w, const Udv &l, const Udv &r, const Udv &o)
 
 // udv_inv_one_plus_work (src/linalg.c:911-1115) in the form
 //   G = (Db^-1 U^T + Ds T)^-1 Db^-1 U^T = Rn_N^-1 D_N^-1 Q_N^T X,
-// N = Q_N D_N Rn_N by Householder QR, so no T^-1 is ever formed; the sign is
+// N = Q_N D_N Rn_N by Householder QR, so no T^-1 is ever formed and Rn_N^-1
+// is applied by a triangular solve; the sign is
 // sgn det(1 + U D T) = sgn det U * prod_{|D_i|>1} sgn D_i * sgn det N with
 // det N = det Q_N prod (D_N)_i (src/linalg.c:1039-1083 computes the same
 // quantity through the LU of the reference's M).
@@ -436,16 +476,14 @@ static int udv_inv_one_plus(Work &w, const Udv &u, double *Gout, long long sG)
     ProfScope ps_(DQMC_GPU_PROF_INV, w.s);
     const int n = w.n;
     const long long nn = (long long)n * n;
-    hipLaunchKernelGGL(k_build_N, dim3(w.nb), dim3(256), 0, w.s, u.U, u.D, u.T, u.sU, u.sD, w.W1, w.W4, w.I1, n,
+    hipLaunchKernelGGL(k_build_N, grid2((long long)n * n / 4, w.nb), dim3(256), 0, w.s, u.U, u.D, u.T, u.sU, u.sD, w.W1, w.W4, w.I1, n,
                        w.nb, w.status);
     if (launch_check("k_build_N")) return 1;                         // W1 = N, W4 = X
     if (qr(w, w.W1, w.W2, nn, w.W3, nn, w.V3, n, w.I2, 1)) return 1; // W2 = Q_N, V3 = D_N, W3 = Rn_N
-    if (trtri(w, w.W3, nn, w.W1)) return 1;                          // W1 = Rn_N^-1
-    GemmArgs<double> g = gemm_args<double>(n, n, n, w.nb, w.W2, n, nn, 1, w.W4, n, nn, 0, w.W3, n, nn);
+    GemmArgs<double> g = gemm_args<double>(n, n, n, w.nb, w.W2, n, nn, 1, w.W4, n, nn, 0, w.W1, n, nn);
     g.cr = vec<double>(w.V3, n, 1);
-    if (gemm(w, g)) return 1; // W3 = D_N^-1 Q_N^T X
-    g = gemm_args<double>(n, n, n, w.nb, w.W1, n, nn, 0, w.W3, n, nn, 0, Gout, n, sG);
-    if (gemm(w, g)) return 1; // G = Rn_N^-1 W3
+    if (gemm(w, g)) return 1;                           // W1 = D_N^-1 Q_N^T X
+    if (trsm(w, w.W3, nn, w.W1, nn, Gout, sG)) return 1; // G = Rn_N^-1 W1
     hipLaunchKernelGGL(k_sign_finish, dim3(w.nb), dim3(256), 0, w.s, w.I1, u.q, u.sq, w.I2, w.V3, n,
                        w.det_sign, w.status, w.reason, w.nb);
     return launch_check("k_sign_finish");
@@ -482,6 +520,8 @@ struct Model {
     int *Cq = nullptr;
     unsigned long long *acc = nullptr, *att = nullptr, *acc_prev = nullptr, *att_prev = nullptr;
     int *pcount = nullptr; // pending delayed-update pairs per replica after a slice
+    unsigned long long *dtmax = nullptr; // drift gather slots (k_drift_copy -> k_drift_finish)
+    int *dtbad = nullptr;
     // block products Bblock_j = B_{(j+1)s-1} ... B_{js} (green_build_Bblock,
     // src/green.c:151-166), formed by the left chain after block j's
     // Metropolis and read by the next sweep's suffix stack: M x R x n^2
@@ -588,8 +628,12 @@ static int model_alloc(dqmc_gpu_ctx *ctx, Model &m)
         dalloc(&m.CT, nn * R) || dalloc(&m.Cq, R))
         return 1;
     own(m.LU_); own(m.LD); own(m.LT); own(m.Lq); own(m.CU); own(m.CD); own(m.CT); own(m.Cq);
-    if (dalloc(&m.pcount, R)) return 1;
+    if (dalloc(&m.pcount, R) || dalloc(&m.dtmax, R) || dalloc(&m.dtbad, R)) return 1;
     own(m.pcount);
+    own(m.dtmax);
+    own(m.dtbad);
+    HIPCHK(hipMemset(m.dtmax, 0, sizeof(unsigned long long) * R));
+    HIPCHK(hipMemset(m.dtbad, 0, sizeof(int) * R));
     if (ctx->opts.precision != DQMC_GPU_MIXED_FP32_CHAIN) {
         if (dalloc(&m.BB, (size_t)m.M * R * m.n * m.n)) return 1;
         own(m.BB);
@@ -929,10 +973,13 @@ static int stage_accept_rebuild(dqmc_gpu_ctx *ctx, Model &m, int record_drift)
 {
     ProfScope ps_(DQMC_GPU_PROF_ACCEPT, m.w.s);
     const long long nn = (long long)m.n * m.n;
-    hipLaunchKernelGGL(k_drift_copy, dim3(m.nrep), dim3(256), 0, m.w.s, m.G, m.Gf, m.Gnew, nn, m.n, m.nrep,
-                       record_drift && ctx->opts.drift ? 1 : 0, is_mixed(ctx) ? 1 : 0, m.dmax, m.dsum, m.dcnt,
-                       m.w.status, m.w.reason, m.sign);
-    return launch_check("k_drift_copy");
+    const int drift = record_drift && ctx->opts.drift ? 1 : 0;
+    hipLaunchKernelGGL(k_drift_copy, grid2(nn / 4, m.nrep), dim3(256), 0, m.w.s, m.G, m.Gf, m.Gnew, nn, m.n, m.nrep,
+                       drift, is_mixed(ctx) ? 1 : 0, m.dtmax, m.dtbad, m.w.status);
+    if (launch_check("k_drift_copy")) return 1;
+    hipLaunchKernelGGL(k_drift_finish, dim3((m.nrep + 255) / 256), dim3(256), 0, m.w.s, m.nrep, drift, m.dtmax,
+                       m.dtbad, m.dmax, m.dsum, m.dcnt, m.w.status, m.w.reason, m.sign);
+    return launch_check("k_drift_finish");
 }
 
 // green_from_scratch(G, l0) (src/green.c:262-306): B_{l0-1} ... B_0 B_{L-1}
@@ -1025,7 +1072,7 @@ static int set_lds_limits()
                              (const void *)k_metro_slice<double, 8>,  (const void *)k_metro_slice<float, 32>,
                              (const void *)k_metro_slice<float, 16>,  (const void *)k_metro_slice<float, 8>,
                              (const void *)k_qr_udv<1>, (const void *)k_qr_udv<2>, (const void *)k_qr_udv<4>, (const void *)k_qr_udv<8>,
-                             (const void *)k_trtri_unit, (const void *)k_measure, (const void *)k_qr_formq,
+                             (const void *)k_trsm_unit_upper, (const void *)k_measure, (const void *)k_qr_formq,
                              (const void *)k_qr_formq_reg<16>, (const void *)k_qr_formq_reg<32>,
                              (const void *)k_qr_formq_reg<64>,
                              (const void *)k_gemm_big<double, 0, 0>, (const void *)k_gemm_big<double, 1, 0>,

@@ -20,9 +20,10 @@
 //
 // k_qr_formq: explicit Q from the reflectors, one wave per 16-column block.
 //
-// k_trtri_unit: inverse of a unit upper-triangular matrix
-// (dtrtri('U','U'), src/linalg.c:992-1004), one wave per 32-column block
-// column, back substitution over 32-row blocks with MFMA products.
+// k_trsm_unit_upper: X = U^-1 B for unit upper-triangular U (the
+// reference inverts T with dtrtri, src/linalg.c:992-1004; here the
+// triangular factor is applied by a solve), 32-column blocks of X per
+// workgroup, back substitution over 32-row blocks with MFMA products.
 #pragma once
 #include "common.hpp"
 
@@ -665,93 +666,115 @@ static inline int qr_formq_blocks(int n, int batch)
     return ((nblk + FQ_WAVES - 1) / FQ_WAVES) * batch;
 }
 
-// X = T^-1 for unit upper-triangular T (the diagonal of T is not read).
-struct TrtriArgs {
+// X = U^-1 B for unit upper-triangular U (the diagonal of U is not read):
+// dtrsm('L','U','N','U').  Column blocks of X are independent, so a
+// workgroup takes 32 columns of one matrix (n/32 workgroups per matrix) and
+// walks the 32-row blocks from the bottom: S = B_I - U[I, I+1:] X[I+1:] on
+// MFMA (one 16x16 tile per wave, U streamed from global memory, X resident
+// in LDS), then the unit solve U_II X_I = S, one column per lane of wave 0
+// with U_II staged in LDS.
+struct TrsmArgs {
     int n, batch;
-    const double *T;
-    long long sT;
+    const double *U;
+    long long sU;
+    const double *B;
+    long long sB;
     double *X;
     long long sX;
     const int *status;
 };
 
-#define TR_NB 32
+#define TS_NB 32
 
-// One wave per 32-wide block column J of X: for block rows I = J down to 0,
-// X_IJ = T_II^-1 (delta_IJ - sum_{K>I} T_IK X_KJ); the sum is a 32x32 MFMA
-// product accumulated over 32-row blocks of K (read back from X in global
-// memory, written earlier by the same wave), the 32x32 unit solve runs one
-// column per lane.
-__global__ __launch_bounds__(256) void k_trtri_unit(TrtriArgs a)
+static inline size_t trsm_lds_bytes(int n)
 {
-    const int b = blockIdx.x;
+    return sizeof(double) * ((size_t)TS_NB * (n + 4) + TS_NB * (TS_NB + 1));
+}
+
+static inline int trsm_blocks(int n, int batch) { return ((n + TS_NB - 1) / TS_NB) * batch; }
+
+__global__ __launch_bounds__(256) void k_trsm_unit_upper(TrsmArgs a)
+{
+    const int n = a.n;
+    const int nblk = (n + TS_NB - 1) / TS_NB;
+    const int b = blockIdx.x / nblk, cb = blockIdx.x % nblk;
     if (b >= a.batch) return;
     if (a.status && a.status[b]) return;
-    const int n = a.n;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lc = lane & 15, lr = lane >> 4;
-    const double *T = a.T + (long long)b * a.sT;
+    const int c0 = cb * TS_NB, cw = min(TS_NB, n - c0);
+    const double *U = a.U + (long long)b * a.sU;
+    const double *Bm = a.B + (long long)b * a.sB;
     double *X = a.X + (long long)b * a.sX;
     extern __shared__ double lds[];
-    double *S = lds + (size_t)wave * TR_NB * TR_NB; // per-wave 32x32 rhs, S[i + jj*32]
-    const int nblk = (n + TR_NB - 1) / TR_NB;
-    for (int J = wave; J < nblk; J += nw) {
-        const int c0 = J * TR_NB;
-        const int jb = min(TR_NB, n - c0);
-        // zero the part of the block column below its diagonal block
-        for (long long idx = lane; idx < (long long)(n - (c0 + jb)) * jb; idx += 64) {
-            const int r = c0 + jb + (int)(idx % (n - c0 - jb)), jj = (int)(idx / (n - c0 - jb));
-            X[r + (long long)(c0 + jj) * n] = 0.0;
-        }
-        for (int I = J; I >= 0; I--) {
-            const int r0 = I * TR_NB;
-            const int ib = min(TR_NB, n - r0);
-            // S = delta - T[r0:r0+ib, r0+ib : c0+jb] X[r0+ib : c0+jb, c0:c0+jb]
-            for (int ti = 0; ti < 2; ti++)
-                for (int tj = 0; tj < 2; tj++) {
-                    f64x4 acc = {0, 0, 0, 0};
-                    const int i = r0 + ti * 16 + lc;
-                    const int jj = tj * 16 + lc;
-                    for (int kb = r0 + ib; kb < c0 + jb; kb += 32) {
-                        double ta[8], xb[8];
+    const int ldx = n + 4;              // Xs[c*ldx + r]: column-major, conflict-free fragment reads
+    double *Xs = lds;
+    double *Ud = Xs + (size_t)TS_NB * ldx; // U_II, Ud[i*(TS_NB+1) + k]
+    for (int idx = tid; idx < TS_NB * n; idx += blockDim.x) {
+        const int r = idx % n, c = idx / n;
+        Xs[c * ldx + r] = c < cw ? Bm[r + (long long)(c0 + c) * n] : 0.0;
+    }
+    __syncthreads();
+    const int ti = wave >> 1, tj = wave & 1;
+    for (int I = nblk - 1; I >= 0; I--) {
+        const int r0 = I * TS_NB, ib = min(TS_NB, n - r0);
+        // S = B_I - U[r0:r0+ib, r0+ib:n] X[r0+ib:n]: tile (ti, tj), rows i, columns c
+        {
+            f64x4 acc;
+            const int i0 = r0 + ti * 16, cc = tj * 16 + lc;
 #pragma unroll
-                        for (int u = 0; u < 8; u++) {
-                            const int k = kb + 4 * u + lr;
-                            const bool kok = k < c0 + jb;
-                            ta[u] = (kok && i < r0 + ib) ? T[i + (long long)k * n] : 0.0;
-                            xb[u] = (kok && jj < jb) ? X[k + (long long)(c0 + jj) * n] : 0.0;
-                        }
+            for (int v = 0; v < 4; v++) {
+                const int i = i0 + Mfma<double>::row(lane, v);
+                acc[v] = (i < r0 + ib) ? Xs[cc * ldx + i] : 0.0;
+            }
+            const bool iok = i0 + lc < r0 + ib;
+            for (int k0 = r0 + ib; k0 < n; k0 += 32) {
+                double ua[8], xb[8];
 #pragma unroll
-                        for (int u = 0; u < 8; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[u], xb[u], acc, 0, 0, 0);
-                    }
-#pragma unroll
-                    for (int v = 0; v < 4; v++) {
-                        const int i = ti * 16 + Mfma<double>::row(lane, v);
-                        const int jj = tj * 16 + lc;
-                        const double d = (r0 + i == c0 + jj) ? 1.0 : 0.0;
-                        S[i + jj * TR_NB] = d - acc[v];
-                    }
+                for (int u = 0; u < 8; u++) {
+                    const int k = k0 + 4 * u + lr;
+                    ua[u] = (iok && k < n) ? U[(i0 + lc) + (long long)k * n] : 0.0;
+                    xb[u] = k < n ? Xs[cc * ldx + k] : 0.0;
                 }
-            __builtin_amdgcn_s_waitcnt(0xc07f);
-            __builtin_amdgcn_wave_barrier();
-            // unit upper solve T_II x = S(:, jj), one column per lane
-            if (lane < jb) {
-                const int jj = lane;
-                for (int i = ib - 1; i >= 0; i--) {
-                    double s = S[i + jj * TR_NB];
-                    for (int k = i + 1; k < ib; k++) s -= T[(r0 + i) + (long long)(r0 + k) * n] * S[k + jj * TR_NB];
-                    S[i + jj * TR_NB] = s;
-                    X[(r0 + i) + (long long)(c0 + jj) * n] = s;
+#pragma unroll
+                for (int u = 0; u < 8; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-ua[u], xb[u], acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = i0 + Mfma<double>::row(lane, v);
+                if (i < r0 + ib) Xs[cc * ldx + i] = acc[v];
+            }
+        }
+        for (int idx = tid; idx < TS_NB * TS_NB; idx += blockDim.x) {
+            const int i = idx % TS_NB, k = idx / TS_NB;
+            Ud[i * (TS_NB + 1) + k] = (i < ib && k < ib) ? U[(r0 + i) + (long long)(r0 + k) * n] : 0.0;
+        }
+        __syncthreads();
+        if (wave == 0 && lane < TS_NB) {
+            // column `lane`: x_i = s_i - sum_{k>i} U_ik x_k, rows ib-1 .. 0
+            double x[TS_NB];
+            double *col = Xs + lane * ldx + r0;
+#pragma unroll
+            for (int i = TS_NB - 1; i >= 0; i--) {
+                if (i < ib) {
+                    double sacc = col[i];
+#pragma unroll
+                    for (int k = i + 1; k < TS_NB; k++)
+                        if (k < ib) sacc -= Ud[i * (TS_NB + 1) + k] * x[k];
+                    x[i] = sacc;
+                } else {
+                    x[i] = 0.0;
                 }
             }
-            __builtin_amdgcn_s_waitcnt(0);
-            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int i = 0; i < TS_NB; i++)
+                if (i < ib) col[i] = x[i];
         }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < TS_NB * n; idx += blockDim.x) {
+        const int r = idx % n, c = idx / n;
+        if (c < cw) X[r + (long long)(c0 + c) * n] = Xs[c * ldx + r];
     }
 }
 
-static inline size_t trtri_lds_bytes(int n)
-{
-    (void)n;
-    return sizeof(double) * (size_t)4 * TR_NB * TR_NB;
-}

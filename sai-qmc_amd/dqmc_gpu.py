@@ -286,7 +286,7 @@ class Engine:
         return v.value
 
     PROF_NAMES = ["metro", "wrap", "stack", "lmul", "combine", "inv_one_plus", "accept", "measure", "qr",
-                  "trtri", "gemm", "flush"]
+                  "trsm", "gemm", "flush"]
 
     def profile(self, nsweep=1):
         """Per-stage milliseconds and stage counts of nsweep sweeps
