@@ -179,16 +179,34 @@ __device__ void wy_apply_block(const double *P, int ld, const double *Tl, int nb
     __builtin_amdgcn_wave_barrier();
 }
 
+// Reflector of a column from its diagonal entry alpha and the squared norm
+// xn2 below it (LAPACK dlarfg): beta = -sign(alpha) sqrt(alpha^2 + xn2),
+// tau = (beta - alpha) / beta, scale 1 / (alpha - beta); tau = 0 (identity)
+// when xn2 = 0.
+__device__ __forceinline__ void reflector(double alpha, double xn2, double &tau, double &sc, double &beta)
+{
+    tau = 0.0;
+    sc = 0.0;
+    beta = alpha;
+    if (xn2 != 0.0) {
+        beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+        tau = (beta - alpha) / beta;
+        sc = 1.0 / (alpha - beta);
+    }
+}
+
 // Panel factorisation with register-resident columns: wave w owns panel
-// columns c2 = 4k + w (slot k), lane l holds rows l + 64 s (slot s).  Step c
-// reads the unscaled pivot column from LDS (published by its owner during
-// step c-1), every wave updates all of its columns with one fused pass
-// (independent dot products, DPP reductions interleaved), the owner of
-// column c+1 publishes it with its norm, then one barrier.
+// columns c2 = QR_WAVES k + w (slot k), lane l holds rows l + 64 s (slot s).
+// Step c reads reflector c (the unscaled pivot column in LDS, tau and the
+// scale published by its owner during step c-1).  The owner of column c+1
+// works with lookahead: it applies reflector c to column c+1 first,
+// publishes that column, its squared norm below the diagonal and its own
+// reflector scalars (the square root and divisions of dlarfg), and only
+// then updates its other columns, while the other waves update theirs; one
+// barrier per step.
 template <int MR>
 __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int ld, double *xcol,
-                         double *s_alpha, double *s_nrm, double *s_sc, double *s_beta, double *Tl, int *bad,
-                         int *ntau)
+                         double *s_tau, double *s_sc, double *s_beta, double *Tl, int *bad)
 {
     constexpr int NS = QR_NB / QR_WAVES; // column slots per wave
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -197,7 +215,7 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
     for (int k = 0; k < NS; k++)
 #pragma unroll
         for (int s = 0; s < MR; s++) {
-            const int c2 = NS * 0 + QR_WAVES * k + wave, r = lane + 64 * s;
+            const int c2 = QR_WAVES * k + wave, r = lane + 64 * s;
             pc[k][s] = (c2 < nb && r < m) ? A[(p0 + r) + (long long)(p0 + c2) * n] : 0.0;
         }
 #pragma unroll
@@ -205,7 +223,7 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
 #pragma unroll
         for (int s = 0; s < MR; s++)
             if (!isfinite(pc[k][s])) *bad = 1;
-    // column 0 is owned by wave 0, slot 0
+    // column 0 (wave 0, slot 0): publish it with its reflector
     if (wave == 0) {
         double q = 0.0;
 #pragma unroll
@@ -215,64 +233,54 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
             if (r > 0 && r < m) q += pc[0][s] * pc[0][s];
         }
         q = wave_sum(q);
+        const double alpha = bcast_lane(pc[0][0], 0);
+        double tau, sc, beta;
+        reflector(alpha, q, tau, sc, beta);
         if (lane == 0) {
-            s_nrm[0] = q;
-            s_alpha[0] = pc[0][0];
+            s_tau[0] = tau;
+            s_sc[0] = sc;
+            s_beta[0] = beta;
+            Tl[0] = tau;
+            if (!isfinite(tau) || !isfinite(sc)) *bad = 1;
         }
     }
     __syncthreads();
     for (int c = 0; c < nb; c++) {
         const double *xc = xcol + (c & 1) * m;
         double *xn = xcol + ((c + 1) & 1) * m;
-        const double alpha = s_alpha[c], xn2 = s_nrm[c];
-        double tau = 0.0, sc = 0.0, beta = alpha;
-        if (xn2 != 0.0) {
-            beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
-            tau = (beta - alpha) / beta;
-            sc = 1.0 / (alpha - beta);
-        }
-        if (threadIdx.x == 0) {
-            if (xn2 != 0.0) (*ntau)++;
-            if (!isfinite(tau) || !isfinite(sc)) *bad = 1;
-            Tl[c + c * QR_NB] = tau;
-            s_sc[c] = sc;
-            s_beta[c] = beta;
-        }
+        const double tau = s_tau[c], sc = s_sc[c];
         double x[MR];
 #pragma unroll
         for (int s = 0; s < MR; s++) {
             const int r = lane + 64 * s;
             x[s] = (r > c && r < m) ? xc[r] : 0.0;
         }
-        double d[NS];
+        const int cl = c; // the pivot row c < QR_NB <= 64 sits in register slot 0
+        const bool owner = c + 1 < nb && ((c + 1) % QR_WAVES) == wave;
+        const int k1 = (c + 1) / QR_WAVES;
+        // column c2 <- column c2 - w v_c, w = tau (pcc + sc v_c^T x_below), v_c = e_c + sc x
+        auto apply = [&](double (&col)[MR], double dsum) {
+            const double pcc = bcast_lane(col[0], cl);
+            const double w = tau * (pcc + sc * dsum);
+            const double ws = w * sc;
 #pragma unroll
-        for (int k = 0; k < NS; k++) {
-            double acc = 0.0;
+            for (int s = 0; s < MR; s++) col[s] -= ws * x[s];
+            if (lane == cl) col[0] -= w;
+        };
+        if (owner && tau != 0.0) {
+            // lookahead: column c+1 first
 #pragma unroll
-            for (int s = 0; s < MR; s++) acc += x[s] * pc[k][s];
-            d[k] = acc;
-        }
+            for (int k = 0; k < NS; k++) {
+                if (k == k1) {
+                    double acc = 0.0;
 #pragma unroll
-        for (int k = 0; k < NS; k++) d[k] = wave_sum(d[k]);
-        // the pivot row c < QR_NB <= 64 always sits in register slot 0
-        const int cl = c;
-#pragma unroll
-        for (int k = 0; k < NS; k++) {
-            const int c2 = QR_WAVES * k + wave;
-            if (c2 > c && c2 < nb && tau != 0.0) {
-                const double pcc = bcast_lane(pc[k][0], cl);
-                const double w = tau * (pcc + sc * d[k]);
-                const double ws = w * sc;
-#pragma unroll
-                for (int s = 0; s < MR; s++) pc[k][s] -= ws * x[s];
-                if (lane == cl) pc[k][0] -= w;
+                    for (int s = 0; s < MR; s++) acc += x[s] * pc[k][s];
+                    apply(pc[k], wave_sum(acc));
+                }
             }
         }
-        // the owner of column c+1 publishes it (unscaled), its diagonal and
-        // the squared norm below the diagonal
-        if (c + 1 < nb && ((c + 1) % QR_WAVES) == wave) {
-            const int k1 = (c + 1) / QR_WAVES;
-            double q = 0.0;
+        if (owner) {
+            double q = 0.0, av = 0.0;
 #pragma unroll
             for (int k = 0; k < NS; k++) {
                 if (k == k1) {
@@ -282,16 +290,38 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
                         if (r < m) xn[r] = pc[k][s];
                         if (r > c + 1 && r < m) q += pc[k][s] * pc[k][s];
                     }
+                    av = bcast_lane(pc[k][0], c + 1);
                 }
             }
             q = wave_sum(q);
-            if (lane == 0) s_nrm[c + 1] = q;
-            if (lane == ((c + 1) & 63)) {
-                double av = 0.0;
+            double t1, s1, b1;
+            reflector(av, q, t1, s1, b1);
+            if (lane == 0) {
+                s_tau[c + 1] = t1;
+                s_sc[c + 1] = s1;
+                s_beta[c + 1] = b1;
+                Tl[(c + 1) * (QR_NB + 1)] = t1;
+                if (!isfinite(t1) || !isfinite(s1)) *bad = 1;
+            }
+        }
+        if (tau != 0.0) {
+            double d[NS];
 #pragma unroll
-                for (int k = 0; k < NS; k++)
-                    if (k == k1) av = pc[k][0];
-                s_alpha[c + 1] = av;
+            for (int k = 0; k < NS; k++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int s = 0; s < MR; s++) acc += x[s] * pc[k][s];
+                d[k] = acc;
+            }
+#pragma unroll
+            for (int k = 0; k < NS; k++) {
+                const int c2 = QR_WAVES * k + wave;
+                if (c2 > c && c2 < nb && !(owner && k == k1)) d[k] = wave_sum(d[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < NS; k++) {
+                const int c2 = QR_WAVES * k + wave;
+                if (c2 > c && c2 < nb && !(owner && k == k1)) apply(pc[k], d[k]);
             }
         }
         __syncthreads();
@@ -402,9 +432,8 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_udv(QrArgs a)
     double *Tl = P + (size_t)QR_NB * ld;       // QR_NB x QR_NB
     double *Wl = Tl + QR_NB * QR_NB;           // QR_WAVES x (QR_NB x 16); also Y and T scratch
     double *xcol = Wl + QR_WAVES * QR_NB * 16; // 2 x n pivot columns
-    double *s_alpha = xcol + 2 * n;
-    double *s_nrm = s_alpha + QR_NB;
-    double *s_sc = s_nrm + QR_NB;
+    double *s_tau = xcol + 2 * n;
+    double *s_sc = s_tau + QR_NB;
     double *s_beta = s_sc + QR_NB;
     __shared__ int s_bad, s_ntau;
     if (tid == 0) { s_bad = 0; s_ntau = 0; }
@@ -417,7 +446,9 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_udv(QrArgs a)
         const int m = n - p0;
         for (int idx = tid; idx < QR_NB * QR_NB; idx += blockDim.x) Tl[idx] = 0.0;
         __syncthreads();
-        qr_panel<MR>(A, n, p0, nb, m, P, ld, xcol, s_alpha, s_nrm, s_sc, s_beta, Tl, &s_bad, &s_ntau);
+        qr_panel<MR>(A, n, p0, nb, m, P, ld, xcol, s_tau, s_sc, s_beta, Tl, &s_bad);
+        if (tid == 0)
+            for (int c = 0; c < nb; c++) s_ntau += (s_tau[c] != 0.0); // non-trivial reflectors
         qr_tfactor(P, ld, nb, m, Tl, Wl);
         for (int idx = tid; idx < QR_NB * QR_NB; idx += blockDim.x) Tw[pi * QR_NB * QR_NB + idx] = Tl[idx];
         // trailing update A[p0:, p0+nb:] <- (I - V T^T V^T) A
