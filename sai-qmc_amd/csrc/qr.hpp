@@ -55,6 +55,18 @@ __device__ __forceinline__ double panel_v(const double *P, int ld, int c, int r)
     return r > c ? P[c * ld + r] : (r == c ? 1.0 : 0.0);
 }
 
+// panel_v without a divergent branch: the LDS read always happens (column
+// and row clamped into the panel) and the entry is selected afterwards, so
+// reads can be batched ahead of the MFMAs that consume them; 0 outside the
+// nb x m panel
+__device__ __forceinline__ double panel_vb(const double *P, int ld, int c, int r, int nb, int m)
+{
+    const int cc = c < QR_NB - 1 ? c : QR_NB - 1;
+    const int rr = r < 0 ? 0 : (r < m ? r : m - 1);
+    const double raw = P[cc * ld + rr];
+    return (c < nb && r < m) ? (r > c ? raw : (r == c ? 1.0 : 0.0)) : 0.0;
+}
+
 __device__ __forceinline__ double bcast_lane(double v, int lane)
 {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
@@ -85,14 +97,22 @@ __device__ __forceinline__ void wy_load(double (&xb)[WY_NU], const double *col, 
 __device__ __forceinline__ void wy_accum_w(const double (&xb)[WY_NU], const double *P, int ld, int nb, int r0, int m,
                                            int lc, int lr, f64x4 &w0, f64x4 &w1)
 {
+    // eight k-steps of V fragments read ahead of their sixteen MFMAs
 #pragma unroll
-    for (int u = 0; u < WY_NU; u++) {
-        if (r0 + 4 * u < m) {
-            const int k = r0 + 4 * u + lr;
-            const double a0 = (lc < nb && k < m) ? panel_v(P, ld, lc, k) : 0.0;
-            const double a1 = (16 + lc < nb && k < m) ? panel_v(P, ld, 16 + lc, k) : 0.0;
-            w0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, xb[u], w0, 0, 0, 0);
-            w1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, xb[u], w1, 0, 0, 0);
+    for (int u0 = 0; u0 < WY_NU; u0 += 8) {
+        if (r0 + 4 * u0 < m) {
+            double a0[8], a1[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int k = r0 + 4 * (u0 + u) + lr;
+                a0[u] = panel_vb(P, ld, lc, k, nb, m);
+                a1[u] = panel_vb(P, ld, 16 + lc, k, nb, m);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                w0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], xb[u0 + u], w0, 0, 0, 0);
+                w1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], xb[u0 + u], w1, 0, 0, 0);
+            }
         }
     }
 }
@@ -100,19 +120,25 @@ __device__ __forceinline__ void wy_accum_w(const double (&xb)[WY_NU], const doub
 __device__ __forceinline__ void wy_update_store(double (&xb)[WY_NU], const double (&wv)[QR_NB / 4], const double *P,
                                                 int ld, int nb, int r0, int m, double *col, bool jok, int lc, int lr)
 {
+    // two 16-row tiles at a time: their V fragments read ahead of 16 MFMAs
 #pragma unroll
-    for (int t = 0; t < WY_NU / 4; t++) {
+    for (int t = 0; t < WY_NU / 4; t += 2) {
         if (r0 + 16 * t < m) {
-            f64x4 c = {xb[4 * t], xb[4 * t + 1], xb[4 * t + 2], xb[4 * t + 3]};
-            const int r = r0 + 16 * t + lc;
+            double av[2][QR_NB / 4];
 #pragma unroll
-            for (int kk = 0; kk < QR_NB / 4; kk++) {
-                const int k = 4 * kk + lr;
-                const double a = (k < nb && r < m) ? -panel_v(P, ld, k, r) : 0.0;
-                c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, wv[kk], c, 0, 0, 0);
+            for (int h = 0; h < 2; h++) {
+                const int r = r0 + 16 * (t + h) + lc;
+#pragma unroll
+                for (int kk = 0; kk < QR_NB / 4; kk++) av[h][kk] = -panel_vb(P, ld, 4 * kk + lr, r, nb, m);
             }
 #pragma unroll
-            for (int v = 0; v < 4; v++) xb[4 * t + v] = c[v];
+            for (int h = 0; h < 2; h++) {
+                f64x4 c = {xb[4 * (t + h)], xb[4 * (t + h) + 1], xb[4 * (t + h) + 2], xb[4 * (t + h) + 3]};
+#pragma unroll
+                for (int kk = 0; kk < QR_NB / 4; kk++) c = __builtin_amdgcn_mfma_f64_16x16x4f64(av[h][kk], wv[kk], c, 0, 0, 0);
+#pragma unroll
+                for (int v = 0; v < 4; v++) xb[4 * (t + h) + v] = c[v];
+            }
         }
     }
 #pragma unroll
