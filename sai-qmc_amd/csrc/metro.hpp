@@ -114,11 +114,25 @@ __device__ void window_correct(const T *G, int n, int i0, int wn, const T *C, co
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     const int rr = Mfma<T>::row(lane, v);
-                    // cols tile: rows w = w0 + rr, columns j = j0 + lc
-                    ac[t][tw][v] = (w0 + rr < wn && jok[t]) ? G[(j0[t] + lc) + (long long)(i0 + w0 + rr) * n] : T(0);
+                    // cols tile: rows w = w0 + rr, columns j = j0 + lc;
                     // rows tile: rows j = j0 + rr, columns w = w0 + lc
-                    ar[t][tw][v] =
-                        (w0 + lc < wn && j0[t] + rr < n) ? G[(i0 + w0 + lc) + (long long)(j0[t] + rr) * n] : T(0);
+                    // (clamped addresses, zeroed below)
+                    const int wc = i0 + min(w0 + rr, wn - 1), jc = min(j0[t] + lc, n - 1);
+                    const int wr = i0 + min(w0 + lc, wn - 1), jr = min(j0[t] + rr, n - 1);
+                    ac[t][tw][v] = G[jc + (long long)wc * n];
+                    ar[t][tw][v] = G[wr + (long long)jr * n];
+                }
+            }
+#pragma unroll
+        for (int t = 0; t < JT; t++)
+#pragma unroll
+            for (int tw = 0; tw < NTW; tw++) {
+                const int w0 = tw * 16;
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int rr = Mfma<T>::row(lane, v);
+                    ac[t][tw][v] = (w0 + rr < wn && jok[t]) ? ac[t][tw][v] : T(0);
+                    ar[t][tw][v] = (w0 + lc < wn && j0[t] + rr < n) ? ar[t][tw][v] : T(0);
                 }
             }
         // big operands C[q][j0+lc] (cols) and V[q][j0+lc] (rows): 8 k-steps of
@@ -126,13 +140,25 @@ __device__ void window_correct(const T *G, int n, int i0, int wn, const T *C, co
         // wave on the SIMD computes while this one waits
         for (int q0 = 0; q0 < P; q0 += 32) {
             T cc[JT][8], cv[JT][8];
+            // loads from clamped (always valid) addresses, zeroed afterwards:
+            // no divergent branch between them
 #pragma unroll
             for (int u = 0; u < 8; u++) {
-                const int q = q0 + 4 * u + lr;
+                const int q = min(q0 + 4 * u + lr, P - 1);
 #pragma unroll
                 for (int t = 0; t < JT; t++) {
-                    cc[t][u] = (q < P && jok[t]) ? C[(long long)q * n + j0[t] + lc] : T(0);
-                    cv[t][u] = (q < P && jok[t]) ? V[(long long)q * n + j0[t] + lc] : T(0);
+                    const long long o = (long long)q * n + min(j0[t] + lc, n - 1);
+                    cc[t][u] = C[o];
+                    cv[t][u] = V[o];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const bool qok = q0 + 4 * u + lr < P;
+#pragma unroll
+                for (int t = 0; t < JT; t++) {
+                    cc[t][u] = (qok && jok[t]) ? cc[t][u] : T(0);
+                    cv[t][u] = (qok && jok[t]) ? cv[t][u] : T(0);
                 }
             }
 #pragma unroll
@@ -141,8 +167,10 @@ __device__ void window_correct(const T *G, int n, int i0, int wn, const T *C, co
 #pragma unroll
                 for (int tw = 0; tw < NTW; tw++) {
                     const int w = tw * 16 + lc;
-                    const T vw = (q < P && w < wn) ? Vw[q * W + w] : T(0);
-                    const T cw = (q < P && w < wn) ? Cw[q * W + w] : T(0);
+                    const int o = min(q, P - 1) * W + min(w, W - 1);
+                    const bool ok = q < P && w < wn;
+                    const T vw = ok ? Vw[o] : T(0);
+                    const T cw = ok ? Cw[o] : T(0);
 #pragma unroll
                     for (int t = 0; t < JT; t++) {
                         // cols: A[w][q] = -V[q][i0+w], B[q][j] = C[q][j]
