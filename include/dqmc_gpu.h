@@ -166,8 +166,8 @@ void dqmc_gpu_destroy(dqmc_gpu_ctx *ctx);
 
 /* ---- batched linear-algebra entry points on host arrays ----
    Each takes `batch` independent problems laid out back to back
-   (stride n*n for matrices, n for vectors) and runs them on `device`.
-   They replace the reference's linalg.c calls one for one. */
+   (stride n*n for matrices, n for vectors) and runs them on `device`,
+   1 <= n <= 512.  They replace the reference's linalg.c calls one for one. */
 
 /* udv_lmul_work (src/linalg.c:515-642), in place. */
 int dqmc_gpu_udv_lmul(int device, int n, int batch, double *U, double *D,

@@ -1608,7 +1608,7 @@ struct HostBatch {
 
 static int hb_init(HostBatch &h, int device, int n, int batch)
 {
-    if (n <= 0 || n > 1024 || batch <= 0) {
+    if (n <= 0 || n > METRO_MAX_N || batch <= 0) {
         set_err("invalid n=%d batch=%d", n, batch);
         return 1;
     }

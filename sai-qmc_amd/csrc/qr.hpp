@@ -49,16 +49,10 @@ struct QrArgs {
 };
 
 // reflector entry v(c, r) of the panel P (column c, row r, leading dim ld):
-// 1 on the diagonal, stored values below, 0 above
-__device__ __forceinline__ double panel_v(const double *P, int ld, int c, int r)
-{
-    return r > c ? P[c * ld + r] : (r == c ? 1.0 : 0.0);
-}
-
-// panel_v without a divergent branch: the LDS read always happens (column
-// and row clamped into the panel) and the entry is selected afterwards, so
-// reads can be batched ahead of the MFMAs that consume them; 0 outside the
-// nb x m panel
+// 1 on the diagonal, stored values below, 0 above and outside the nb x m
+// panel.  No divergent branch: the LDS read always happens (column and row
+// clamped into the panel) and the entry is selected afterwards, so reads
+// can be batched ahead of the MFMAs that consume them
 __device__ __forceinline__ double panel_vb(const double *P, int ld, int c, int r, int nb, int m)
 {
     const int cc = c < QR_NB - 1 ? c : QR_NB - 1;
@@ -396,15 +390,22 @@ __device__ void qr_tfactor(const double *P, int ld, int nb, int m, double *Tl, d
         const int steps = (m + 3) >> 2, hs = (steps + 1) >> 1;
         const int s0 = half ? hs : 0, s1 = half ? steps : hs;
         f64x4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
-        for (int st = s0; st < s1; st += 2) {
-            const int k = 4 * st + lr, k2 = k + 4;
-            const double va = (ci < nb && k < m) ? panel_v(P, ld, ci, k) : 0.0;
-            const double vb = (cj < nb && k < m) ? panel_v(P, ld, cj, k) : 0.0;
-            const bool two = st + 1 < s1;
-            const double va2 = (two && ci < nb && k2 < m) ? panel_v(P, ld, ci, k2) : 0.0;
-            const double vb2 = (two && cj < nb && k2 < m) ? panel_v(P, ld, cj, k2) : 0.0;
-            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(va, vb, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(va2, vb2, acc1, 0, 0, 0);
+        // batches of 8 k-steps: all 16 branch-free LDS reads first, then the
+        // MFMAs; rows past this half's range read as row m (zero)
+        for (int st0 = s0; st0 < s1; st0 += 8) {
+            double va[8], vb[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int st = st0 + u;
+                const int k = st < s1 ? 4 * st + lr : m;
+                va[u] = panel_vb(P, ld, ci, k, nb, m);
+                vb[u] = panel_vb(P, ld, cj, k, nb, m);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u += 2) {
+                acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(va[u], vb[u], acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(va[u + 1], vb[u + 1], acc1, 0, 0, 0);
+            }
         }
         double *dst = half ? Y2 : Y;
 #pragma unroll
@@ -483,31 +484,31 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_udv(QrArgs a)
         __syncthreads();
     }
 
-    // D = diag R, Rn = D^-1 R
+    // D = diag R, Rn = D^-1 R (row i scaled by 1 / D_i, the reciprocals
+    // kept in the panel buffer, which is free now)
     double *Dd = a.D + (long long)b * a.sD;
+    double *rinv = P;
     for (int i = tid; i < n; i += blockDim.x) {
         const double d = A[i + (long long)i * n];
         Dd[i] = d;
+        rinv[i] = 1.0 / d;
         if (!isfinite(d) || d == 0.0) s_bad = (s_bad == 1) ? 1 : 2;
     }
-    if (a.Rn) {
-        // Rn = D^-1 R, eight elements per thread in flight
+    __syncthreads();
+    if (a.Rn && tid < (int)(blockDim.x / n) * n) {
+        // thread (i, jc) scales row i of columns jc + cpb t; eight columns in
+        // flight, loads from clamped addresses (n <= blockDim.x)
         double *Rn = a.Rn + (long long)b * a.sR;
-        const long long nn = (long long)n * n;
-        for (long long base = tid; base < nn; base += 8LL * blockDim.x) {
-            double v[8], d[8];
+        const int cpb = blockDim.x / n, i = tid % n, jc = tid / n;
+        const double ri = rinv[i];
+        for (int j0 = jc; j0 < n; j0 += 8 * cpb) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = A[i + (long long)min(j0 + u * cpb, n - 1) * n];
 #pragma unroll
             for (int u = 0; u < 8; u++) {
-                const long long idx = base + (long long)u * blockDim.x;
-                const int i = (int)(idx % n), j = (int)(idx / n);
-                const bool up = idx < nn && i <= j;
-                v[u] = up ? A[idx] : 0.0;
-                d[u] = up ? A[i + (long long)i * n] : 1.0;
-            }
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const long long idx = base + (long long)u * blockDim.x;
-                if (idx < nn) Rn[idx] = v[u] / d[u];
+                const int j = j0 + u * cpb;
+                if (j < n) Rn[i + (long long)j * n] = i <= j ? v[u] * ri : 0.0;
             }
         }
     }
@@ -649,8 +650,8 @@ __global__ __launch_bounds__(FQ_WAVES * 64) void k_qr_formq_reg(QrArgs a)
         for (int u = 0; u < NU; u++) {
             if (4 * u >= p0 && 4 * u < n) {
                 const int k = 4 * u + lr - p0;
-                const double a0 = (lc < nb && k < m) ? panel_v(P, ld, lc, k) : 0.0;
-                const double a1 = (16 + lc < nb && k < m) ? panel_v(P, ld, 16 + lc, k) : 0.0;
+                const double a0 = panel_vb(P, ld, lc, k, nb, m);
+                const double a1 = panel_vb(P, ld, 16 + lc, k, nb, m);
                 w0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, xb[u], w0, 0, 0, 0);
                 w1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, xb[u], w1, 0, 0, 0);
             }
@@ -694,7 +695,7 @@ __global__ __launch_bounds__(FQ_WAVES * 64) void k_qr_formq_reg(QrArgs a)
 #pragma unroll
                 for (int kk = 0; kk < QR_NB / 4; kk++) {
                     const int k = 4 * kk + lr;
-                    const double av = (k < nb && r < m) ? -panel_v(P, ld, k, r) : 0.0;
+                    const double av = -panel_vb(P, ld, k, r, nb, m);
                     c = __builtin_amdgcn_mfma_f64_16x16x4f64(av, wv[kk], c, 0, 0, 0);
                 }
 #pragma unroll
