@@ -113,6 +113,27 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs<T> g)
         __syncthreads();
     }
 
+    // epilogue: scales and the old C values are loaded as one batch from
+    // clamped addresses; the stores are predicated
+    T sr[2][4], sc[2], old[2][2][4];
+#pragma unroll
+    for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) sr[mi][v] = g.cr.raw(b, min(m0 + wr * 32 + mi * 16 + Mfma<T>::row(lane, v), g.M - 1));
+#pragma unroll
+    for (int nj = 0; nj < 2; nj++) sc[nj] = g.cc.raw(b, min(n0 + wc * 32 + nj * 16 + (lane & 15), g.N - 1));
+    if (g.beta != T(0)) {
+#pragma unroll
+        for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+            for (int nj = 0; nj < 2; nj++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int i = min(m0 + wr * 32 + mi * 16 + Mfma<T>::row(lane, v), g.M - 1);
+                    const int j = min(n0 + wc * 32 + nj * 16 + (lane & 15), g.N - 1);
+                    old[mi][nj][v] = C[i + (long long)j * g.ldc];
+                }
+    }
 #pragma unroll
     for (int mi = 0; mi < 2; mi++)
 #pragma unroll
@@ -121,13 +142,10 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs<T> g)
             for (int v = 0; v < 4; v++) {
                 const int i = m0 + wr * 32 + mi * 16 + Mfma<T>::row(lane, v);
                 const int j = n0 + wc * 32 + nj * 16 + (lane & 15);
-                if (i < g.M && j < g.N) {
-                    T r = g.alpha * acc[mi][nj][v] * g.cr.at(b, i) * g.cc.at(b, j);
-                    if (g.dg.p && i == j) r += g.dg.at(b, i);
-                    T *cp = &C[i + (long long)j * g.ldc];
-                    if (g.beta != T(0)) r += g.beta * *cp;
-                    *cp = r;
-                }
+                T r = g.alpha * acc[mi][nj][v] * g.cr.fix(sr[mi][v]) * g.cc.fix(sc[nj]);
+                if (g.dg.p && i == j && i < g.M) r += g.dg.at(b, i);
+                if (g.beta != T(0)) r += g.beta * old[mi][nj][v];
+                if (i < g.M && j < g.N) C[i + (long long)j * g.ldc] = r;
             }
 }
 
@@ -345,23 +363,45 @@ __global__ __launch_bounds__(256, 2) void k_gemm_big(GemmArgs<T> g)
         u ^= 1;
     }
 
+    // epilogue: the 4 row and 16 column scales of a lane are loaded as one
+    // batch, the old C values (beta != 0) one 16-row block at a time, all
+    // from clamped addresses; the stores are predicated
+    T sr[4], sc[4][4];
+#pragma unroll
+    for (int it = 0; it < 4; it++) sr[it] = g.cr.raw(b, min(m0 + wr * 64 + it * 16 + lc, g.M - 1));
+#pragma unroll
+    for (int jt = 0; jt < 4; jt++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) sc[jt][v] = g.cc.raw(b, min(n0 + wc * 64 + jt * 16 + Mfma<T>::row(lane, v), g.N - 1));
+#pragma unroll
+    for (int it = 0; it < 4; it++) sr[it] = g.cr.fix(sr[it]);
+#pragma unroll
+    for (int jt = 0; jt < 4; jt++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) sc[jt][v] = g.cc.fix(sc[jt][v]);
 #pragma unroll
     for (int it = 0; it < 4; it++) {
         const int i = m0 + wr * 64 + it * 16 + lc;
-        const bool iok = i < g.M;
-        const T sr = iok ? g.cr.at(b, i) : T(0);
+        const int ic = min(i, g.M - 1);
+        T old[4][4];
+        if (g.beta != T(0)) {
+#pragma unroll
+            for (int jt = 0; jt < 4; jt++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int j = min(n0 + wc * 64 + jt * 16 + Mfma<T>::row(lane, v), g.N - 1);
+                    old[jt][v] = C[ic + (long long)j * g.ldc];
+                }
+        }
 #pragma unroll
         for (int jt = 0; jt < 4; jt++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int j = n0 + wc * 64 + jt * 16 + Mfma<T>::row(lane, v);
-                if (iok && j < g.N) {
-                    T r = g.alpha * acc[it][jt][v] * sr * g.cc.at(b, j);
-                    if (g.dg.p && i == j) r += g.dg.at(b, i);
-                    T *cp = &C[i + (long long)j * g.ldc];
-                    if (g.beta != T(0)) r += g.beta * *cp;
-                    *cp = r;
-                }
+                T r = g.alpha * acc[it][jt][v] * (i < g.M ? sr[it] : T(0)) * sc[jt][v];
+                if (g.dg.p && i == j && i < g.M) r += g.dg.at(b, i);
+                if (g.beta != T(0)) r += g.beta * old[jt][v];
+                if (i < g.M && j < g.N) C[i + (long long)j * g.ldc] = r;
             }
     }
 }

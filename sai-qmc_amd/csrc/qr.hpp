@@ -79,12 +79,20 @@ __device__ __forceinline__ double bcast_lane(double v, int lane)
 // Tl[i + c*QR_NB], upper), Wl a wave-private LDS buffer of QR_NB x 16.
 // Columns at or beyond jend are untouched.
 #define WY_NU 32
+// X is reached through a global-address-space pointer (global_load/store
+// rather than flat, whose waits would also cover the LDS reads); the loads
+// come from clamped rows and are zeroed afterwards, so all WY_NU are in
+// flight at once
+typedef __attribute__((address_space(1))) double gdouble;
 __device__ __forceinline__ void wy_load(double (&xb)[WY_NU], const double *col, int r0, int m, bool jok, int lr)
 {
+    const gdouble *g = (const gdouble *)col;
+#pragma unroll
+    for (int u = 0; u < WY_NU; u++) xb[u] = g[min(r0 + 4 * u + lr, m - 1)];
 #pragma unroll
     for (int u = 0; u < WY_NU; u++) {
         const int k = r0 + 4 * u + lr;
-        xb[u] = (jok && k < m) ? col[k] : 0.0;
+        xb[u] = (jok && k < m) ? xb[u] : 0.0;
     }
 }
 
@@ -135,10 +143,11 @@ __device__ __forceinline__ void wy_update_store(double (&xb)[WY_NU], const doubl
             }
         }
     }
+    gdouble *g = (gdouble *)col;
 #pragma unroll
     for (int u = 0; u < WY_NU; u++) {
         const int k = r0 + 4 * u + lr;
-        if (jok && k < m) col[k] = xb[u];
+        if (jok && k < m) g[k] = xb[u];
     }
 }
 
