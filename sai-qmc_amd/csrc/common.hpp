@@ -31,18 +31,25 @@ template <> struct Mfma<float> {
 };
 
 // Wave-wide reductions on DPP lane moves (quad_perm swaps, half-row and row
-// mirrors, then row_bcast15 / row_bcast31 across the four 16-lane rows); the
-// total lands in lane 63 and is broadcast with readlane.  64-bit values move
-// as two 32-bit halves.
+// mirrors within the 16-lane rows, then across rows); 64-bit values move as
+// two 32-bit halves.  With every row written (RMASK 0xF) and every source
+// lane valid, the old value of the destination is never kept, so no zero
+// needs to be moved in first; partial row masks keep 0 in the rows they
+// skip.
+template <int CTRL, int RMASK> __device__ __forceinline__ int dpp_move_i(int x)
+{
+    if constexpr (RMASK == 0xF) return __builtin_amdgcn_mov_dpp(x, CTRL, RMASK, 0xF, false);
+    else return __builtin_amdgcn_update_dpp(0, x, CTRL, RMASK, 0xF, false);
+}
 template <int CTRL, int RMASK> __device__ __forceinline__ double dpp_move(double x)
 {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, RMASK, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, RMASK, 0xF, false);
+    const int lo = dpp_move_i<CTRL, RMASK>(__double2loint(x));
+    const int hi = dpp_move_i<CTRL, RMASK>(__double2hiint(x));
     return __hiloint2double(hi, lo);
 }
 template <int CTRL, int RMASK> __device__ __forceinline__ float dpp_move(float x)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, RMASK, 0xF, false));
+    return __int_as_float(dpp_move_i<CTRL, RMASK>(__float_as_int(x)));
 }
 __device__ __forceinline__ double lane63(double v)
 {
@@ -52,15 +59,46 @@ __device__ __forceinline__ double lane63(double v)
 }
 __device__ __forceinline__ float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
 
+// Exchange with the lane 16 (32) apart: the gfx950 row swaps return the
+// pair (own, partner) in an order that differs between the two rows, and
+// their sum is the same either way.
+template <int BITS> __device__ __forceinline__ double swap_add(double v)
+{
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const auto l = BITS == 16 ? __builtin_amdgcn_permlane16_swap(lo, lo, false, false)
+                              : __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto h = BITS == 16 ? __builtin_amdgcn_permlane16_swap(hi, hi, false, false)
+                              : __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    return __hiloint2double(h[0], l[0]) + __hiloint2double(h[1], l[1]);
+}
+template <int BITS> __device__ __forceinline__ float swap_add(float v)
+{
+    const int x = __float_as_int(v);
+    const auto r = BITS == 16 ? __builtin_amdgcn_permlane16_swap(x, x, false, false)
+                              : __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    return __int_as_float(r[0]) + __int_as_float(r[1]);
+}
+__device__ __forceinline__ double first_lane(double v)
+{
+    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float first_lane(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+
+// Sum over the 64 lanes: butterflies within rows of 16 (DPP), then across
+// rows with the gfx950 row swaps.  Every lane ends with the same bits
+// ((R0 + R1) + (R2 + R3) for row sums R); the result is returned as a
+// wave-uniform value.
 template <typename T> __device__ __forceinline__ T wave_sum(T v)
 {
     v += dpp_move<0xB1, 0xF>(v);
     v += dpp_move<0x4E, 0xF>(v);
     v += dpp_move<0x141, 0xF>(v);
     v += dpp_move<0x140, 0xF>(v);
-    v += dpp_move<0x142, 0xA>(v);
-    v += dpp_move<0x143, 0xC>(v);
-    return lane63(v);
+    v = swap_add<16>(v);
+    v = swap_add<32>(v);
+    return first_lane(v);
 }
 
 __device__ __forceinline__ double wave_max(double v)

@@ -9,14 +9,14 @@
 // reflectors), which replaces the LU determinant of U in the sign
 // (src/linalg.c:1044-1083).
 //
-//   panel (QR_NB columns, resident in LDS): columns are owned round-robin by
-//     the waves; step c needs one barrier: every wave applies reflector c to
-//     the columns it owns using the unscaled column and the scalar
-//     1/(alpha-beta), and the owner of column c+1 publishes its new norm;
+//   panel (QR_NB = 16 columns): for n <= 256 one wave holds the whole
+//     panel in registers and factors it without a barrier (dgeqr2: one wave
+//     reduction per norm and per column update); taller matrices spread the
+//     columns round-robin over the waves with one barrier per column;
 //   T factor of the compact WY form (dlarft, forward, columnwise);
 //   trailing update A <- (I - V T^T V^T) A and Q <- (I - V T V^T) Q on FP64
-//     MFMA 16x16x4: each wave takes 16-column blocks, W = V^T A (2 row tiles,
-//     K = panel height), W' = op(T) W, A -= V W'.
+//     MFMA 16x16x4: each wave takes 16-column blocks, W = V^T A (QR_NB / 16
+//     row tiles, K = panel height), W' = op(T) W, A -= V W'.
 //
 // k_qr_formq: explicit Q from the reflectors, one wave per 16-column block.
 //
@@ -28,6 +28,7 @@
 #include "common.hpp"
 
 #define QR_NB 32
+#define QR_NT (QR_NB / 16) // 16-row tiles of a panel's reflectors
 #define QR_THREADS 512
 #define QR_WAVES (QR_THREADS / 64)
 
@@ -97,24 +98,23 @@ __device__ __forceinline__ void wy_load(double (&xb)[WY_NU], const double *col, 
 }
 
 __device__ __forceinline__ void wy_accum_w(const double (&xb)[WY_NU], const double *P, int ld, int nb, int r0, int m,
-                                           int lc, int lr, f64x4 &w0, f64x4 &w1)
+                                           int lc, int lr, f64x4 (&w)[QR_NT])
 {
-    // eight k-steps of V fragments read ahead of their sixteen MFMAs
+    // eight k-steps of V fragments read ahead of their MFMAs
 #pragma unroll
     for (int u0 = 0; u0 < WY_NU; u0 += 8) {
         if (r0 + 4 * u0 < m) {
-            double a0[8], a1[8];
+            double a[QR_NT][8];
 #pragma unroll
             for (int u = 0; u < 8; u++) {
                 const int k = r0 + 4 * (u0 + u) + lr;
-                a0[u] = panel_vb(P, ld, lc, k, nb, m);
-                a1[u] = panel_vb(P, ld, 16 + lc, k, nb, m);
+#pragma unroll
+                for (int h = 0; h < QR_NT; h++) a[h][u] = panel_vb(P, ld, 16 * h + lc, k, nb, m);
             }
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
-                w0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], xb[u0 + u], w0, 0, 0, 0);
-                w1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], xb[u0 + u], w1, 0, 0, 0);
-            }
+            for (int u = 0; u < 8; u++)
+#pragma unroll
+                for (int h = 0; h < QR_NT; h++) w[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[h][u], xb[u0 + u], w[h], 0, 0, 0);
         }
     }
 }
@@ -160,38 +160,40 @@ __device__ void wy_apply_block(const double *P, int ld, const double *Tl, int nb
     const bool jok = j < jend;
     double *col = X + (long long)(jok ? j : j0) * n + p0;
     double xb[WY_NU];
-    f64x4 w0 = {0, 0, 0, 0}, w1 = {0, 0, 0, 0};
+    f64x4 w[QR_NT];
+#pragma unroll
+    for (int h = 0; h < QR_NT; h++) w[h] = f64x4{0, 0, 0, 0};
     const int chunk = 4 * WY_NU;
     for (int r0 = 0; r0 < m; r0 += chunk) {
         wy_load(xb, col, r0, m, jok, lr);
-        wy_accum_w(xb, P, ld, nb, r0, m, lc, lr, w0, w1);
+        wy_accum_w(xb, P, ld, nb, r0, m, lc, lr, w);
     }
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-        const int r = Mfma<double>::row(lane, v);
-        Wl[r * 16 + lc] = w0[v];
-        Wl[(16 + r) * 16 + lc] = w1[v];
-    }
+    for (int h = 0; h < QR_NT; h++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) Wl[(16 * h + Mfma<double>::row(lane, v)) * 16 + lc] = w[h][v];
     __builtin_amdgcn_s_waitcnt(0xc07f);
     __builtin_amdgcn_wave_barrier();
-    f64x4 u0 = {0, 0, 0, 0}, u1 = {0, 0, 0, 0};
+    f64x4 uw[QR_NT];
+#pragma unroll
+    for (int h = 0; h < QR_NT; h++) uw[h] = f64x4{0, 0, 0, 0};
 #pragma unroll
     for (int k0 = 0; k0 < QR_NB; k0 += 4) {
         const int k = k0 + lr;
         const double bw = Wl[k * 16 + lc];
-        const double t0 = transT ? Tl[k + lc * QR_NB] : Tl[lc + k * QR_NB];
-        const double t1 = transT ? Tl[k + (16 + lc) * QR_NB] : Tl[(16 + lc) + k * QR_NB];
-        u0 = __builtin_amdgcn_mfma_f64_16x16x4f64(t0, bw, u0, 0, 0, 0);
-        u1 = __builtin_amdgcn_mfma_f64_16x16x4f64(t1, bw, u1, 0, 0, 0);
+#pragma unroll
+        for (int h = 0; h < QR_NT; h++) {
+            const int i = 16 * h + lc;
+            const double t = transT ? Tl[k + i * QR_NB] : Tl[i + k * QR_NB];
+            uw[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(t, bw, uw[h], 0, 0, 0);
+        }
     }
     __builtin_amdgcn_s_waitcnt(0xc07f);
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-        const int r = Mfma<double>::row(lane, v);
-        Wl[r * 16 + lc] = u0[v];
-        Wl[(16 + r) * 16 + lc] = u1[v];
-    }
+    for (int h = 0; h < QR_NT; h++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) Wl[(16 * h + Mfma<double>::row(lane, v)) * 16 + lc] = uw[h][v];
     __builtin_amdgcn_s_waitcnt(0xc07f);
     __builtin_amdgcn_wave_barrier();
     double wv[QR_NB / 4];
@@ -226,63 +228,98 @@ __device__ __forceinline__ void reflector(double alpha, double xn2, double &tau,
 
 // Panel factorisation with register-resident columns: wave w owns panel
 // columns c2 = QR_WAVES k + w (slot k), lane l holds rows l + 64 s (slot s).
-// Step c reads reflector c (the unscaled pivot column in LDS, tau and the
-// scale published by its owner during step c-1).  The owner of column c+1
-// works with lookahead: it applies reflector c to column c+1 first,
-// publishes that column, its squared norm below the diagonal and its own
-// reflector scalars (the square root and divisions of dlarfg), and only
-// then updates its other columns, while the other waves update theirs; one
-// barrier per step.
+// Reflector c is the unscaled pivot column in P (column c, written once by
+// its owner) with tau and the scale 1/(alpha - beta) in s_tau / s_sc.  The
+// waves synchronise through a counter instead of a barrier per column:
+// s_pub = c + 1 once reflector c is published (release store after the
+// column and scalars; consumers acquire-load it, sleeping while it is
+// behind).  For reflector c every wave applies it to its live columns in
+// order; the owner of column c+1 first updates that column, publishes it
+// with its reflector scalars (the square root and divisions of dlarfg) at
+// raised priority, and only then updates its other columns, so the pivot
+// chain never waits for the bulk of the updates.  The spin is bounded: a
+// wait that runs out flags the factorisation as failed and goes on.
+__device__ __forceinline__ int pub_load(const int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void pub_store(int *p, int v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void pub_wait(const int *p, int need, int *bad)
+{
+    for (int it = 0; pub_load(p) < need; it++) {
+        if (it >= (1 << 18)) {
+            *bad = 1;
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
 template <int MR>
-__device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int ld, double *xcol,
-                         double *s_tau, double *s_sc, double *s_beta, double *Tl, int *bad)
+__device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int ld, int *s_pub, double *s_tau,
+                         double *s_sc, double *s_beta, double *Tl, int *bad)
 {
     constexpr int NS = QR_NB / QR_WAVES; // column slots per wave
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // wave-uniform: scalar branches
     double pc[NS][MR];
+    const double *A0 = A + (long long)p0 * n + p0; // the panel's top-left entry
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+        const double *colp = A0 + (long long)min(QR_WAVES * k + wave, nb - 1) * n;
+#pragma unroll
+        for (int s = 0; s < MR; s++) pc[k][s] = colp[min(lane + 64 * s, m - 1)];
+    }
+    bool nonfinite = false;
 #pragma unroll
     for (int k = 0; k < NS; k++)
 #pragma unroll
         for (int s = 0; s < MR; s++) {
-            const int c2 = QR_WAVES * k + wave, r = lane + 64 * s;
-            pc[k][s] = (c2 < nb && r < m) ? A[(p0 + r) + (long long)(p0 + c2) * n] : 0.0;
+            pc[k][s] = (QR_WAVES * k + wave < nb && lane + 64 * s < m) ? pc[k][s] : 0.0;
+            nonfinite |= !isfinite(pc[k][s]);
         }
-#pragma unroll
-    for (int k = 0; k < NS; k++)
-#pragma unroll
-        for (int s = 0; s < MR; s++)
-            if (!isfinite(pc[k][s])) *bad = 1;
-    // column 0 (wave 0, slot 0): publish it with its reflector
-    if (wave == 0) {
+    // publish column c2 = col (slot k of this wave) as reflector c2
+    auto publish = [&](const double (&col)[MR], int c2) {
         double q = 0.0;
 #pragma unroll
         for (int s = 0; s < MR; s++) {
             const int r = lane + 64 * s;
-            if (r < m) xcol[r] = pc[0][s];
-            if (r > 0 && r < m) q += pc[0][s] * pc[0][s];
+            if (r < m) P[c2 * ld + r] = col[s];
+            if (r > c2 && r < m) q += col[s] * col[s];
         }
         q = wave_sum(q);
-        const double alpha = bcast_lane(pc[0][0], 0);
+        const double alpha = bcast_lane(col[0], c2);
         double tau, sc, beta;
         reflector(alpha, q, tau, sc, beta);
         if (lane == 0) {
-            s_tau[0] = tau;
-            s_sc[0] = sc;
-            s_beta[0] = beta;
-            Tl[0] = tau;
+            s_tau[c2] = tau;
+            s_sc[c2] = sc;
+            s_beta[c2] = beta;
+            Tl[c2 * (QR_NB + 1)] = tau;
             if (!isfinite(tau) || !isfinite(sc)) *bad = 1;
         }
+        pub_store(s_pub, c2 + 1);
+    };
+    if (wave == 0) {
+        __builtin_amdgcn_s_setprio(2);
+        publish(pc[0], 0);
+        __builtin_amdgcn_s_setprio(0);
     }
-    __syncthreads();
     for (int c = 0; c < nb; c++) {
-        const double *xc = xcol + (c & 1) * m;
-        double *xn = xcol + ((c + 1) & 1) * m;
+        // columns of this wave still to update: c2 > c
+        if (QR_WAVES * (NS - 1) + wave <= c) break;
+        pub_wait(s_pub, c + 1, bad);
         const double tau = s_tau[c], sc = s_sc[c];
+        const double *xc = P + c * ld;
         double x[MR];
 #pragma unroll
         for (int s = 0; s < MR; s++) {
             const int r = lane + 64 * s;
-            x[s] = (r > c && r < m) ? xc[r] : 0.0;
+            const double raw = xc[min(r, m - 1)];
+            x[s] = (r > c && r < m) ? raw : 0.0;
         }
         const int cl = c; // the pivot row c < QR_NB <= 64 sits in register slot 0
         const bool owner = c + 1 < nb && ((c + 1) % QR_WAVES) == wave;
@@ -296,42 +333,22 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
             for (int s = 0; s < MR; s++) col[s] -= ws * x[s];
             if (lane == cl) col[0] -= w;
         };
-        if (owner && tau != 0.0) {
-            // lookahead: column c+1 first
-#pragma unroll
-            for (int k = 0; k < NS; k++) {
-                if (k == k1) {
-                    double acc = 0.0;
-#pragma unroll
-                    for (int s = 0; s < MR; s++) acc += x[s] * pc[k][s];
-                    apply(pc[k], wave_sum(acc));
-                }
-            }
-        }
         if (owner) {
-            double q = 0.0, av = 0.0;
+            // lookahead: column c+1 first, then publish it as reflector c+1
+            __builtin_amdgcn_s_setprio(2);
 #pragma unroll
             for (int k = 0; k < NS; k++) {
                 if (k == k1) {
+                    if (tau != 0.0) {
+                        double acc = 0.0;
 #pragma unroll
-                    for (int s = 0; s < MR; s++) {
-                        const int r = lane + 64 * s;
-                        if (r < m) xn[r] = pc[k][s];
-                        if (r > c + 1 && r < m) q += pc[k][s] * pc[k][s];
+                        for (int s = 0; s < MR; s++) acc += x[s] * pc[k][s];
+                        apply(pc[k], wave_sum(acc));
                     }
-                    av = bcast_lane(pc[k][0], c + 1);
+                    publish(pc[k], c + 1);
                 }
             }
-            q = wave_sum(q);
-            double t1, s1, b1;
-            reflector(av, q, t1, s1, b1);
-            if (lane == 0) {
-                s_tau[c + 1] = t1;
-                s_sc[c + 1] = s1;
-                s_beta[c + 1] = b1;
-                Tl[(c + 1) * (QR_NB + 1)] = t1;
-                if (!isfinite(t1) || !isfinite(s1)) *bad = 1;
-            }
+            __builtin_amdgcn_s_setprio(0);
         }
         if (tau != 0.0) {
             double d[NS];
@@ -353,8 +370,8 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
                 if (c2 > c && c2 < nb && !(owner && k == k1)) apply(pc[k], d[k]);
             }
         }
-        __syncthreads();
     }
+    __syncthreads();
     // final panel: R above and on the diagonal, scaled reflectors below;
     // into LDS (for T and the trailing update) and back to A
 #pragma unroll
@@ -362,6 +379,7 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
         const int c2 = QR_WAVES * k + wave;
         if (c2 < nb) {
             const double sc = s_sc[c2], beta = s_beta[c2];
+            double *colp = A + (long long)(p0 + c2) * n + p0;
 #pragma unroll
             for (int s = 0; s < MR; s++) {
                 const int r = lane + 64 * s;
@@ -370,37 +388,39 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
                     if (r == c2) v = beta;
                     else if (r > c2 && sc != 0.0) v *= sc;
                     P[c2 * ld + r] = v;
-                    A[(p0 + r) + (long long)(p0 + c2) * n] = v;
+                    colp[r] = v;
                 }
             }
         }
     }
+    if (nonfinite) *bad = 1;
     __syncthreads();
 }
 
 // T of the compact WY form Q = I - V T V^T of the panel.  For Householder
 // reflectors tau_i = 2 / v_i^T v_i, which gives T^-1 = diag(1/tau) +
 // triu(V^T V, 1) (reflectors with tau = 0 are the identity: their row and
-// column of T vanish).  V^T V on MFMA: waves w and w+4 take one 16x16 tile
-// over the two halves of the rows, each with two interleaved accumulators;
-// then lane c of wave 0 solves T^-1 t = e_c for column c of T in column
-// order, so the dependent chain is 32 steps long.
+// column of T vanish).  V^T V on MFMA: the QR_NT^2 16x16 tiles are spread
+// over the waves, each tile's k range split over QR_WAVES / QR_NT^2 waves
+// whose partial sums meet in Y; then lane r of wave 0 builds row r of T
+// in column order, so the dependent chain is QR_NB steps long.  Y holds
+// QR_WAVES / QR_NT^2 blocks of QR_NB x QR_NB.
 __device__ void qr_tfactor(const double *P, int ld, int nb, int m, double *Tl, double *Y)
 {
-    static_assert(QR_WAVES == 8, "two waves per Y tile");
+    constexpr int KS = QR_WAVES / (QR_NT * QR_NT); // waves per tile
+    static_assert(KS >= 2, "two partial sums at least");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lc = lane & 15, lr = lane >> 4;
-    double *Y2 = Y + QR_NB * QR_NB;     // partial sums of the second half
-    double *Ts = Y2 + QR_NB * QR_NB;    // columns of T
+    double *Ts = Y + QR_NB * QR_NB; // columns of T, in the second block once Y is summed
     {
-        const int tile = wave & 3, half = wave >> 2;
-        const int ti = tile >> 1, tj = tile & 1;
+        const int tile = wave % (QR_NT * QR_NT), part = wave / (QR_NT * QR_NT);
+        const int ti = tile / QR_NT, tj = tile % QR_NT;
         const int ci = ti * 16 + lc, cj = tj * 16 + lc;
-        const int steps = (m + 3) >> 2, hs = (steps + 1) >> 1;
-        const int s0 = half ? hs : 0, s1 = half ? steps : hs;
+        const int steps = (m + 3) >> 2, per = (steps + KS - 1) / KS;
+        const int s0 = part * per, s1 = min(steps, s0 + per);
         f64x4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
         // batches of 8 k-steps: all 16 branch-free LDS reads first, then the
-        // MFMAs; rows past this half's range read as row m (zero)
+        // MFMAs; rows past this wave's range read as row m (zero)
         for (int st0 = s0; st0 < s1; st0 += 8) {
             double va[8], vb[8];
 #pragma unroll
@@ -416,7 +436,7 @@ __device__ void qr_tfactor(const double *P, int ld, int nb, int m, double *Tl, d
                 acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(va[u + 1], vb[u + 1], acc1, 0, 0, 0);
             }
         }
-        double *dst = half ? Y2 : Y;
+        double *dst = Y + part * QR_NB * QR_NB;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int i = ti * 16 + Mfma<double>::row(lane, v), c = tj * 16 + lc;
@@ -424,13 +444,18 @@ __device__ void qr_tfactor(const double *P, int ld, int nb, int m, double *Tl, d
         }
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < QR_NB * QR_NB; idx += blockDim.x) Y[idx] += Y2[idx];
+    for (int idx = threadIdx.x; idx < QR_NB * QR_NB; idx += blockDim.x) {
+        double sum = Y[idx];
+#pragma unroll
+        for (int p = 1; p < KS; p++) sum += Y[p * QR_NB * QR_NB + idx];
+        Y[idx] = sum;
+    }
     __syncthreads();
     if (wave == 0 && lane < QR_NB) {
         // lane r builds row r of T column by column (dlarft, forward,
         // columnwise): T(r, i) = -tau_i sum_{k<i} T(r, k) Y(k, i), T(i, i) =
-        // tau_i; column i of Y is one broadcast read per pair of entries and
-        // the sum runs as four independent chains
+        // tau_i; column i of Y is one broadcast read per entry and the sum
+        // runs as four independent chains
         const int r = lane;
         double trow[QR_NB];
 #pragma unroll
@@ -439,8 +464,8 @@ __device__ void qr_tfactor(const double *P, int ld, int nb, int m, double *Tl, d
             double s4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int k = 0; k < i; k++) s4[k & 3] += trow[k] * Y[k + i * QR_NB];
-            const double s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
-            trow[i] = (r < i) ? -tau * s : (r == i ? tau : 0.0);
+            const double sum = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+            trow[i] = (r < i) ? -tau * sum : (r == i ? tau : 0.0);
             __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
@@ -467,11 +492,10 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_udv(QrArgs a)
     double *P = lds;                           // QR_NB x n panel, P[c*ld + r]
     double *Tl = P + (size_t)QR_NB * ld;       // QR_NB x QR_NB
     double *Wl = Tl + QR_NB * QR_NB;           // QR_WAVES x (QR_NB x 16); also Y and T scratch
-    double *xcol = Wl + QR_WAVES * QR_NB * 16; // 2 x n pivot columns
-    double *s_tau = xcol + 2 * n;
+    double *s_tau = Wl + QR_WAVES * QR_NB * 16;
     double *s_sc = s_tau + QR_NB;
     double *s_beta = s_sc + QR_NB;
-    __shared__ int s_bad, s_ntau;
+    __shared__ int s_bad, s_ntau, s_pub;
     if (tid == 0) { s_bad = 0; s_ntau = 0; }
     __syncthreads();
 
@@ -481,8 +505,9 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_udv(QrArgs a)
         const int nb = min(QR_NB, n - p0);
         const int m = n - p0;
         for (int idx = tid; idx < QR_NB * QR_NB; idx += blockDim.x) Tl[idx] = 0.0;
+        if (tid == 0) s_pub = 0;
         __syncthreads();
-        qr_panel<MR>(A, n, p0, nb, m, P, ld, xcol, s_tau, s_sc, s_beta, Tl, &s_bad);
+        qr_panel<MR>(A, n, p0, nb, m, P, ld, &s_pub, s_tau, s_sc, s_beta, Tl, &s_bad);
         if (tid == 0)
             for (int c = 0; c < nb; c++) s_ntau += (s_tau[c] != 0.0); // non-trivial reflectors
         qr_tfactor(P, ld, nb, m, Tl, Wl);
@@ -533,8 +558,7 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_udv(QrArgs a)
 
 static inline size_t qr_lds_bytes(int n)
 {
-    return sizeof(double) * ((size_t)QR_NB * (n + 1) + QR_NB * QR_NB + QR_WAVES * QR_NB * 16 + 2 * (size_t)n +
-                             4 * QR_NB);
+    return sizeof(double) * ((size_t)QR_NB * (n + 1) + QR_NB * QR_NB + QR_WAVES * QR_NB * 16 + 4 * QR_NB);
 }
 
 // Explicit Q = H_0 H_1 ... H_{npan-1} (dorgqr) from the reflectors and T
@@ -654,42 +678,42 @@ __global__ __launch_bounds__(FQ_WAVES * 64) void k_qr_formq_reg(QrArgs a)
         __syncthreads();
         if (jb >= nblk || j0 < p0) continue; // columns j < p0 are still e_j below row p0
         // W = V^T X[p0:, block] (rows k >= p0 of X; V(k, c) = panel column c, row k - p0)
-        f64x4 w0 = {0, 0, 0, 0}, w1 = {0, 0, 0, 0};
+        f64x4 w[QR_NT];
+#pragma unroll
+        for (int h = 0; h < QR_NT; h++) w[h] = f64x4{0, 0, 0, 0};
 #pragma unroll
         for (int u = 0; u < NU; u++) {
             if (4 * u >= p0 && 4 * u < n) {
                 const int k = 4 * u + lr - p0;
-                const double a0 = panel_vb(P, ld, lc, k, nb, m);
-                const double a1 = panel_vb(P, ld, 16 + lc, k, nb, m);
-                w0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, xb[u], w0, 0, 0, 0);
-                w1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, xb[u], w1, 0, 0, 0);
+#pragma unroll
+                for (int h = 0; h < QR_NT; h++)
+                    w[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(panel_vb(P, ld, 16 * h + lc, k, nb, m), xb[u], w[h], 0, 0, 0);
             }
         }
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int r = Mfma<double>::row(lane, v);
-            Wl[r * 16 + lc] = w0[v];
-            Wl[(16 + r) * 16 + lc] = w1[v];
-        }
+        for (int h = 0; h < QR_NT; h++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) Wl[(16 * h + Mfma<double>::row(lane, v)) * 16 + lc] = w[h][v];
         __builtin_amdgcn_s_waitcnt(0xc07f);
         __builtin_amdgcn_wave_barrier();
         // W' = T W
-        f64x4 u0 = {0, 0, 0, 0}, u1 = {0, 0, 0, 0};
+        f64x4 uw[QR_NT];
+#pragma unroll
+        for (int h = 0; h < QR_NT; h++) uw[h] = f64x4{0, 0, 0, 0};
 #pragma unroll
         for (int k0 = 0; k0 < QR_NB; k0 += 4) {
             const int k = k0 + lr;
             const double bw = Wl[k * 16 + lc];
-            u0 = __builtin_amdgcn_mfma_f64_16x16x4f64(Tl[lc + k * QR_NB], bw, u0, 0, 0, 0);
-            u1 = __builtin_amdgcn_mfma_f64_16x16x4f64(Tl[(16 + lc) + k * QR_NB], bw, u1, 0, 0, 0);
+#pragma unroll
+            for (int h = 0; h < QR_NT; h++)
+                uw[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(Tl[(16 * h + lc) + k * QR_NB], bw, uw[h], 0, 0, 0);
         }
         __builtin_amdgcn_s_waitcnt(0xc07f);
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int r = Mfma<double>::row(lane, v);
-            Wl[r * 16 + lc] = u0[v];
-            Wl[(16 + r) * 16 + lc] = u1[v];
-        }
+        for (int h = 0; h < QR_NT; h++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) Wl[(16 * h + Mfma<double>::row(lane, v)) * 16 + lc] = uw[h][v];
         __builtin_amdgcn_s_waitcnt(0xc07f);
         __builtin_amdgcn_wave_barrier();
         double wv[QR_NB / 4];
