@@ -9,10 +9,11 @@
 // reflectors), which replaces the LU determinant of U in the sign
 // (src/linalg.c:1044-1083).
 //
-//   panel (QR_NB = 16 columns): for n <= 256 one wave holds the whole
-//     panel in registers and factors it without a barrier (dgeqr2: one wave
-//     reduction per norm and per column update); taller matrices spread the
-//     columns round-robin over the waves with one barrier per column;
+//   panel (QR_NB columns, register-resident, owned round-robin by the
+//     waves): unblocked dgeqr2 with one wave reduction per norm and per
+//     column update; the waves follow a published-reflector counter in LDS
+//     rather than a barrier per column, and the owner of the next pivot
+//     column publishes it first (lookahead);
 //   T factor of the compact WY form (dlarft, forward, columnwise);
 //   trailing update A <- (I - V T^T V^T) A and Q <- (I - V T V^T) Q on FP64
 //     MFMA 16x16x4: each wave takes 16-column blocks, W = V^T A (QR_NB / 16
@@ -794,78 +795,104 @@ __global__ __launch_bounds__(256) void k_trsm_unit_upper(TrsmArgs a)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lc = lane & 15, lr = lane >> 4;
     const int c0 = cb * TS_NB, cw = min(TS_NB, n - c0);
-    const double *U = a.U + (long long)b * a.sU;
-    const double *Bm = a.B + (long long)b * a.sB;
-    double *X = a.X + (long long)b * a.sX;
+    // global-space views: every load below comes from a clamped, valid
+    // address and is zeroed afterwards, so a batch is in flight at once
+    const gdouble *U = (const gdouble *)(a.U + (long long)b * a.sU);
+    const gdouble *Bm = (const gdouble *)(a.B + (long long)b * a.sB);
+    gdouble *X = (gdouble *)(a.X + (long long)b * a.sX);
     extern __shared__ double lds[];
     const int ldx = n + 4;              // Xs[c*ldx + r]: column-major, conflict-free fragment reads
     double *Xs = lds;
     double *Ud = Xs + (size_t)TS_NB * ldx; // U_II, Ud[i*(TS_NB+1) + k]
-    for (int idx = tid; idx < TS_NB * n; idx += blockDim.x) {
-        const int r = idx % n, c = idx / n;
-        Xs[c * ldx + r] = c < cw ? Bm[r + (long long)(c0 + c) * n] : 0.0;
+    // B block: thread t takes rows t, t + 256, ... of every column
+    for (int r = tid; r < n; r += blockDim.x) {
+        double v[TS_NB];
+#pragma unroll
+        for (int c = 0; c < TS_NB; c++) v[c] = Bm[r + (long long)(c0 + min(c, cw - 1)) * n];
+#pragma unroll
+        for (int c = 0; c < TS_NB; c++) Xs[c * ldx + r] = c < cw ? v[c] : 0.0;
     }
     __syncthreads();
     const int ti = wave >> 1, tj = wave & 1;
     for (int I = nblk - 1; I >= 0; I--) {
         const int r0 = I * TS_NB, ib = min(TS_NB, n - r0);
+        // U_II into LDS (loads issued before the product below)
+        double ud[TS_NB * TS_NB / 256];
+#pragma unroll
+        for (int u = 0; u < TS_NB * TS_NB / 256; u++) {
+            const int idx = tid + 256 * u;
+            const int i = idx % TS_NB, k = idx / TS_NB;
+            ud[u] = U[(r0 + min(i, ib - 1)) + (long long)(r0 + min(k, ib - 1)) * n];
+        }
         // S = B_I - U[r0:r0+ib, r0+ib:n] X[r0+ib:n]: tile (ti, tj), rows i, columns c
         {
-            f64x4 acc;
+            f64x4 acc0, acc1 = {0, 0, 0, 0};
             const int i0 = r0 + ti * 16, cc = tj * 16 + lc;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int i = i0 + Mfma<double>::row(lane, v);
-                acc[v] = (i < r0 + ib) ? Xs[cc * ldx + i] : 0.0;
+                acc0[v] = (i < r0 + ib) ? Xs[cc * ldx + min(i, n - 1)] : 0.0;
             }
             const bool iok = i0 + lc < r0 + ib;
-            for (int k0 = r0 + ib; k0 < n; k0 += 32) {
-                double ua[8], xb[8];
+            const int ic = min(i0 + lc, n - 1);
+            for (int k0 = r0 + ib; k0 < n; k0 += 64) {
+                double ua[16], xb[16];
 #pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    const int k = k0 + 4 * u + lr;
-                    ua[u] = (iok && k < n) ? U[(i0 + lc) + (long long)k * n] : 0.0;
-                    xb[u] = k < n ? Xs[cc * ldx + k] : 0.0;
+                for (int u = 0; u < 16; u++) {
+                    const int kc = min(k0 + 4 * u + lr, n - 1);
+                    ua[u] = U[ic + (long long)kc * n];
+                    xb[u] = Xs[cc * ldx + kc];
                 }
 #pragma unroll
-                for (int u = 0; u < 8; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-ua[u], xb[u], acc, 0, 0, 0);
+                for (int u = 0; u < 16; u++) {
+                    const bool ok = k0 + 4 * u + lr < n;
+                    ua[u] = (ok && iok) ? -ua[u] : 0.0;
+                    xb[u] = ok ? xb[u] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 16; u += 2) {
+                    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(ua[u], xb[u], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ua[u + 1], xb[u + 1], acc1, 0, 0, 0);
+                }
             }
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int i = i0 + Mfma<double>::row(lane, v);
-                if (i < r0 + ib) Xs[cc * ldx + i] = acc[v];
+                if (i < r0 + ib) Xs[cc * ldx + i] = acc0[v] + acc1[v];
             }
         }
-        for (int idx = tid; idx < TS_NB * TS_NB; idx += blockDim.x) {
+#pragma unroll
+        for (int u = 0; u < TS_NB * TS_NB / 256; u++) {
+            const int idx = tid + 256 * u;
             const int i = idx % TS_NB, k = idx / TS_NB;
-            Ud[i * (TS_NB + 1) + k] = (i < ib && k < ib) ? U[(r0 + i) + (long long)(r0 + k) * n] : 0.0;
+            Ud[i * (TS_NB + 1) + k] = (i < ib && k < ib) ? ud[u] : 0.0;
         }
         __syncthreads();
         if (wave == 0 && lane < TS_NB) {
-            // column `lane`: x_i = s_i - sum_{k>i} U_ik x_k, rows ib-1 .. 0
-            double x[TS_NB];
+            // column `lane`, right-looking: once x_i = s_i is final it is
+            // removed from every s_k, k < i, so the dependent chain is one
+            // FMA per row
             double *col = Xs + lane * ldx + r0;
+            double sv[TS_NB];
 #pragma unroll
-            for (int i = TS_NB - 1; i >= 0; i--) {
+            for (int i = 0; i < TS_NB; i++) sv[i] = col[min(i, ib - 1)];
+#pragma unroll
+            for (int i = TS_NB - 1; i >= 1; i--) {
                 if (i < ib) {
-                    double sacc = col[i];
 #pragma unroll
-                    for (int k = i + 1; k < TS_NB; k++)
-                        if (k < ib) sacc -= Ud[i * (TS_NB + 1) + k] * x[k];
-                    x[i] = sacc;
-                } else {
-                    x[i] = 0.0;
+                    for (int k = 0; k < i; k++) sv[k] -= Ud[k * (TS_NB + 1) + i] * sv[i];
                 }
             }
 #pragma unroll
             for (int i = 0; i < TS_NB; i++)
-                if (i < ib) col[i] = x[i];
+                if (i < ib) col[i] = sv[i];
         }
         __syncthreads();
     }
-    for (int idx = tid; idx < TS_NB * n; idx += blockDim.x) {
-        const int r = idx % n, c = idx / n;
-        if (c < cw) X[r + (long long)(c0 + c) * n] = Xs[c * ldx + r];
+    for (int r = tid; r < n; r += blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < TS_NB; c++)
+            if (c < cw) X[r + (long long)(c0 + c) * n] = Xs[c * ldx + r];
     }
 }
 
