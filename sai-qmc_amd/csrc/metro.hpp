@@ -243,7 +243,7 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
     __syncthreads();
 
     int naccept = 0, negflips = 0, P = 0;
-    T gc[W], gr[W];
+    T gc[W];
     for (int i0 = a.s0; i0 < a.s1; i0 += W) {
         const int wn = min(W, a.s1 - i0);
         // 1. corrected window columns -> Sc, rows -> Sr, the window block -> Bw
@@ -320,30 +320,41 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
         }
         __syncthreads();
 
-        // 3. full pairs of the window's accepts, appended to C, V
-        const unsigned mask = s_mask;
-        if (act) {
+        // 3. full pairs of the window's accepts, appended to C, V.  The c
+        // and v halves are independent: with 2n threads or more, thread j <
+        // n forms c_q(j) and thread n + j forms v_q(j); otherwise a thread
+        // forms both, one after the other.  The mask is wave-uniform (scalar
+        // branches) and the window-restricted pair entries of one accept
+        // are read from LDS as one batch.
+        const unsigned mask = __builtin_amdgcn_readfirstlane(s_mask);
+        const bool split = (int)blockDim.x >= 2 * n;
+        const int jj = (split && j >= n) ? j - n : j;
+        if (jj < n) {
+            const int nparts = split ? 1 : 2;
+            for (int part = 0; part < nparts; part++) {
+                const bool vside = split ? (j >= n) : (part == 1);
+                const T *Sx = vside ? Sr : Sc;
+                const T *coef = vside ? cwin : vwin;
+                T *dst = vside ? Vp : Cp;
 #pragma unroll
-            for (int w = 0; w < W; w++) {
-                gc[w] = (w < wn) ? Sc[w * (n + 1) + j] : T(0);
-                gr[w] = (w < wn) ? Sr[w * (n + 1) + j] : T(0);
-            }
+                for (int w = 0; w < W; w++) {
+                    const T x = Sx[min(w, wn - 1) * (n + 1) + jj];
+                    gc[w] = (w < wn) ? x : T(0);
+                }
 #pragma unroll
-            for (int w = 0; w < W; w++) {
-                if ((mask >> w) & 1u) {
-                    T s = gc[w], t = gr[w];
+                for (int w = 0; w < W; w++) {
+                    if ((mask >> w) & 1u) {
+                        T cf[W];
 #pragma unroll
-                    for (int u = 0; u < w; u++) {
-                        if ((mask >> u) & 1u) {
-                            s -= gc[u] * vwin[u * W + w];
-                            t -= cwin[u * W + w] * gr[u];
-                        }
+                        for (int u = 0; u < w; u++) cf[u] = coef[u * W + w];
+                        T x = gc[w];
+#pragma unroll
+                        for (int u = 0; u < w; u++)
+                            if ((mask >> u) & 1u) x -= gc[u] * cf[u];
+                        gc[w] = vside ? x : facw[w] * ((jj == i0 + w ? T(1) : T(0)) - x);
+                        const int q = P + __builtin_popcount(mask & ((1u << w) - 1u));
+                        dst[(long long)q * n + jj] = gc[w];
                     }
-                    gc[w] = facw[w] * ((j == i0 + w ? T(1) : T(0)) - s);
-                    gr[w] = t;
-                    const int q = P + __builtin_popcount(mask & ((1u << w) - 1u));
-                    Cp[(long long)q * n + j] = gc[w];
-                    Vp[(long long)q * n + j] = gr[w];
                 }
             }
         }
