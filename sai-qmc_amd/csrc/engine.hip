@@ -71,9 +71,16 @@ __global__ void k_copy_d(const double *src, long long sS, double *dst, long long
 {
     const int b = blockIdx.y;
     if (b >= batch) return;
-    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < len;
-         idx += (long long)gridDim.x * blockDim.x)
-        dst[b * sD + idx] = src[b * sS + idx];
+    // four elements per thread, loads from clamped indices issued together
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long base = blockIdx.x * (long long)blockDim.x + threadIdx.x; base < len; base += 4 * stride) {
+        double v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = src[b * sS + min(base + k * stride, len - 1)];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (base + k * stride < len) dst[b * sD + base + k * stride] = v[k];
+    }
 }
 
 // acceptance counters at the start of the next measured sweep: a bin adds
@@ -112,10 +119,19 @@ __global__ void k_scale_cols_d(const double *expK, const double *d, long long sd
 {
     const int b = blockIdx.y;
     if (b >= batch) return;
-    const long long nn = (long long)n * n;
-    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < nn;
-         idx += (long long)gridDim.x * blockDim.x)
-        out[b * sO + idx] = expK[idx] * d[b * sdv + idx / n];
+    const int nn = n * n, stride = gridDim.x * blockDim.x;
+    for (int base = blockIdx.x * blockDim.x + threadIdx.x; base < nn; base += 4 * stride) {
+        double e[4], s[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int idx = min(base + k * stride, nn - 1);
+            e[k] = expK[idx];
+            s[k] = d[b * sdv + idx / n];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (base + k * stride < nn) out[b * sO + base + k * stride] = e[k] * s[k];
+    }
 }
 
 __global__ void k_scale_cols_f(const float *expK, const float *d, long long sdv, float *out,
@@ -134,42 +150,54 @@ __global__ void k_scale_cols_f(const float *expK, const float *d, long long sdv,
 // |Ds| <= 1, src/linalg.c:963-984): since 1 + U D T = U Db N with
 // N = Db^-1 U^T + Ds T, the Green's function is G = N^-1 X, X = Db^-1 U^T.
 // Writes N, X and the sign of prod_{|D_i|>1} D_i.
-__global__ void k_build_N(const double *U, const double *D, const double *T, long long sU, long long sD, double *N,
-                          double *X, int *bigsign, int n, int batch, const int *status)
+// N = Db^-1 U^T + Ds T and X = Db^-1 U^T (Db = max(|D|, 1) with D's sign,
+// Ds = D / Db) per 32 x 32 tile; U^T is read through an LDS tile so that
+// both the U reads and the N, X, T accesses are coalesced.  Block (0, b)
+// also records the sign of prod_{|D_i| > 1} D_i.
+#define BN_T 32
+__global__ __launch_bounds__(256) void k_build_N(const double *U, const double *D, const double *T, long long sU,
+                                                 long long sD, double *N, double *X, int *bigsign, int n, int batch,
+                                                 const int *status)
 {
     const int b = blockIdx.y;
     if (b >= batch || status[b]) return;
     const double *Ub = U + b * sU, *Tb = T + b * sU, *Db = D + b * sD;
     double *Nb = N + (long long)b * n * n, *Xb = X + (long long)b * n * n;
-    const long long nn = (long long)n * n, stride = (long long)gridDim.x * blockDim.x;
-    for (long long base = blockIdx.x * (long long)blockDim.x + threadIdx.x; base < nn; base += 4 * stride) {
-        double u[4], t[4], d[4];
+    const int nt = (n + BN_T - 1) / BN_T;
+    const int i0 = (blockIdx.x % nt) * BN_T, j0 = (blockIdx.x / nt) * BN_T;
+    const int tx = threadIdx.x % BN_T, ty = threadIdx.x / BN_T; // 32 x 8
+    __shared__ double tl[BN_T][BN_T + 1];
+    double u[BN_T / 8];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const long long idx = base + k * stride;
-            const int i = (int)(idx % n), j = (int)(idx / n);
-            const bool ok = idx < nn;
-            u[k] = ok ? Ub[j + (long long)i * n] : 0.0;
-            t[k] = ok ? Tb[idx] : 0.0;
-            d[k] = ok ? Db[i] : 1.0;
-        }
+    for (int k = 0; k < BN_T / 8; k++) {
+        const int i = min(i0 + ty + 8 * k, n - 1), j = min(j0 + tx, n - 1);
+        u[k] = Ub[j + (long long)i * n]; // U(j, i) = U^T(i, j)
+    }
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const long long idx = base + k * stride;
-            if (idx < nn) {
-                const bool big = fabs(d[k]) > 1.0;
-                const double x = big ? u[k] / d[k] : u[k];
-                Xb[idx] = x;
-                Nb[idx] = x + (big ? 1.0 : d[k]) * t[k];
-            }
+    for (int k = 0; k < BN_T / 8; k++) tl[ty + 8 * k][tx] = u[k];
+    __syncthreads();
+    const int i = i0 + tx;
+    const double d = Db[min(i, n - 1)];
+    const bool big = fabs(d) > 1.0;
+    double t[BN_T / 8];
+#pragma unroll
+    for (int k = 0; k < BN_T / 8; k++) t[k] = Tb[min(i, n - 1) + (long long)min(j0 + ty + 8 * k, n - 1) * n];
+#pragma unroll
+    for (int k = 0; k < BN_T / 8; k++) {
+        const int j = j0 + ty + 8 * k;
+        if (i < n && j < n) {
+            const double uu = tl[tx][ty + 8 * k];
+            const double x = big ? uu / d : uu;
+            Xb[i + (long long)j * n] = x;
+            Nb[i + (long long)j * n] = x + (big ? 1.0 : d) * t[k];
         }
     }
     if (blockIdx.x == 0) {
         __shared__ int neg;
         if (threadIdx.x == 0) neg = 0;
         __syncthreads();
-        for (int i = threadIdx.x; i < n; i += blockDim.x)
-            if (fabs(Db[i]) > 1.0 && Db[i] < 0.0) atomicXor(&neg, 1);
+        for (int r = threadIdx.x; r < n; r += blockDim.x)
+            if (fabs(Db[r]) > 1.0 && Db[r] < 0.0) atomicXor(&neg, 1);
         __syncthreads();
         if (threadIdx.x == 0) bigsign[b] = neg ? -1 : 1;
     }
@@ -228,12 +256,13 @@ __global__ void k_drift_copy(double *G, float *Gf, const double *Gnew, long long
     int bad = 0;
     for (long long base = blockIdx.x * (long long)blockDim.x + threadIdx.x; base < nn; base += 4 * stride) {
         double v[4], o[4];
+        // loads from clamped indices, issued together
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            const long long idx = base + k * stride;
-            const bool ok = idx < nn;
-            v[k] = ok ? Gnew[b * sG + idx] : 0.0;
-            o[k] = (ok && drift) ? (from_f32 ? (double)Gf[b * sG + idx] : G[b * sG + idx]) : 0.0;
+            const long long idx = min(base + k * stride, nn - 1);
+            v[k] = Gnew[b * sG + idx];
+            o[k] = 0.0;
+            if (drift) o[k] = from_f32 ? (double)Gf[b * sG + idx] : G[b * sG + idx];
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -476,7 +505,8 @@ static int udv_inv_one_plus(Work &w, const Udv &u, double *Gout, long long sG)
     ProfScope ps_(DQMC_GPU_PROF_INV, w.s);
     const int n = w.n;
     const long long nn = (long long)n * n;
-    hipLaunchKernelGGL(k_build_N, grid2((long long)n * n / 4, w.nb), dim3(256), 0, w.s, u.U, u.D, u.T, u.sU, u.sD, w.W1, w.W4, w.I1, n,
+    const int bnt = (n + BN_T - 1) / BN_T;
+    hipLaunchKernelGGL(k_build_N, dim3(bnt * bnt, w.nb), dim3(256), 0, w.s, u.U, u.D, u.T, u.sU, u.sD, w.W1, w.W4, w.I1, n,
                        w.nb, w.status);
     if (launch_check("k_build_N")) return 1;                         // W1 = N, W4 = X
     if (qr(w, w.W1, w.W2, nn, w.W3, nn, w.V3, n, w.I2, 1)) return 1; // W2 = Q_N, V3 = D_N, W3 = Rn_N

@@ -509,8 +509,11 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_udv(QrArgs a)
         if (tid == 0) s_pub = 0;
         __syncthreads();
         qr_panel<MR>(A, n, p0, nb, m, P, ld, &s_pub, s_tau, s_sc, s_beta, Tl, &s_bad);
-        if (tid == 0)
-            for (int c = 0; c < nb; c++) s_ntau += (s_tau[c] != 0.0); // non-trivial reflectors
+        if (wave == 0) { // non-trivial reflectors, counted with one ballot
+            const bool nz = (tid < nb) && s_tau[min(tid, QR_NB - 1)] != 0.0;
+            const unsigned long long bal = __ballot(nz);
+            if (tid == 0) s_ntau += __popcll(bal);
+        }
         qr_tfactor(P, ld, nb, m, Tl, Wl);
         for (int idx = tid; idx < QR_NB * QR_NB; idx += blockDim.x) Tw[pi * QR_NB * QR_NB + idx] = Tl[idx];
         // trailing update A[p0:, p0+nb:] <- (I - V T^T V^T) A
