@@ -276,8 +276,15 @@ __global__ void k_drift_copy(double *G, float *Gf, const double *Gnew, long long
         }
     }
     if (drift) {
+        // one atomic per workgroup: wave maxima meet in LDS first
+        __shared__ double wm[16];
         m = wave_max(m);
-        if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(&tmax[b], (unsigned long long)__double_as_longlong(m));
+        if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, wm[w]);
+            if (m > 0.0) atomicMax(&tmax[b], (unsigned long long)__double_as_longlong(m));
+        }
     }
     if (bad) atomicOr(&tbad[b], 1);
 }
