@@ -231,14 +231,16 @@ __device__ __forceinline__ void reflector(double alpha, double xn2, double &tau,
 // columns c2 = QR_WAVES k + w (slot k), lane l holds rows l + 64 s (slot s).
 // Reflector c is the unscaled pivot column in P (column c, written once by
 // its owner) with tau and the scale 1/(alpha - beta) in s_tau / s_sc.  The
-// waves synchronise through a counter instead of a barrier per column:
-// s_pub = c + 1 once reflector c is published (release store after the
-// column and scalars; consumers acquire-load it, sleeping while it is
-// behind).  For reflector c every wave applies it to its live columns in
-// order; the owner of column c+1 first updates that column, publishes it
-// with its reflector scalars (the square root and divisions of dlarfg) at
-// raised priority, and only then updates its other columns, so the pivot
-// chain never waits for the bulk of the updates.  The spin is bounded: a
+// waves synchronise through two counters instead of a barrier per column:
+// s_col = c + 1 once column c is in P and s_pub = c + 1 once its scalars
+// are (release stores; consumers acquire-load them, sleeping while they
+// are behind).  For reflector c every wave applies it to its live columns
+// in order; the owner of column c+1 starts its reduction x_c^T a_{c+1} as
+// soon as column c is out, while the publisher of c still works on the
+// square root and divisions of dlarfg, then updates column c+1 with the
+// scalars and publishes it, at raised priority, and only then updates its
+// other columns, so the pivot chain never waits for the bulk of the
+// updates.  The spin is bounded: a
 // wait that runs out flags the factorisation as failed and goes on.
 __device__ __forceinline__ int pub_load(const int *p)
 {
@@ -260,8 +262,8 @@ __device__ __forceinline__ void pub_wait(const int *p, int need, int *bad)
 }
 
 template <int MR>
-__device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int ld, int *s_pub, double *s_tau,
-                         double *s_sc, double *s_beta, double *Tl, int *bad)
+__device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int ld, int *s_col, int *s_pub,
+                         double *s_tau, double *s_sc, double *s_beta, double *Tl, int *bad)
 {
     constexpr int NS = QR_NB / QR_WAVES; // column slots per wave
     const int lane = threadIdx.x & 63;
@@ -282,7 +284,9 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
             pc[k][s] = (QR_WAVES * k + wave < nb && lane + 64 * s < m) ? pc[k][s] : 0.0;
             nonfinite |= !isfinite(pc[k][s]);
         }
-    // publish column c2 = col (slot k of this wave) as reflector c2
+    // publish column c2 = col (slot k of this wave) as reflector c2 in two
+    // steps: the column itself (s_col = c2 + 1), then, after its norm, the
+    // square root and the divisions of dlarfg, its scalars (s_pub = c2 + 1)
     auto publish = [&](const double (&col)[MR], int c2) {
         double q = 0.0;
 #pragma unroll
@@ -291,6 +295,7 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
             if (r < m) P[c2 * ld + r] = col[s];
             if (r > c2 && r < m) q += col[s] * col[s];
         }
+        pub_store(s_col, c2 + 1);
         q = wave_sum(q);
         const double alpha = bcast_lane(col[0], c2);
         double tau, sc, beta;
@@ -312,19 +317,20 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
     for (int c = 0; c < nb; c++) {
         // columns of this wave still to update: c2 > c
         if (QR_WAVES * (NS - 1) + wave <= c) break;
-        pub_wait(s_pub, c + 1, bad);
-        const double tau = s_tau[c], sc = s_sc[c];
         const double *xc = P + c * ld;
         double x[MR];
+        auto read_x = [&]() {
 #pragma unroll
-        for (int s = 0; s < MR; s++) {
-            const int r = lane + 64 * s;
-            const double raw = xc[min(r, m - 1)];
-            x[s] = (r > c && r < m) ? raw : 0.0;
-        }
+            for (int s = 0; s < MR; s++) {
+                const int r = lane + 64 * s;
+                const double raw = xc[min(r, m - 1)];
+                x[s] = (r > c && r < m) ? raw : 0.0;
+            }
+        };
         const int cl = c; // the pivot row c < QR_NB <= 64 sits in register slot 0
         const bool owner = c + 1 < nb && ((c + 1) % QR_WAVES) == wave;
         const int k1 = (c + 1) / QR_WAVES;
+        double tau = 0.0, sc = 0.0;
         // column c2 <- column c2 - w v_c, w = tau (pcc + sc v_c^T x_below), v_c = e_c + sc x
         auto apply = [&](double (&col)[MR], double dsum) {
             const double pcc = bcast_lane(col[0], cl);
@@ -335,21 +341,32 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
             if (lane == cl) col[0] -= w;
         };
         if (owner) {
-            // lookahead: column c+1 first, then publish it as reflector c+1
+            // lookahead: the reduction x_c^T a_{c+1} as soon as column c is
+            // out (it overlaps the publisher's square root and divisions),
+            // then the update with reflector c's scalars, then publish c+1
             __builtin_amdgcn_s_setprio(2);
+            pub_wait(s_col, c + 1, bad);
+            read_x();
 #pragma unroll
             for (int k = 0; k < NS; k++) {
                 if (k == k1) {
-                    if (tau != 0.0) {
-                        double acc = 0.0;
+                    double acc = 0.0;
 #pragma unroll
-                        for (int s = 0; s < MR; s++) acc += x[s] * pc[k][s];
-                        apply(pc[k], wave_sum(acc));
-                    }
+                    for (int s = 0; s < MR; s++) acc += x[s] * pc[k][s];
+                    const double dsum = wave_sum(acc);
+                    pub_wait(s_pub, c + 1, bad);
+                    tau = s_tau[c];
+                    sc = s_sc[c];
+                    if (tau != 0.0) apply(pc[k], dsum);
                     publish(pc[k], c + 1);
                 }
             }
             __builtin_amdgcn_s_setprio(0);
+        } else {
+            pub_wait(s_pub, c + 1, bad);
+            read_x();
+            tau = s_tau[c];
+            sc = s_sc[c];
         }
         if (tau != 0.0) {
             double d[NS];
@@ -496,7 +513,7 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_udv(QrArgs a)
     double *s_tau = Wl + QR_WAVES * QR_NB * 16;
     double *s_sc = s_tau + QR_NB;
     double *s_beta = s_sc + QR_NB;
-    __shared__ int s_bad, s_ntau, s_pub;
+    __shared__ int s_bad, s_ntau, s_col, s_pub;
     if (tid == 0) { s_bad = 0; s_ntau = 0; }
     __syncthreads();
 
@@ -506,9 +523,12 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_udv(QrArgs a)
         const int nb = min(QR_NB, n - p0);
         const int m = n - p0;
         for (int idx = tid; idx < QR_NB * QR_NB; idx += blockDim.x) Tl[idx] = 0.0;
-        if (tid == 0) s_pub = 0;
+        if (tid == 0) {
+            s_col = 0;
+            s_pub = 0;
+        }
         __syncthreads();
-        qr_panel<MR>(A, n, p0, nb, m, P, ld, &s_pub, s_tau, s_sc, s_beta, Tl, &s_bad);
+        qr_panel<MR>(A, n, p0, nb, m, P, ld, &s_col, &s_pub, s_tau, s_sc, s_beta, Tl, &s_bad);
         if (wave == 0) { // non-trivial reflectors, counted with one ballot
             const bool nz = (tid < nb) && s_tau[min(tid, QR_NB - 1)] != 0.0;
             const unsigned long long bal = __ballot(nz);
