@@ -66,6 +66,34 @@ def test_udv_lmul_rmul_vs_oracle(orc, eng_mod, Lx, Ly):
             assert relerr(Tg[k], To) < 1e-11
 
 
+@pytest.mark.parametrize("n", [40, 96])
+def test_udv_lmul_rmul_identity_reflectors(orc, eng_mod, n):
+    """Columns already zero below the diagonal give tau = 0 (dlarfg's
+    identity reflector, R(j,j) = alpha): an upper-triangular B gives Q = I
+    exactly, and a B with only some columns filled below the diagonal mixes
+    identity and proper reflectors within one panel."""
+    rng = np.random.default_rng(11 + n)
+    tri = np.triu(rng.standard_normal((n, n))) + np.diag(np.where(rng.random(n) < 0.5, -1.0, 1.0))
+    mixed = tri.copy()
+    for j in rng.choice(n, n // 3, replace=False):
+        mixed[j + 1:, j] = rng.standard_normal(n - j - 1)
+    B = np.array([tri.ravel(order="F"), mixed.ravel(order="F")])
+    U = np.array([np.eye(n).ravel()] * 2)
+    D = np.ones((2, n))
+    T = np.array([np.eye(n).ravel()] * 2)
+    for left in (True, False):
+        Ug, Dg, Tg = (eng_mod.udv_lmul if left else eng_mod.udv_rmul)(U, D, T, B)
+        ofn = orc.orc().orc_udv_lmul if left else orc.orc().orc_udv_rmul
+        for k in range(2):
+            Uo, Do, To = U[k].copy(), D[k].copy(), T[k].copy()
+            assert ofn(n, P(Uo), P(Do), P(To), P(B[k])) == 0
+            assert np.allclose(Dg[k], Do, rtol=1e-12, atol=0), (left, k)
+            assert np.abs(Ug[k] - Uo).max() < 1e-12, (left, k)
+            assert relerr(Tg[k], To) < 1e-12, (left, k)
+        if left:
+            assert np.array_equal(Ug[0], np.eye(n).ravel())  # all reflectors are the identity
+
+
 @pytest.mark.parametrize("Lx,Ly,nl,nr", [(4, 4, 16, 24), (6, 6, 16, 24), (8, 8, 16, 24), (12, 12, 6, 6)])
 def test_udv_combine_inv_one_plus_vs_oracle(orc, eng_mod, Lx, Ly, nl, nr):
     n = Lx * Ly
