@@ -203,6 +203,71 @@ __global__ __launch_bounds__(256) void k_build_N(const double *U, const double *
     }
 }
 
+// Db^-1 and Ds of the left and right D (Db = D where |D| > 1, else 1;
+// Ds = D / Db) and the sign of prod_{|D_i| > 1} D_i over both, one
+// workgroup per replica.
+__global__ void k_split2(const double *Dl, long long sDl, const double *Dr, long long sDr, int n, double *lbi,
+                         double *ls, double *rbi, double *rs_, int *bigsign, int batch, const int *status)
+{
+    const int b = blockIdx.x;
+    if (b >= batch || status[b]) return;
+    __shared__ int neg;
+    if (threadIdx.x == 0) neg = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double dl = Dl[b * sDl + i], dr = Dr[b * sDr + i];
+        const bool bl = fabs(dl) > 1.0, br = fabs(dr) > 1.0;
+        lbi[(long long)b * n + i] = bl ? 1.0 / dl : 1.0;
+        ls[(long long)b * n + i] = bl ? 1.0 : dl;
+        rbi[(long long)b * n + i] = br ? 1.0 / dr : 1.0;
+        rs_[(long long)b * n + i] = br ? 1.0 : dr;
+        if ((bl && dl < 0.0) != (br && dr < 0.0)) atomicXor(&neg, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) bigsign[b] = neg ? -1 : 1;
+}
+
+// M = Dlb^-1 Z^T Drb^-1 + Dls X Drs per 32 x 32 tile, Z read through an
+// LDS tile (as in k_build_N)
+__global__ __launch_bounds__(256) void k_build_M(const double *Z, const double *X, const double *lbi, const double *ls,
+                                                 const double *rbi, const double *rs_, double *M, int n, int batch,
+                                                 const int *status)
+{
+    const int b = blockIdx.y;
+    if (b >= batch || status[b]) return;
+    const long long nn = (long long)n * n;
+    const double *Zb = Z + b * nn, *Xb = X + b * nn;
+    double *Mb = M + b * nn;
+    const int nt = (n + BN_T - 1) / BN_T;
+    const int i0 = (blockIdx.x % nt) * BN_T, j0 = (blockIdx.x / nt) * BN_T;
+    const int tx = threadIdx.x % BN_T, ty = threadIdx.x / BN_T;
+    __shared__ double tl[BN_T][BN_T + 1];
+    double z[BN_T / 8];
+#pragma unroll
+    for (int k = 0; k < BN_T / 8; k++) {
+        const int i = min(i0 + ty + 8 * k, n - 1), j = min(j0 + tx, n - 1);
+        z[k] = Zb[j + (long long)i * n]; // Z(j, i) = Z^T(i, j)
+    }
+#pragma unroll
+    for (int k = 0; k < BN_T / 8; k++) tl[ty + 8 * k][tx] = z[k];
+    __syncthreads();
+    const int i = i0 + tx, ic = min(i, n - 1);
+    const double li = lbi[(long long)b * n + ic], lsi = ls[(long long)b * n + ic];
+    double x[BN_T / 8], rb[BN_T / 8], rsv[BN_T / 8];
+#pragma unroll
+    for (int k = 0; k < BN_T / 8; k++) {
+        const int jc = min(j0 + ty + 8 * k, n - 1);
+        x[k] = Xb[ic + (long long)jc * n];
+        rb[k] = rbi[(long long)b * n + jc];
+        rsv[k] = rs_[(long long)b * n + jc];
+    }
+#pragma unroll
+    for (int k = 0; k < BN_T / 8; k++) {
+        const int j = j0 + ty + 8 * k;
+        if (i < n && j < n) Mb[i + (long long)j * n] = li * tl[tx][ty + 8 * k] * rb[k] + lsi * x[k] * rsv[k];
+    }
+}
+
 // det sign of 1 + U D T = sgn(prod_big D) sgn det U sgn det M, with
 // det M = det Q_M prod (D_M)_i (src/linalg.c:1039-1083).
 __global__ void k_sign_finish(const int *bigsign, const int *uq, long long suq, const int *mq,
@@ -333,7 +398,7 @@ struct Work {
     int n = 0, nb = 0;
     hipStream_t s = nullptr;
     double *W1 = nullptr, *W2 = nullptr, *W3 = nullptr, *W4 = nullptr;
-    double *V1 = nullptr, *V2 = nullptr, *V3 = nullptr, *Tws = nullptr;
+    double *V1 = nullptr, *V2 = nullptr, *V3 = nullptr, *V4 = nullptr, *V5 = nullptr, *Tws = nullptr;
     int *I1 = nullptr, *I2 = nullptr, *status = nullptr, *reason = nullptr, *det_sign = nullptr;
     float *F1 = nullptr, *F2 = nullptr;
     std::vector<void *> owned;
@@ -344,7 +409,8 @@ struct Work {
         nb = nb_;
         const size_t nn = (size_t)n * n * nb;
         if (dalloc(&W1, nn) || dalloc(&W2, nn) || dalloc(&W3, nn) || dalloc(&W4, nn)) return 1;
-        if (dalloc(&V1, (size_t)n * nb) || dalloc(&V2, (size_t)n * nb) || dalloc(&V3, (size_t)n * nb))
+        if (dalloc(&V1, (size_t)n * nb) || dalloc(&V2, (size_t)n * nb) || dalloc(&V3, (size_t)n * nb) ||
+            dalloc(&V4, (size_t)n * nb) || dalloc(&V5, (size_t)n * nb))
             return 1;
         const int npan = (n + QR_NB - 1) / QR_NB;
         if (dalloc(&Tws, (size_t)npan * QR_NB * QR_NB * nb)) return 1;
@@ -352,7 +418,7 @@ struct Work {
             dalloc(&det_sign, nb))
             return 1;
         if (mixed && (dalloc(&F1, nn) || dalloc(&F2, nn))) return 1;
-        void *ps[] = {W1, W2, W3, W4, V1, V2, V3, Tws, I1, I2, status, reason, det_sign, F1, F2};
+        void *ps[] = {W1, W2, W3, W4, V1, V2, V3, V4, V5, Tws, I1, I2, status, reason, det_sign, F1, F2};
         for (void *p : ps)
             if (p) owned.push_back(p);
         HIPCHK(hipMemset(status, 0, sizeof(int) * nb));
@@ -451,7 +517,8 @@ static int qr(Work &w, double *A, double *Q, long long sQ, double *Rn, long long
 }
 
 // X = U^-1 B, U unit upper triangular (dtrsm, left, upper, unit diagonal)
-static int trsm(Work &w, const double *U, long long sU, const double *B, long long sB, double *X, long long sX)
+static int trsm(Work &w, const double *U, long long sU, const double *B, long long sB, double *X, long long sX,
+                int trans = 0, const double *rs = nullptr, long long srs = 0)
 {
     ProfScope ps_(DQMC_GPU_PROF_TRSM, w.s);
     TrsmArgs a;
@@ -460,6 +527,8 @@ static int trsm(Work &w, const double *U, long long sU, const double *B, long lo
     a.B = B; a.sB = sB;
     a.X = X; a.sX = sX;
     a.status = w.status;
+    a.trans = trans;
+    a.rs = rs; a.srs = srs;
     hipLaunchKernelGGL(k_trsm_unit_upper, dim3(trsm_blocks(w.n, w.nb)), dim3(256), trsm_lds_bytes(w.n), w.s, a);
     return launch_check("k_trsm_unit_upper");
 }
@@ -526,11 +595,48 @@ static int udv_inv_one_plus(Work &w, const Udv &u, double *Gout, long long sG)
     return launch_check("k_sign_finish");
 }
 
+// G = (1 + Ul Dl Tl Ur Dr Tr)^-1 at a block boundary with one QR, the
+// fused form of udv_combine + udv_inv_one_plus (src/linalg.c:796-1115):
+// 1 + L R = Ul Dlb M Drb Tr with
+//   M = Dlb^-1 (Ul^T Tr^-1) Drb^-1 + Dls (Tl Ur) Drs
+// (both terms bounded; T of every UDV here is unit upper triangular), so
+//   G = Tr^-1 Drb^-1 Rn_M^-1 D_M^-1 Q_M^T Dlb^-1 Ul^T,
+// triangular solves for Tr^-T, Rn_M^-1 and Tr^-1, one GEMM for Tl Ur and
+// one for Q_M^T Ul^T.  sgn det(1 + L R) = sgn det Ul * prod_{|D|>1} sgn D
+// (both sides) * sgn det Q_M * prod sgn (D_M)_i.
+static int udv_inv_two_sided(Work &w, const Udv &l, const Udv &r, double *Gout, long long sG)
+{
+    ProfScope ps_(DQMC_GPU_PROF_INV, w.s);
+    const int n = w.n;
+    const long long nn = (long long)n * n;
+    hipLaunchKernelGGL(k_split2, dim3(w.nb), dim3(256), 0, w.s, l.D, l.sD, r.D, r.sD, n, w.V1, w.V2, w.V4, w.V5,
+                       w.I1, w.nb, w.status);
+    if (launch_check("k_split2")) return 1;
+    if (trsm(w, r.T, r.sU, l.U, l.sU, w.W1, nn, 1)) return 1; // W1 = Tr^-T Ul = (Ul^T Tr^-1)^T
+    GemmArgs<double> g = gemm_args<double>(n, n, n, w.nb, l.T, n, l.sU, 0, r.U, n, r.sU, 0, w.W2, n, nn);
+    if (gemm(w, g)) return 1; // W2 = Tl Ur
+    const int bnt = (n + BN_T - 1) / BN_T;
+    hipLaunchKernelGGL(k_build_M, dim3(bnt * bnt, w.nb), dim3(256), 0, w.s, w.W1, w.W2, w.V1, w.V2, w.V4, w.V5, w.W4,
+                       n, w.nb, w.status);
+    if (launch_check("k_build_M")) return 1;
+    if (qr(w, w.W4, w.W1, nn, w.W2, nn, w.V3, n, w.I2, 1)) return 1; // W1 = Q_M, W2 = Rn_M, V3 = D_M
+    g = gemm_args<double>(n, n, n, w.nb, w.W1, n, nn, 1, l.U, n, l.sU, 1, w.W3, n, nn);
+    g.br = vec<double>(w.V1, n);    // Dlb^-1 on the rows of Ul^T
+    g.cr = vec<double>(w.V3, n, 1); // D_M^-1
+    if (gemm(w, g)) return 1;                                            // W3 = D_M^-1 Q_M^T Dlb^-1 Ul^T
+    if (trsm(w, w.W2, nn, w.W3, nn, w.W4, nn)) return 1;                 // W4 = Rn_M^-1 W3
+    if (trsm(w, r.T, r.sU, w.W4, nn, Gout, sG, 0, w.V4, n)) return 1;    // G = Tr^-1 Drb^-1 W4
+    hipLaunchKernelGGL(k_sign_finish, dim3(w.nb), dim3(256), 0, w.s, w.I1, l.q, l.sq, w.I2, w.V3, n,
+                       w.det_sign, w.status, w.reason, w.nb);
+    return launch_check("k_sign_finish");
+}
+
 // ------------------------------------------------------------------ model
 
 struct Model {
     int n = 0, L = 0, stab = 0, M = 0, W = 0, Wf = 0, nrep = 0, first = 0, cap = 0;
     int msplit = 1; // Metropolis launches (site segments, each with its flush) per slice
+    bool two_sided = true; // boundary rebuilds by udv_inv_two_sided (one QR) instead of combine + inv_one_plus
     int Lx = 1, Ly = 1, nq = 0, nbin = 0;
     double U = 0, dtau = 0, mu = 0, lambda = 0;
     double expv[2], expvinv[2], nu[2], nd[2];
@@ -1048,8 +1154,12 @@ static int sweep_once(dqmc_gpu_ctx *ctx, Model &m)
             const int tau = l + 1, j = tau / m.stab;
             if (stage_lmul(ctx, m, lf, tau_left, tau - tau_left)) return 1;
             tau_left = tau;
-            if (udv_combine(m.w, lf, m.stack(j), m.comb())) return 1;
-            if (udv_inv_one_plus(m.w, m.comb(), m.Gnew, (long long)m.n * m.n)) return 1;
+            if (m.two_sided) {
+                if (udv_inv_two_sided(m.w, lf, m.stack(j), m.Gnew, (long long)m.n * m.n)) return 1;
+            } else {
+                if (udv_combine(m.w, lf, m.stack(j), m.comb())) return 1;
+                if (udv_inv_one_plus(m.w, m.comb(), m.Gnew, (long long)m.n * m.n)) return 1;
+            }
             if (stage_accept_rebuild(ctx, m, 1)) return 1;
         }
     }
@@ -1218,6 +1328,7 @@ int dqmc_gpu_add_model(dqmc_gpu_ctx *ctx, int n, int L, int stab, int delay_k, c
     m->W = metro_window(n, delay_k > 0 ? delay_k : 16, sizeof(double));
     m->msplit = n >= 128 ? 2 : 1;
     if (const char *e = getenv("DQMC_GPU_METRO_SPLIT")) m->msplit = std::max(1, std::min(8, atoi(e)));
+    if (const char *e = getenv("DQMC_GPU_REBUILD")) m->two_sided = strcmp(e, "combine") != 0;
     m->Wf = metro_window(n, delay_k > 0 ? delay_k : 32, sizeof(float));
     // expv for sigma = +1 (src/green.c:63-67), N = exp(-2 lambda sigma s) - 1 (src/field.c:10-17)
     const double c = -dtau * U / 2.0;

@@ -781,8 +781,9 @@ static inline int qr_formq_blocks(int n, int batch)
     return ((nblk + FQ_WAVES - 1) / FQ_WAVES) * batch;
 }
 
-// X = U^-1 B for unit upper-triangular U (the diagonal of U is not read):
-// dtrsm('L','U','N','U').  Column blocks of X are independent, so a
+// X = U^-1 B (or X = U^-T B) for unit upper-triangular U (the diagonal of
+// U is not read): dtrsm('L','U','N','U') (or 'T'), B rows optionally scaled
+// on load.  Column blocks of X are independent, so a
 // workgroup takes 32 columns of one matrix (n/32 workgroups per matrix) and
 // walks the 32-row blocks from the bottom: S = B_I - U[I, I+1:] X[I+1:] on
 // MFMA (one 16x16 tile per wave, U streamed from global memory, X resident
@@ -797,6 +798,9 @@ struct TrsmArgs {
     double *X;
     long long sX;
     const int *status;
+    int trans;        // 0: X = U^-1 B; 1: X = U^-T B (forward substitution)
+    const double *rs; // optional row scaling of B (B(i, :) * rs(i)), may be null
+    long long srs;
 };
 
 #define TS_NB 32
@@ -828,16 +832,20 @@ __global__ __launch_bounds__(256) void k_trsm_unit_upper(TrsmArgs a)
     double *Xs = lds;
     double *Ud = Xs + (size_t)TS_NB * ldx; // U_II, Ud[i*(TS_NB+1) + k]
     // B block: thread t takes rows t, t + 256, ... of every column
+    const double *rsb = a.rs ? a.rs + (long long)b * a.srs : nullptr;
     for (int r = tid; r < n; r += blockDim.x) {
         double v[TS_NB];
+        const double sr = rsb ? rsb[r] : 1.0;
 #pragma unroll
         for (int c = 0; c < TS_NB; c++) v[c] = Bm[r + (long long)(c0 + min(c, cw - 1)) * n];
 #pragma unroll
-        for (int c = 0; c < TS_NB; c++) Xs[c * ldx + r] = c < cw ? v[c] : 0.0;
+        for (int c = 0; c < TS_NB; c++) Xs[c * ldx + r] = c < cw ? v[c] * sr : 0.0;
     }
     __syncthreads();
     const int ti = wave >> 1, tj = wave & 1;
-    for (int I = nblk - 1; I >= 0; I--) {
+    const bool tr = a.trans != 0;
+    for (int It = 0; It < nblk; It++) {
+        const int I = tr ? It : nblk - 1 - It; // blocks bottom-up, or top-down for U^T
         const int r0 = I * TS_NB, ib = min(TS_NB, n - r0);
         // U_II into LDS (loads issued before the product below)
         double ud[TS_NB * TS_NB / 256];
@@ -845,9 +853,11 @@ __global__ __launch_bounds__(256) void k_trsm_unit_upper(TrsmArgs a)
         for (int u = 0; u < TS_NB * TS_NB / 256; u++) {
             const int idx = tid + 256 * u;
             const int i = idx % TS_NB, k = idx / TS_NB;
-            ud[u] = U[(r0 + min(i, ib - 1)) + (long long)(r0 + min(k, ib - 1)) * n];
+            ud[u] = tr ? U[(r0 + min(k, ib - 1)) + (long long)(r0 + min(i, ib - 1)) * n]  // (U^T)_II(i, k)
+                       : U[(r0 + min(i, ib - 1)) + (long long)(r0 + min(k, ib - 1)) * n];
         }
-        // S = B_I - U[r0:r0+ib, r0+ib:n] X[r0+ib:n]: tile (ti, tj), rows i, columns c
+        // S = B_I - U[r0:r0+ib, r0+ib:n] X[r0+ib:n] (U^T: B_I - U[0:r0, r0:r0+ib]^T X[0:r0]):
+        // tile (ti, tj), rows i, columns c
         {
             f64x4 acc0, acc1 = {0, 0, 0, 0};
             const int i0 = r0 + ti * 16, cc = tj * 16 + lc;
@@ -858,17 +868,18 @@ __global__ __launch_bounds__(256) void k_trsm_unit_upper(TrsmArgs a)
             }
             const bool iok = i0 + lc < r0 + ib;
             const int ic = min(i0 + lc, n - 1);
-            for (int k0 = r0 + ib; k0 < n; k0 += 64) {
+            const int kbeg = tr ? 0 : r0 + ib, kend = tr ? r0 : n;
+            for (int k0 = kbeg; k0 < kend; k0 += 64) {
                 double ua[16], xb[16];
 #pragma unroll
                 for (int u = 0; u < 16; u++) {
                     const int kc = min(k0 + 4 * u + lr, n - 1);
-                    ua[u] = U[ic + (long long)kc * n];
+                    ua[u] = tr ? U[kc + (long long)ic * n] : U[ic + (long long)kc * n];
                     xb[u] = Xs[cc * ldx + kc];
                 }
 #pragma unroll
                 for (int u = 0; u < 16; u++) {
-                    const bool ok = k0 + 4 * u + lr < n;
+                    const bool ok = k0 + 4 * u + lr < kend;
                     ua[u] = (ok && iok) ? -ua[u] : 0.0;
                     xb[u] = ok ? xb[u] : 0.0;
                 }
@@ -899,11 +910,22 @@ __global__ __launch_bounds__(256) void k_trsm_unit_upper(TrsmArgs a)
             double sv[TS_NB];
 #pragma unroll
             for (int i = 0; i < TS_NB; i++) sv[i] = col[min(i, ib - 1)];
+            if (!tr) {
 #pragma unroll
-            for (int i = TS_NB - 1; i >= 1; i--) {
-                if (i < ib) {
+                for (int i = TS_NB - 1; i >= 1; i--) {
+                    if (i < ib) {
 #pragma unroll
-                    for (int k = 0; k < i; k++) sv[k] -= Ud[k * (TS_NB + 1) + i] * sv[i];
+                        for (int k = 0; k < i; k++) sv[k] -= Ud[k * (TS_NB + 1) + i] * sv[i];
+                    }
+                }
+            } else {
+                // Ud holds (U^T)_II, unit lower: forward, right-looking
+#pragma unroll
+                for (int i = 0; i < TS_NB - 1; i++) {
+                    if (i < ib) {
+#pragma unroll
+                        for (int k = i + 1; k < TS_NB; k++) sv[k] -= Ud[k * (TS_NB + 1) + i] * sv[i];
+                    }
                 }
             }
 #pragma unroll
