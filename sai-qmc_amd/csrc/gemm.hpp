@@ -15,6 +15,7 @@
 // are consecutive logical ids and xcd_remap keeps them on one XCD.
 #pragma once
 #include "common.hpp"
+#include <cstdlib>
 
 template <typename T> struct GemmArgs {
     int M, N, K, batch;
@@ -406,6 +407,155 @@ __global__ __launch_bounds__(256, 2) void k_gemm_big(GemmArgs<T> g)
     }
 }
 
+// FP64 variant for op(A) = A, op(B) = B with M, N multiples of 128, K a
+// multiple of 16 and 16-byte aligned columns: the k-tiles go from memory
+// straight into LDS (gfx950 global_load_lds_dwordx4, 16 bytes per lane,
+// written at the wave's LDS base + 16 * lane), so no registers stage the
+// next tile and no VALU work stores it; the tile of k + 16 is in flight
+// while tile k is multiplied.  A tile: k-major rows of 128 doubles (one
+// wave instruction each), padded to GD_LD; B tile: j-major, each column's 16
+// k as 8 pairs whose order is XOR-swizzled by (j & 7), so the fragment
+// reads of 16 columns spread over the banks.  Scalings: k-scales (ac, br)
+// multiply the op(A) fragments, row / column scales (ar, cr / bc, cc) are
+// applied in the epilogue.
+#define GD_LD 144
+#define GD_ABUF (GB_K * GD_LD)          // doubles of one A tile
+#define GD_BUF (GD_ABUF + GB_M * GB_K)   // one buffer: A tile + B tile
+
+static inline size_t gemm_dl_lds_bytes() { return sizeof(double) * 2 * GD_BUF; }
+// DQMC_GPU_GEMM_DL=0 keeps every FP64 GEMM on k_gemm_big
+static inline bool gemm_dl_enabled()
+{
+    static const int on = [] {
+        const char *e = getenv("DQMC_GPU_GEMM_DL");
+        return (e && e[0] == '0') ? 0 : 1;
+    }();
+    return on != 0;
+}
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+__global__ __launch_bounds__(256, 2) void k_gemm_dl(GemmArgs<double> g)
+{
+    static_assert(GB_K == 16 && GB_M == 128, "16 x 128 tiles");
+    const int tiles_m = g.M / GB_M, tiles_n = g.N / GB_M;
+    const int per = tiles_m * tiles_n;
+    const int logical = xcd_remap(blockIdx.x, gridDim.x);
+    const int b = logical / per;
+    const int t = logical % per;
+    const int m0 = (t % tiles_m) * GB_M;
+    const int n0 = (t / tiles_m) * GB_M;
+    if (b >= g.batch) return;
+    if (g.status && g.status[b]) return;
+    const int K = g.K;
+    extern __shared__ __align__(16) unsigned char gd_raw[];
+    double *lds = reinterpret_cast<double *>(gd_raw);
+    const double *A = g.A + (long long)b * g.sA;
+    const double *B = g.B + (long long)b * g.sB;
+    double *C = g.C + (long long)b * g.sC;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lc = lane & 15, lr = lane >> 4;
+    const int wr = wave >> 1, wc = wave & 1;
+    // this lane's pieces of the tile loads: A rows k = 4 wave + q, i pair 2 lane;
+    // B column groups of 8, column 8 grp + (lane >> 3), k pair (lane & 7) ^ (lane >> 3)
+    const double *a_src = A + m0 + 2 * lane;
+    const int bj = lane >> 3, bkp = (lane & 7) ^ bj;
+    auto issue = [&](int u, int k0) {
+        double *As = lds + u * GD_BUF;
+        double *Bs = As + GD_ABUF;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int k = 4 * wave + q;
+            __builtin_amdgcn_global_load_lds(a_src + (long long)(k0 + k) * g.lda, (lds_void *)(As + k * GD_LD), 16, 0, 0);
+            const int grp = 4 * wave + q;
+            const double *bsrc = B + (k0 + 2 * bkp) + (long long)(n0 + 8 * grp + bj) * g.ldb;
+            __builtin_amdgcn_global_load_lds(bsrc, (lds_void *)(Bs + grp * 8 * GB_K), 16, 0, 0);
+        }
+    };
+    // k-scales of this lane's four k rows of a tile
+    const bool ksc = g.ac.p || g.br.p;
+    auto kscale = [&](int k0, double (&sk)[4]) {
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++) {
+            const int k = k0 + 4 * kk + lr;
+            sk[kk] = g.ac.fix(g.ac.raw(b, k)) * g.br.fix(g.br.raw(b, k));
+        }
+    };
+
+    f64x4 acc[4][4];
+#pragma unroll
+    for (int it = 0; it < 4; it++)
+#pragma unroll
+        for (int jt = 0; jt < 4; jt++) acc[it][jt] = f64x4{0, 0, 0, 0};
+
+    issue(0, 0);
+    int u = 0;
+    for (int k0 = 0; k0 < K; k0 += GB_K) {
+        double sk[4] = {1.0, 1.0, 1.0, 1.0};
+        if (ksc) kscale(k0, sk);
+        __builtin_amdgcn_s_waitcnt(0); // this wave's tile loads have landed
+        __syncthreads();               // ... and everyone's; the other buffer is free
+        if (k0 + GB_K < K) issue(u ^ 1, k0 + GB_K);
+        const double *As = lds + u * GD_BUF;
+        const double *Bs = As + GD_ABUF;
+#pragma unroll
+        for (int kk = 0; kk < GB_K; kk += 4) {
+            const int k = kk + lr;
+            double fa[4], fb[4];
+#pragma unroll
+            for (int jt = 0; jt < 4; jt++) {
+                const int jl = wc * 64 + jt * 16 + lc;
+                fa[jt] = Bs[jl * GB_K + 2 * ((k >> 1) ^ (jl & 7)) + (k & 1)];
+            }
+#pragma unroll
+            for (int it = 0; it < 4; it++) fb[it] = As[k * GD_LD + wr * 64 + it * 16 + lc] * sk[kk >> 2];
+#pragma unroll
+            for (int it = 0; it < 4; it++)
+#pragma unroll
+                for (int jt = 0; jt < 4; jt++) acc[it][jt] = Mfma<double>::mma(fa[jt], fb[it], acc[it][jt]);
+        }
+        u ^= 1;
+    }
+
+    // epilogue as in k_gemm_big, with the row scales of op(A) (ar) and the
+    // column scales of op(B) (bc) folded in
+    double sr[4], sc[4][4];
+#pragma unroll
+    for (int it = 0; it < 4; it++) {
+        const int i = m0 + wr * 64 + it * 16 + lc;
+        sr[it] = g.cr.fix(g.cr.raw(b, i)) * g.ar.fix(g.ar.raw(b, i));
+    }
+#pragma unroll
+    for (int jt = 0; jt < 4; jt++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int j = n0 + wc * 64 + jt * 16 + Mfma<double>::row(lane, v);
+            sc[jt][v] = g.cc.fix(g.cc.raw(b, j)) * g.bc.fix(g.bc.raw(b, j));
+        }
+#pragma unroll
+    for (int it = 0; it < 4; it++) {
+        const int i = m0 + wr * 64 + it * 16 + lc;
+#pragma unroll
+        for (int jt = 0; jt < 4; jt++) {
+            double old[4] = {0.0, 0.0, 0.0, 0.0};
+            if (g.beta != 0.0) {
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    old[v] = C[i + (long long)(n0 + wc * 64 + jt * 16 + Mfma<double>::row(lane, v)) * g.ldc];
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int j = n0 + wc * 64 + jt * 16 + Mfma<double>::row(lane, v);
+                double r = g.alpha * acc[it][jt][v] * sr[it] * sc[jt][v];
+                if (g.dg.p && i == j) r += g.dg.at(b, i);
+                if (g.beta != 0.0) r += g.beta * old[v];
+                C[i + (long long)j * g.ldc] = r;
+            }
+        }
+    }
+}
+
 template <typename T>
 static inline GemmArgs<T> gemm_args(int M, int N, int K, int batch, const T *A, long long lda,
                                     long long sA, int transA, const T *B, long long ldb,
@@ -434,6 +584,15 @@ static inline void gemm_launch(const GemmArgs<T> &g0, hipStream_t s)
         g.vec2 = ((reinterpret_cast<uintptr_t>(g.A) & al) == 0) && ((reinterpret_cast<uintptr_t>(g.B) & al) == 0) &&
                  g.lda % 2 == 0 && g.ldb % 2 == 0 && g.sA % 2 == 0 && g.sB % 2 == 0 &&
                  (g.transA ? even_k : g.M % 2 == 0) && (g.transB ? g.N % 2 == 0 : even_k);
+    }
+    if constexpr (sizeof(T) == 8) {
+        if (g.vec2 && !g.transA && !g.transB && !g.Kb && g.M % GB_M == 0 && g.N % GB_M == 0 && g.K % GB_K == 0 &&
+            g.K > 0 && gemm_dl_enabled()) {
+            const int tb = (g.M / GB_M) * (g.N / GB_M) * g.batch;
+            if (tb <= 0) return;
+            hipLaunchKernelGGL(k_gemm_dl, dim3(tb), dim3(256), gemm_dl_lds_bytes(), s, g);
+            return;
+        }
     }
     if (g.M >= GB_M && g.N >= GB_M && g.vec2) {
         const int tb = ((g.M + GB_M - 1) / GB_M) * ((g.N + GB_M - 1) / GB_M) * g.batch;
