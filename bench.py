@@ -76,16 +76,17 @@ def traffic_from_profile(kernel):
 
 def roofline_gemm(prof, mixed, nrep_launch=NREP):
     """Roofline of the dominant kernel by total time: the batched n x n x n
-    GEMM (k_gemm_big), 2 n^3 flops per replica per launch, average launch
-    duration from HIP events around every such launch of a profiled sweep."""
+    FP64 GEMM (k_gemm_dl; the few transposed ones run on k_gemm_big), 2 n^3
+    flops per replica per launch, average launch duration from HIP events
+    around every such launch of a profiled sweep."""
     n = LX * LY
     ms, cnt = prof["gemm"]
     t = ms / cnt / 1e3
     flops = 2.0 * n ** 3 * nrep_launch
     peak = P64
-    return {"bound": "mfma", "kernel": "k_gemm_big<double,0,0>", "achieved": round(flops / t / 1e12, 3),
+    return {"bound": "mfma", "kernel": "k_gemm_dl", "achieved": round(flops / t / 1e12, 3),
             "peak": round(peak / 1e12, 3), "unit": "TFLOP/s", "frac": round(flops / t / peak, 4),
-            "traffic": None if mixed else traffic_from_profile("k_gemm_big"),
+            "traffic": None if mixed else traffic_from_profile("k_gemm_dl"),
             "avg_launch_ms": round(t * 1e3, 4), "launches_per_sweep": cnt, "flops_per_launch": flops}
 
 
