@@ -407,8 +407,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm_big(GemmArgs<T> g)
     }
 }
 
-// FP64 variant for op(A) = A, op(B) = B with M, N multiples of 128, K a
-// multiple of 16 and 16-byte aligned columns: the k-tiles go from memory
+// FP64 variant for op(A) = A, op(B) = B or B^T (TB) with M, N multiples of
+// 128 and 16-byte aligned columns; K a multiple of 16, or, with TB = 1, any
+// per-entry K (Kb: rows past it are read from row K-1 and zeroed by the
+// k-scale).  The k-tiles go from memory
 // straight into LDS (gfx950 global_load_lds_dwordx4, 16 bytes per lane,
 // written at the wave's LDS base + 16 * lane), so no registers stage the
 // next tile and no VALU work stores it; the tile of k + 16 is in flight
@@ -417,10 +419,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm_big(GemmArgs<T> g)
 // k as 8 pairs whose order is XOR-swizzled by (j & 7), so the fragment
 // reads of 16 columns spread over the banks.  Scalings: k-scales (ac, br)
 // multiply the op(A) fragments, row / column scales (ar, cr / bc, cc) are
-// applied in the epilogue.
+// applied in the epilogue.  With TB = 1 the B tile is k-major like A.
 #define GD_LD 144
-#define GD_ABUF (GB_K * GD_LD)          // doubles of one A tile
-#define GD_BUF (GD_ABUF + GB_M * GB_K)   // one buffer: A tile + B tile
+#define GD_ABUF (GB_K * GD_LD)          // doubles of one A tile (and of a k-major B tile)
+#define GD_BUF (2 * GD_ABUF)             // one buffer: A tile + B tile
 
 static inline size_t gemm_dl_lds_bytes() { return sizeof(double) * 2 * GD_BUF; }
 // DQMC_GPU_GEMM_DL=0 keeps every FP64 GEMM on k_gemm_big
@@ -435,6 +437,7 @@ static inline bool gemm_dl_enabled()
 
 typedef __attribute__((address_space(3))) void lds_void;
 
+template <int TB>
 __global__ __launch_bounds__(256, 2) void k_gemm_dl(GemmArgs<double> g)
 {
     static_assert(GB_K == 16 && GB_M == 128, "16 x 128 tiles");
@@ -447,7 +450,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_dl(GemmArgs<double> g)
     const int n0 = (t / tiles_m) * GB_M;
     if (b >= g.batch) return;
     if (g.status && g.status[b]) return;
-    const int K = g.K;
+    const int K = g.Kb ? min(g.K, g.Kb[b]) : g.K;
+    if (K <= 0 && g.beta == 1.0 && !g.dg.p) return; // C unchanged
     extern __shared__ __align__(16) unsigned char gd_raw[];
     double *lds = reinterpret_cast<double *>(gd_raw);
     const double *A = g.A + (long long)b * g.sA;
@@ -460,6 +464,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_dl(GemmArgs<double> g)
     // this lane's pieces of the tile loads: A rows k = 4 wave + q, i pair 2 lane;
     // B column groups of 8, column 8 grp + (lane >> 3), k pair (lane & 7) ^ (lane >> 3)
     const double *a_src = A + m0 + 2 * lane;
+    const double *b_src = B + n0 + 2 * lane; // TB = 1: rows of op(B) are columns of B
     const int bj = lane >> 3, bkp = (lane & 7) ^ bj;
     auto issue = [&](int u, int k0) {
         double *As = lds + u * GD_BUF;
@@ -467,19 +472,24 @@ __global__ __launch_bounds__(256, 2) void k_gemm_dl(GemmArgs<double> g)
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int k = 4 * wave + q;
-            __builtin_amdgcn_global_load_lds(a_src + (long long)(k0 + k) * g.lda, (lds_void *)(As + k * GD_LD), 16, 0, 0);
-            const int grp = 4 * wave + q;
-            const double *bsrc = B + (k0 + 2 * bkp) + (long long)(n0 + 8 * grp + bj) * g.ldb;
-            __builtin_amdgcn_global_load_lds(bsrc, (lds_void *)(Bs + grp * 8 * GB_K), 16, 0, 0);
+            const int kr = min(k0 + k, K - 1); // rows past K: a valid row, zeroed by the k-scale
+            __builtin_amdgcn_global_load_lds(a_src + (long long)kr * g.lda, (lds_void *)(As + k * GD_LD), 16, 0, 0);
+            if (TB) {
+                __builtin_amdgcn_global_load_lds(b_src + (long long)kr * g.ldb, (lds_void *)(Bs + k * GD_LD), 16, 0, 0);
+            } else {
+                const int grp = 4 * wave + q;
+                const double *bsrc = B + (k0 + 2 * bkp) + (long long)(n0 + 8 * grp + bj) * g.ldb;
+                __builtin_amdgcn_global_load_lds(bsrc, (lds_void *)(Bs + grp * 8 * GB_K), 16, 0, 0);
+            }
         }
     };
-    // k-scales of this lane's four k rows of a tile
-    const bool ksc = g.ac.p || g.br.p;
+    // k-scales of this lane's four k rows of a tile (0 past K)
+    const bool ksc = g.ac.p || g.br.p || (K % GB_K) != 0;
     auto kscale = [&](int k0, double (&sk)[4]) {
 #pragma unroll
         for (int kk = 0; kk < 4; kk++) {
-            const int k = k0 + 4 * kk + lr;
-            sk[kk] = g.ac.fix(g.ac.raw(b, k)) * g.br.fix(g.br.raw(b, k));
+            const int k = k0 + 4 * kk + lr, kc = min(k, K - 1);
+            sk[kk] = k < K ? g.ac.fix(g.ac.raw(b, kc)) * g.br.fix(g.br.raw(b, kc)) : 0.0;
         }
     };
 
@@ -489,7 +499,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_dl(GemmArgs<double> g)
 #pragma unroll
         for (int jt = 0; jt < 4; jt++) acc[it][jt] = f64x4{0, 0, 0, 0};
 
-    issue(0, 0);
+    if (K > 0) issue(0, 0);
     int u = 0;
     for (int k0 = 0; k0 < K; k0 += GB_K) {
         double sk[4] = {1.0, 1.0, 1.0, 1.0};
@@ -506,7 +516,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_dl(GemmArgs<double> g)
 #pragma unroll
             for (int jt = 0; jt < 4; jt++) {
                 const int jl = wc * 64 + jt * 16 + lc;
-                fa[jt] = Bs[jl * GB_K + 2 * ((k >> 1) ^ (jl & 7)) + (k & 1)];
+                fa[jt] = TB ? Bs[k * GD_LD + jl] : Bs[jl * GB_K + 2 * ((k >> 1) ^ (jl & 7)) + (k & 1)];
             }
 #pragma unroll
             for (int it = 0; it < 4; it++) fb[it] = As[k * GD_LD + wr * 64 + it * 16 + lc] * sk[kk >> 2];
@@ -586,11 +596,14 @@ static inline void gemm_launch(const GemmArgs<T> &g0, hipStream_t s)
                  (g.transA ? even_k : g.M % 2 == 0) && (g.transB ? g.N % 2 == 0 : even_k);
     }
     if constexpr (sizeof(T) == 8) {
-        if (g.vec2 && !g.transA && !g.transB && !g.Kb && g.M % GB_M == 0 && g.N % GB_M == 0 && g.K % GB_K == 0 &&
-            g.K > 0 && gemm_dl_enabled()) {
+        const bool aligned = (reinterpret_cast<uintptr_t>(g.A) & 15) == 0 && (reinterpret_cast<uintptr_t>(g.B) & 15) == 0 &&
+                             g.lda % 2 == 0 && g.ldb % 2 == 0 && g.sA % 2 == 0 && g.sB % 2 == 0;
+        if (aligned && !g.transA && g.M % GB_M == 0 && g.N % GB_M == 0 && g.K > 0 && gemm_dl_enabled() &&
+            ((!g.transB && !g.Kb && g.K % GB_K == 0) || g.transB)) {
             const int tb = (g.M / GB_M) * (g.N / GB_M) * g.batch;
             if (tb <= 0) return;
-            hipLaunchKernelGGL(k_gemm_dl, dim3(tb), dim3(256), gemm_dl_lds_bytes(), s, g);
+            if (g.transB) hipLaunchKernelGGL(k_gemm_dl<1>, dim3(tb), dim3(256), gemm_dl_lds_bytes(), s, g);
+            else hipLaunchKernelGGL(k_gemm_dl<0>, dim3(tb), dim3(256), gemm_dl_lds_bytes(), s, g);
             return;
         }
     }
