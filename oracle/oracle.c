@@ -545,6 +545,20 @@ void orc_wrap(int n, const double *expK, const double *expKinv,
     free(B); free(Bi); free(tmp);
 }
 
+/* green_wrap_backward (src/green.c:529-546): G <- B_l^-1 G B_l. */
+void orc_wrap_backward(int n, const double *expK, const double *expKinv,
+                       const double expv[2], const signed char *s_l, double *g)
+{
+    const size_t nn = (size_t)n * (size_t)n;
+    double *B = malloc(sizeof(double) * nn), *Bi = malloc(sizeof(double) * nn);
+    double *tmp = malloc(sizeof(double) * nn);
+    orc_build_B(n, expK, expv, s_l, B);
+    orc_build_Binv(n, expKinv, expv, s_l, Bi);
+    orc_matmul(n, Bi, g, tmp);
+    orc_matmul(n, tmp, B, g);
+    free(B); free(Bi); free(tmp);
+}
+
 /* src/green.c:372-382: G_dn = delta - P_i P_j G_up(j,i). */
 void orc_ph_down(int n, const int *bipart, const double *gu, double *gd)
 {
@@ -668,6 +682,8 @@ struct orc_replica {
     unsigned long long acc, att, sweeps, drift_samples;
     double drift_max, drift_sum;
     Udv *S, left, comb;
+    Udv *A;             /* prefix stack of the alternating mode (M + 1) */
+    int alternating, dir, prefix_ok, suffix_ok;
 };
 
 static void udv_alloc(Udv *u, int n)
@@ -838,43 +854,90 @@ int orc_replica_rebuild(orc_replica *R)
 
 int orc_replica_status(const orc_replica *R) { return R->status; }
 
-/* dqmc_sweep_forward, PH branch, carry_prefix = 0 (src/dqmc.c:609-813). */
-int orc_replica_sweep(orc_replica *R)
+/* One slice of single-site Metropolis with delayed updates, then the flush
+   (the loop bodies of src/dqmc.c:656-679 and :874-891; green_wrap and
+   green_wrap_backward flush first, src/green.c:509-511, 529-531). */
+static void metro_slice(orc_replica *R, int l)
 {
-    if (R->status) return R->status;
+    const int n = R->n;
+    for (int i = 0; i < n; i++) {
+        const signed char s = R->s[(size_t)l * n + i];
+        const double Nu = R->ncache[1][s > 0];
+        const double Nd = R->ncache[0][s > 0];
+        const double gii = delay_diag(R, i);
+        const double Ru = 1.0 + (1.0 - gii) * Nu;
+        const double Rd = 1.0 + gii * Nd;
+        const double Rt = Ru * Rd;
+        R->att++;
+        if (orc_rng_double(R->rng) < fabs(Rt)) {
+            R->acc++;
+            delay_accept(R, i, Nu);
+            R->s[(size_t)l * n + i] = (signed char)-s;
+            if (Rt < 0.0) R->sign = -R->sign;
+        }
+    }
+    delay_flush(R);
+}
+
+/* green_from_boundary_factors in combine mode (src/green.c:308-345) followed
+   by the PH map and the sign reset (src/dqmc.c:731-738, 843-846). */
+static int rebuild_from_factors(orc_replica *R, const Udv *prefix, const Udv *suffix)
+{
+    const int n = R->n;
+    int sg = 0;
+    if (orc_udv_combine(n, prefix->U, prefix->D, prefix->T, suffix->U, suffix->D, suffix->T, R->comb.U,
+                        R->comb.D, R->comb.T) ||
+        orc_udv_inv_one_plus(n, R->comb.U, R->comb.D, R->comb.T, R->g, &sg) || sg == 0) {
+        R->status = 1;
+        R->det_sign = 0;
+        return 1;
+    }
+    R->det_sign = sg;
+    orc_ph_down(n, R->bipart, R->g, R->gd);
+    R->sign = 1.0;
+    return 0;
+}
+
+/* green_from_left_udv (src/green.c:352-370) + PH map + sign reset. */
+static int rebuild_from_left(orc_replica *R, const Udv *u)
+{
+    int sg = 0;
+    if (orc_udv_inv_one_plus(R->n, u->U, u->D, u->T, R->g, &sg) || sg == 0) {
+        R->status = 1;
+        R->det_sign = 0;
+        return 1;
+    }
+    R->det_sign = sg;
+    orc_ph_down(R->n, R->bipart, R->g, R->gd);
+    R->sign = 1.0;
+    return 0;
+}
+
+/* dqmc_sweep_forward, PH branch (src/dqmc.c:609-813).  carry != 0 is the
+   alternating mode's forward leg: the suffix stack is rebuilt only when the
+   previous backward sweep did not leave it, and the left UDV is stored into
+   the prefix stack at every boundary and at tau = L. */
+static int sweep_forward(orc_replica *R, int carry)
+{
     const int n = R->n, L = R->L, stab = R->stab, M = R->M;
-    const size_t nn = (size_t)n * (size_t)n;
-    double *Bb = malloc(sizeof(double) * nn);
+    double *Bb = malloc(sizeof(double) * (size_t)n * n);
     int tau_left = 0;
 
-    /* green_stack_build_suffix (src/green.c:203-226) */
-    udv_eye(&R->S[M], n);
-    udv_eye(&R->S[0], n);
-    for (int j = M - 1; j >= 1; j--) {
-        orc_build_Bblock(n, R->expK, R->expv, R->s, R->b[j], R->b[j + 1] - R->b[j], Bb);
-        udv_cp(&R->S[j], &R->S[j + 1], n);
-        if (orc_udv_rmul(n, R->S[j].U, R->S[j].D, R->S[j].T, Bb)) { R->status = 1; goto done; }
+    if (!carry || !R->suffix_ok) {
+        /* green_stack_build_suffix (src/green.c:203-226) */
+        udv_eye(&R->S[M], n);
+        udv_eye(&R->S[0], n);
+        for (int j = M - 1; j >= 1; j--) {
+            orc_build_Bblock(n, R->expK, R->expv, R->s, R->b[j], R->b[j + 1] - R->b[j], Bb);
+            udv_cp(&R->S[j], &R->S[j + 1], n);
+            if (orc_udv_rmul(n, R->S[j].U, R->S[j].D, R->S[j].T, Bb)) { R->status = 1; goto done; }
+        }
     }
     udv_eye(&R->left, n);
+    if (carry) udv_eye(&R->A[0], n);
 
     for (int l = 0; l < L; l++) {
-        for (int i = 0; i < n; i++) {
-            const signed char s = R->s[(size_t)l * n + i];
-            const double Nu = R->ncache[1][s > 0];
-            const double Nd = R->ncache[0][s > 0];
-            const double gii = delay_diag(R, i);
-            const double Ru = 1.0 + (1.0 - gii) * Nu;
-            const double Rd = 1.0 + gii * Nd;
-            const double Rt = Ru * Rd;
-            R->att++;
-            if (orc_rng_double(R->rng) < fabs(Rt)) {
-                R->acc++;
-                delay_accept(R, i, Nu);
-                R->s[(size_t)l * n + i] = (signed char)-s;
-                if (Rt < 0.0) R->sign = -R->sign;
-            }
-        }
-        delay_flush(R);
+        metro_slice(R, l);
         orc_wrap(n, R->expK, R->expKinv, R->expv, R->s + (size_t)l * n, R->g);
         if (((l + 1) % stab) == 0 && (l + 1) < L) {
             const int tau = l + 1, j = tau / stab, len = tau - tau_left;
@@ -882,17 +945,8 @@ int orc_replica_sweep(orc_replica *R)
             orc_build_Bblock(n, R->expK, R->expv, R->s, tau_left, len, Bb);
             if (orc_udv_lmul(n, R->left.U, R->left.D, R->left.T, Bb)) { R->status = 1; goto done; }
             tau_left = tau;
-            int sg = 0;
-            if (orc_udv_combine(n, R->left.U, R->left.D, R->left.T, R->S[j].U, R->S[j].D,
-                                R->S[j].T, R->comb.U, R->comb.D, R->comb.T) ||
-                orc_udv_inv_one_plus(n, R->comb.U, R->comb.D, R->comb.T, R->g, &sg) || sg == 0) {
-                R->status = 1;
-                R->det_sign = 0;
-                goto done;
-            }
-            R->det_sign = sg;
-            orc_ph_down(n, R->bipart, R->g, R->gd);
-            R->sign = 1.0;
+            if (carry) udv_cp(&R->A[j], &R->left, n);
+            if (rebuild_from_factors(R, &R->left, &R->S[j])) goto done;
         }
     }
     {
@@ -902,20 +956,98 @@ int orc_replica_sweep(orc_replica *R)
             orc_build_Bblock(n, R->expK, R->expv, R->s, tau_left, len, Bb);
             if (orc_udv_lmul(n, R->left.U, R->left.D, R->left.T, Bb)) { R->status = 1; goto done; }
         }
-        int sg = 0;
-        if (orc_udv_inv_one_plus(n, R->left.U, R->left.D, R->left.T, R->g, &sg) || sg == 0) {
-            R->status = 1;
-            R->det_sign = 0;
-            goto done;
-        }
-        R->det_sign = sg;
-        orc_ph_down(n, R->bipart, R->g, R->gd);
-        R->sign = 1.0;
+        if (carry) udv_cp(&R->A[M], &R->left, n);
+        if (rebuild_from_left(R, &R->left)) goto done;
+    }
+    if (carry) {
+        R->prefix_ok = 1;
+        R->suffix_ok = 0;
+        R->dir = 1;
     }
     R->sweeps++;
 done:
     free(Bb);
+    if (R->status) R->prefix_ok = R->suffix_ok = 0;
     return R->status;
+}
+
+/* dqmc_sweep_backward (src/dqmc.c:850-950) with dqmc_backward_boundary
+   (:815-848): slices L-1 .. 0, G carried down by green_wrap_backward; at
+   tau = j*stab the suffix UDV (B_{L-1} ... B_tau, accumulated by rmul) is
+   stored as S[j] for the next forward sweep and G is rebuilt from the
+   prefix A[j] of the previous forward sweep and that suffix. */
+static int sweep_backward(orc_replica *R)
+{
+    const int n = R->n, L = R->L, stab = R->stab, M = R->M;
+    double *Bb = malloc(sizeof(double) * (size_t)n * n);
+
+    if (!R->prefix_ok) {
+        /* green_stack_build_prefix (src/green.c:228-250) */
+        udv_eye(&R->A[0], n);
+        for (int j = 1; j <= M; j++) {
+            orc_build_Bblock(n, R->expK, R->expv, R->s, R->b[j - 1], R->b[j] - R->b[j - 1], Bb);
+            udv_cp(&R->A[j], &R->A[j - 1], n);
+            if (orc_udv_lmul(n, R->A[j].U, R->A[j].D, R->A[j].T, Bb)) { R->status = 1; goto done; }
+        }
+    }
+    udv_eye(&R->left, n);
+    udv_eye(&R->S[M], n);
+    orc_wrap_backward(n, R->expK, R->expKinv, R->expv, R->s + (size_t)(L - 1) * n, R->g);
+
+    for (int l = L - 1; l >= 0; l--) {
+        metro_slice(R, l);
+        if (l == 0) break;
+        if ((l % stab) == 0) {
+            const int j = l / stab;
+            if (record_drift(R, l)) { R->status = 1; goto done; }
+            orc_build_Bblock(n, R->expK, R->expv, R->s, R->b[j], R->b[j + 1] - R->b[j], Bb);
+            if (orc_udv_rmul(n, R->left.U, R->left.D, R->left.T, Bb)) { R->status = 1; goto done; }
+            udv_cp(&R->S[j], &R->left, n);
+            if (rebuild_from_factors(R, &R->A[j], &R->left)) goto done;
+        }
+        orc_wrap_backward(n, R->expK, R->expKinv, R->expv, R->s + (size_t)(l - 1) * n, R->g);
+    }
+    {
+        const int len = R->b[1] - R->b[0];
+        if (record_drift(R, 0)) { R->status = 1; goto done; }
+        if (len > 0) {
+            orc_build_Bblock(n, R->expK, R->expv, R->s, 0, len, Bb);
+            if (orc_udv_rmul(n, R->left.U, R->left.D, R->left.T, Bb)) { R->status = 1; goto done; }
+        }
+        if (rebuild_from_left(R, &R->left)) goto done;
+    }
+    R->prefix_ok = 0;
+    R->suffix_ok = 1;
+    R->dir = 0;
+    R->sweeps++;
+done:
+    free(Bb);
+    if (R->status) R->prefix_ok = R->suffix_ok = 0;
+    return R->status;
+}
+
+/* dqmc_sweep (src/dqmc.c:952-963) */
+int orc_replica_sweep(orc_replica *R)
+{
+    if (R->status) return R->status;
+    if (!R->alternating) return sweep_forward(R, 0);
+    return R->dir ? sweep_backward(R) : sweep_forward(R, 1);
+}
+
+/* sweep_order=alternating (src/dqmc.c:431-462): allocate the prefix stack;
+   the next sweep is forward and rebuilds both stacks as it needs them. */
+void orc_replica_set_alternating(orc_replica *R, int on)
+{
+    if (on && !R->A) {
+        R->A = calloc((size_t)R->M + 1, sizeof(Udv));
+        for (int j = 0; j <= R->M; j++) {
+            udv_alloc(&R->A[j], R->n);
+            udv_eye(&R->A[j], R->n);
+        }
+    }
+    R->alternating = on ? 1 : 0;
+    R->dir = 0;
+    R->prefix_ok = R->suffix_ok = 0;
 }
 
 void orc_replica_green(const orc_replica *R, double *gu, double *gd)
@@ -933,6 +1065,7 @@ void orc_replica_field(const orc_replica *R, signed char *s)
 void orc_replica_set_field(orc_replica *R, const signed char *s)
 {
     memcpy(R->s, s, (size_t)R->L * (size_t)R->n);
+    R->prefix_ok = R->suffix_ok = 0;
 }
 
 void orc_replica_stats(const orc_replica *R, unsigned long long *acc,
@@ -973,6 +1106,10 @@ void orc_replica_free(orc_replica *R)
     free(R->bipart); free(R->b); free(R->s);
     for (int j = 0; j <= R->M; j++) udv_release(&R->S[j]);
     free(R->S);
+    if (R->A) {
+        for (int j = 0; j <= R->M; j++) udv_release(&R->A[j]);
+        free(R->A);
+    }
     udv_release(&R->left);
     udv_release(&R->comb);
     free(R);

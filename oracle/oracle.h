@@ -71,6 +71,9 @@ void orc_build_Bblock(int n, const double *expK, const double expv[2],
                       double *out);
 void orc_wrap(int n, const double *expK, const double *expKinv,
               const double expv[2], const signed char *s_l, double *g);
+void orc_wrap_backward(int n, const double *expK, const double *expKinv,
+                       const double expv[2], const signed char *s_l,
+                       double *g);                        /* src/green.c:529-546 */
 void orc_ph_down(int n, const int *bipart, const double *gu, double *gd);
 
 /* ---- measurement ---- */
@@ -86,13 +89,15 @@ int orc_szz_sperp(int Lx, int Ly, int nq, const double *phase_cos,
                   double *sperp);
 void orc_jackknife(const double *x, int N, double *mean, double *err);
 
-/* ---- full replica engine, forward PH sweep (src/dqmc.c:609-813) ---- */
+/* ---- full replica engine, PH sweeps: forward (src/dqmc.c:609-813) and
+        alternating forward/backward (src/dqmc.c:815-963) ---- */
 typedef struct orc_replica orc_replica;
 orc_replica *orc_replica_new(int n, int L, int stab, const double *thop,
                              const int *bipart, double U, double dtau,
                              uint64_t seed, int delay_cap, int drift);
 int orc_replica_status(const orc_replica *R);
 int orc_replica_sweep(orc_replica *R);
+void orc_replica_set_alternating(orc_replica *R, int on);
 void orc_replica_green(const orc_replica *R, double *gu, double *gd);
 void orc_replica_field(const orc_replica *R, signed char *s);
 void orc_replica_set_field(orc_replica *R, const signed char *s);

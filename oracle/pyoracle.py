@@ -52,6 +52,7 @@ def orc():
             "orc_build_Binv": (None, [_I, _P, _P, _P, _P]),
             "orc_build_Bblock": (None, [_I, _P, _P, _P, _I, _I, _P]),
             "orc_wrap": (None, [_I, _P, _P, _P, _P, _P]),
+            "orc_wrap_backward": (None, [_I, _P, _P, _P, _P, _P]),
             "orc_ph_down": (None, [_I, _P, _P, _P]),
             "orc_measure_sample": (None, [_I, _P, _D, _P, _P, _P]),
             "orc_phase_cos": (None, [_I, _I, _I, _P, _P, _P]),
@@ -60,6 +61,7 @@ def orc():
             "orc_replica_new": (_P, [_I, _I, _I, _P, _P, _D, _D, _U64, _I, _I]),
             "orc_replica_status": (_I, [_P]),
             "orc_replica_sweep": (_I, [_P]),
+            "orc_replica_set_alternating": (None, [_P, _I]),
             "orc_replica_green": (None, [_P, _P, _P]),
             "orc_replica_field": (None, [_P, _P]),
             "orc_replica_set_field": (None, [_P, _P]),
@@ -96,9 +98,11 @@ def model(thop, U, dtau):
 
 
 class Replica:
-    """The oracle's forward PH sweep engine for one replica."""
+    """The oracle's PH sweep engine for one replica (forward, or alternating
+    forward/backward sweeps with carried stacks)."""
 
-    def __init__(self, kind, Lx, Ly, U, dtau, L, stab, seed, delay_cap=16, drift=False, t=-1.0):
+    def __init__(self, kind, Lx, Ly, U, dtau, L, stab, seed, delay_cap=16, drift=False, t=-1.0,
+                 alternating=False):
         th, bp, _ = lattice(kind, Lx, Ly, t)
         self.n = th.shape[0]
         self.L = L
@@ -106,6 +110,8 @@ class Replica:
         self.U = U
         self.h = C.c_void_p(orc().orc_replica_new(self.n, L, stab, _ptr(np.asfortranarray(th)), _ptr(bp), U, dtau,
                                                   seed, delay_cap, 1 if drift else 0))
+        if alternating:
+            orc().orc_replica_set_alternating(self.h, 1)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -198,6 +204,7 @@ def ref():
         L = C.CDLL(REF_SO)
         sig = {
             "ref_replica_new": (_P, [_I, _I, _I, _I, _D, _D, _D, _I, _I, C.c_ulonglong, _I]),
+            "ref_replica_new_mode": (_P, [_I, _I, _I, _I, _D, _D, _D, _I, _I, C.c_ulonglong, _I, _I, _I]),
             "ref_replica_init_rc": (_I, [_P]),
             "ref_replica_sweep": (_I, [_P]),
             "ref_replica_green": (None, [_P, _P, _P]),
@@ -228,11 +235,13 @@ def ref():
 class RefReplica:
     """The compiled reference engine for one replica (oracle/ref_trace.c)."""
 
-    def __init__(self, kind, Lx, Ly, U, dtau, L, stab, seed, drift=False, t=-1.0):
+    def __init__(self, kind, Lx, Ly, U, dtau, L, stab, seed, drift=False, t=-1.0, alternating=False,
+                 two_sided=False):
         self.n = Lx * (Ly if kind == "square" else 1)
         self.L = L
-        self.h = C.c_void_p(ref().ref_replica_new(0 if kind == "chain" else 1, Lx, Ly, 1, t, U, dtau, L, stab, seed,
-                                                  1 if drift else 0))
+        self.h = C.c_void_p(ref().ref_replica_new_mode(0 if kind == "chain" else 1, Lx, Ly, 1, t, U, dtau, L, stab,
+                                                       seed, 1 if drift else 0, 1 if alternating else 0,
+                                                       1 if two_sided else 0))
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -34,10 +34,13 @@ typedef struct {
     int init_rc;
 } RefReplica;
 
-/* lattice_type: 0 = chain (src/lattice.c:26-44), 1 = square (:46-82). */
-void *ref_replica_new(int lattice_type, int Lx, int Ly, int pbc, double thop,
-                      double U, double dtau, int Ltr, int stab,
-                      unsigned long long seed, int drift)
+/* lattice_type: 0 = chain (src/lattice.c:26-44), 1 = square (:46-82).
+   alternating selects DQMC_SWEEP_ALTERNATING (src/dqmc.h:45-48),
+   two_sided GREEN_REBUILD_TWO_SIDED (src/green.h:10-14). */
+void *ref_replica_new_mode(int lattice_type, int Lx, int Ly, int pbc,
+                           double thop, double U, double dtau, int Ltr,
+                           int stab, unsigned long long seed, int drift,
+                           int alternating, int two_sided)
 {
     RefReplica *R = calloc(1, sizeof(*R));
     if (R == NULL) return NULL;
@@ -50,12 +53,22 @@ void *ref_replica_new(int lattice_type, int Lx, int Ly, int pbc, double thop,
     model_init(&R->m, &R->lat, U, dtau, 1, 0.0);
     rng_seed(&R->r, seed);
     field_init(&R->f, R->lat.n, Ltr, U, dtau, &R->r);
-    R->init_rc = dqmc_init_modes(&R->D, &R->m, &R->f, &R->r, stab, NULL,
-                                 DQMC_SWEEP_FORWARD, GREEN_REBUILD_COMBINE);
+    R->init_rc = dqmc_init_modes(
+        &R->D, &R->m, &R->f, &R->r, stab, NULL,
+        alternating ? DQMC_SWEEP_ALTERNATING : DQMC_SWEEP_FORWARD,
+        two_sided ? GREEN_REBUILD_TWO_SIDED : GREEN_REBUILD_COMBINE);
     if (drift) {
         dqmc_enable_stab_drift(&R->D, 1);
     }
     return R;
+}
+
+void *ref_replica_new(int lattice_type, int Lx, int Ly, int pbc, double thop,
+                      double U, double dtau, int Ltr, int stab,
+                      unsigned long long seed, int drift)
+{
+    return ref_replica_new_mode(lattice_type, Lx, Ly, pbc, thop, U, dtau, Ltr,
+                                stab, seed, drift, 0, 0);
 }
 
 int ref_replica_init_rc(void *p) { return ((RefReplica *)p)->init_rc; }

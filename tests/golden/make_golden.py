@@ -11,6 +11,10 @@ Fixtures (all numpy .npz, no pickles):
   traj_sq8.npz      8x8, U=4, dtau=0.1, Ltr=40, stab=10, seeds 1 and 2:
                     3 sweeps of field and accept counts, final G_up and
                     per-sweep trace / Frobenius norm of G_up.
+  traj_alt.npz      sweep_order=alternating (forward, backward, forward, ...):
+                    chain L=6 Ltr=20 stab=7 seed=5 and 4x4 Ltr=24 stab=5
+                    seed=3 (both with a shorter last block), U=4, dtau=0.1:
+                    6 sweeps of field, accept counts, sign and G_up.
   model.npz         expK, expKinv, lambda of the reference's model_init for
                     the 4-site chain and the 4x4 square lattice.
   udv_ops_n16.npz   seeded random UDV inputs and the reference's udv_lmul,
@@ -79,6 +83,28 @@ def traj_sq8():
         out[f"fro_s{seed}"] = np.array(fro)
         out[f"green_s{seed}"] = r.green()[0]
     np.savez_compressed(os.path.join(HERE, "traj_sq8.npz"), **out)
+
+
+ALT_CASES = (("chain", 6, 1, 20, 7, 5), ("square", 4, 4, 24, 5, 3))
+
+
+def traj_alt():
+    out = {}
+    for c, (kind, Lx, Ly, L, stab, seed) in enumerate(ALT_CASES):
+        r = pyoracle.RefReplica(kind, Lx, Ly, 4.0, 0.1, L, stab, seed, alternating=True)
+        fs, acc, sg, gs = [], [], [], []
+        for _ in range(6):
+            assert r.sweep() == 0
+            fs.append(r.field())
+            st = r.stats()
+            acc.append(st[0])
+            sg.append(st[2])
+            gs.append(r.green()[0])
+        out[f"fields_{c}"] = np.array(fs)
+        out[f"accepted_{c}"] = np.array(acc, dtype=np.uint64)
+        out[f"sign_{c}"] = np.array(sg)
+        out[f"green_{c}"] = np.array(gs)
+    np.savez_compressed(os.path.join(HERE, "traj_alt.npz"), **out)
 
 
 def model():
@@ -209,13 +235,10 @@ def c2_outputs():
 if __name__ == "__main__":
     if not pyoracle.ref_available():
         sys.exit("oracle/_ref not built: make -C oracle ref")
-    traj_chain4()
-    traj_sq8()
-    model()
-    udv_ops()
-    seeds()
-    ref_fixtures()
-    ed_l4()
-    c1_outputs()
-    c2_outputs()
+    gens = [traj_chain4, traj_sq8, traj_alt, model, udv_ops, seeds, ref_fixtures, ed_l4, c1_outputs, c2_outputs]
+    # optional names on the command line select generators (default: all)
+    pick = set(sys.argv[1:])
+    for g in gens:
+        if not pick or g.__name__ in pick:
+            g()
     print("golden fixtures written to", HERE)

@@ -194,6 +194,64 @@ def test_trajectory_8x8_matches_reference(orc):
         assert np.abs(r.green()[0] - ref).max() / np.abs(ref).max() < 1e-10
 
 
+def test_alternating_trajectory_matches_reference(orc):
+    # sweep_order=alternating (src/dqmc.c:815-963): forward sweeps that
+    # store the prefix stack, backward sweeps that store the suffix stack,
+    # against the compiled reference's own trajectory
+    g = gold("traj_alt.npz")
+    cases = (("chain", 6, 1, 20, 7, 5), ("square", 4, 4, 24, 5, 3))
+    for c, (kind, Lx, Ly, L, stab, seed) in enumerate(cases):
+        r = orc.Replica(kind, Lx, Ly, 4.0, 0.1, L, stab, seed, alternating=True)
+        for s in range(6):
+            assert r.sweep() == 0
+            assert np.array_equal(r.field(), g[f"fields_{c}"][s]), (c, s)
+            acc, _, sign, _ = r.stats()
+            assert acc == int(g[f"accepted_{c}"][s]) and sign == g[f"sign_{c}"][s]
+            ref = g[f"green_{c}"][s]
+            assert np.abs(r.green()[0] - ref).max() / np.abs(ref).max() < 1e-11
+
+
+def test_backward_wrap_inverts_forward_wrap(orc):
+    # green_wrap_backward at slice l undoes green_wrap at slice l:
+    # B_l^-1 (B_l G B_l^-1) B_l = G, and it is not the forward wrap again
+    rep = orc.Replica("square", 4, 4, 4.0, 0.1, 12, 4, 9)
+    eK, eKi, lam = rep.model()
+    n = 16
+    ev = np.zeros(2)
+    orc.orc().orc_expv(lam, 4.0, 0.1, 1.0, orc._ptr(ev))
+    s = rep.field()[3 * n:4 * n].copy()
+    g0 = np.asfortranarray(np.random.default_rng(4).standard_normal((n, n)))
+    g = g0.copy(order="F")
+    args = (n, orc._ptr(np.asfortranarray(eK)), orc._ptr(np.asfortranarray(eKi)), orc._ptr(ev), orc._ptr(s))
+    orc.orc().orc_wrap(*args, orc._ptr(g))
+    fwd = g.copy(order="F")
+    orc.orc().orc_wrap_backward(*args, orc._ptr(g))
+    assert np.abs(g - g0).max() < 1e-12
+    orc.orc().orc_wrap_backward(*args, orc._ptr(g))
+    B = eK * ev[(s > 0).astype(int)][None, :]
+    assert np.abs(g - np.linalg.solve(B, g0) @ B).max() < 1e-11
+    assert np.abs(g - fwd).max() > 1e-3
+
+
+def test_alternating_green_is_the_rebuilt_green(orc):
+    # after every sweep, either direction, G = (1 + B_{L-1} ... B_0)^-1 of
+    # the current field (the end-of-sweep rebuild, src/green.c:352-370)
+    r = orc.Replica("square", 4, 4, 4.0, 0.1, 20, 6, 21, alternating=True)
+    q = orc.Replica("square", 4, 4, 4.0, 0.1, 20, 6, 21)
+    for s in range(4):
+        assert r.sweep() == 0
+        q.set_field(r.field())
+        assert q.rebuild() == 0
+        a, b = r.green()[0], q.green()[0]
+        assert np.abs(a - b).max() < 1e-12
+    # the backward legs visit the slices in the other order: the trajectory
+    # differs from the forward sweep's
+    f = orc.Replica("square", 4, 4, 4.0, 0.1, 20, 6, 21)
+    for _ in range(2):
+        f.sweep()
+    assert not np.array_equal(f.field(), r.field())
+
+
 def test_u0_green_closed_form(orc):
     # U = 0: B_l = expK for every slice, G = (1 + exp(-beta K))^-1
     # (tests/test_green_init.c:29-46)
@@ -248,6 +306,19 @@ def test_oracle_vs_live_reference_4x4(orc):
         a = orc.Replica("square", 4, 4, 6.0, 0.125, 32, 6, seed)
         b = orc.RefReplica("square", 4, 4, 6.0, 0.125, 32, 6, seed)
         for _ in range(4):
+            a.sweep()
+            b.sweep()
+            assert np.array_equal(a.field(), b.field())
+            ga, gb = a.green()[0], b.green()[0]
+            assert np.abs(ga - gb).max() / np.abs(gb).max() < 1e-11
+
+
+@pytest.mark.skipif(not os.path.exists("/root/reference/src"), reason="compiled reference only in the build container")
+def test_oracle_vs_live_reference_alternating(orc):
+    for kind, Lx, Ly, L, stab, seed in (("square", 4, 4, 32, 6, 13), ("chain", 8, 1, 40, 8, 14)):
+        a = orc.Replica(kind, Lx, Ly, 6.0, 0.125, L, stab, seed, alternating=True)
+        b = orc.RefReplica(kind, Lx, Ly, 6.0, 0.125, L, stab, seed, alternating=True)
+        for _ in range(5):
             a.sweep()
             b.sweep()
             assert np.array_equal(a.field(), b.field())
