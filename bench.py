@@ -33,14 +33,16 @@ LX, LY, U, DTAU, BETA, STAB, DELAY, NREP = 16, 16, 4.0, 0.1, 8.0, 8, 32, 128
 P32, P64 = 157.3e12, 78.6e12  # MI355X dense FP32 / FP64 matrix peaks
 
 
-def run_engine(dq, precision, steps, warmup, drift, groups=1, delay=DELAY):
+def run_engine(dq, precision, steps, warmup, drift, groups=1, delay=DELAY, order="forward"):
     """The 128 replicas as `groups` equal models: each model has its own
-    stream and sweep graph, so the groups' sweeps run concurrently."""
+    stream and sweep graph, so the groups' sweeps run concurrently.
+    order = "alternating" runs forward and backward sweeps in turn."""
     eng = dq.Engine(precision, drift=drift)
     L = int(round(BETA / DTAU))
     per = NREP // groups
     for g in range(groups):
         mid = eng.add_model("square", LX, LY, U, DTAU, L, STAB, delay_k=delay)
+        eng.set_sweep_order(mid, order)
         eng.add_replicas(mid, list(range(1 + g * per, 1 + (g + 1) * per)))
         eng.set_measure(mid, nbin=1, q="all")
     eng.init()
@@ -153,6 +155,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--groups", type=int, default=1, help="replica groups swept concurrently")
     ap.add_argument("--delay", type=int, default=DELAY, help="delayed-update window (sites)")
+    ap.add_argument("--no-alternating", action="store_true",
+                    help="skip the FP64 sweep_order=alternating run reported beside the main line")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: replicas are independent Markov chains and this engine runs on a "
@@ -169,6 +173,13 @@ def main():
         runs[name] = run_engine(dq, modes[name], a.steps, a.warmup, drift=True, groups=a.groups, delay=a.delay)
         r = runs[name]
         print(f"# {name}: {r['value']:.1f} replica-sweeps/s, {r['ms'] / a.steps:.2f} ms/sweep, "
+              f"drift max {r['drift_max']:.3e} mean {r['drift_mean']:.3e}", file=sys.stderr, flush=True)
+    if not a.no_alternating and "fp64" in names:
+        r = run_engine(dq, dq.FP64, a.steps, a.warmup, drift=True, groups=a.groups, delay=a.delay,
+                       order="alternating")
+        runs["fp64_alternating"] = r
+        names.append("fp64_alternating")
+        print(f"# fp64_alternating: {r['value']:.1f} replica-sweeps/s, {r['ms'] / a.steps:.2f} ms/sweep, "
               f"drift max {r['drift_max']:.3e} mean {r['drift_mean']:.3e}", file=sys.stderr, flush=True)
     main_name = names[0]
     main_run = runs[main_name]

@@ -91,11 +91,23 @@ int dqmc_gpu_add_replicas(dqmc_gpu_ctx *ctx, int model_id, int nrep,
 int dqmc_gpu_set_measure(dqmc_gpu_ctx *ctx, int model_id, int nq,
                          const double *phase_cos, int nbin);
 
+/* Sweep order of a model (the sweep_order key, src/io.c:284-286, 374-380;
+   dqmc_init_modes' mode, src/dqmc.c:419-462): DQMC_GPU_SWEEP_FORWARD (the
+   default) or DQMC_GPU_SWEEP_ALTERNATING, forward and backward sweeps in
+   turn, each reusing the stack the previous one stored
+   (dqmc_sweep_forward with carry_prefix / dqmc_sweep_backward,
+   src/dqmc.c:609-963).  Setting it restarts the alternation with a forward
+   sweep that rebuilds both stacks. */
+#define DQMC_GPU_SWEEP_FORWARD 0
+#define DQMC_GPU_SWEEP_ALTERNATING 1
+int dqmc_gpu_set_sweep_order(dqmc_gpu_ctx *ctx, int model_id, int order);
+
 /* Initial stabilised Green's function at tau = 0 for every replica
    (dqmc_init_modes -> green_from_scratch(Gu, 0), src/dqmc.c:418-483). */
 int dqmc_gpu_init(dqmc_gpu_ctx *ctx);
 
-/* nsweep forward sweeps of every replica (dqmc_sweep, src/dqmc.c:952-963).
+/* nsweep sweeps of every replica in each model's sweep order (dqmc_sweep,
+   src/dqmc.c:952-963).
    If bin >= 0, each sweep is followed by the tau = 0 measurement
    (measure_sample + measure_szz_sperp_sample) added with the replica's sign
    into bin `bin`, plus the sweep's acceptance deltas

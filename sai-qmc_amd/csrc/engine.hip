@@ -675,10 +675,19 @@ struct Model {
     unsigned long long *dcnt = nullptr;
     double *bins = nullptr, *bszz = nullptr, *bsperp = nullptr;
     int *counts = nullptr;
+    // sweep_order=alternating (src/dqmc.c:815-963): direction of the next
+    // sweep, which carried stack is current, and the prefix stack
+    // A[j] = B_{js-1} ... B_0 (UDV, j = 0..M) of the last forward sweep
+    bool alternating = false;
+    int dir = 0; // 0 forward, 1 backward
+    bool prefix_ok = false, suffix_ok = false;
+    double *AU = nullptr, *AD = nullptr, *AT = nullptr;
+    int *Aq = nullptr;
     Work w;
     bool allocated = false;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
+    // one captured graph per sweep variant (SW_*)
+    hipGraph_t graph[5] = {};
+    hipGraphExec_t gexec[5] = {};
     std::vector<void *> owned;
 
     Udv left() const
@@ -693,6 +702,12 @@ struct Model {
     {
         const long long nn = (long long)n * n;
         return Udv{SU + j * nn, SD + (long long)j * n, ST + j * nn, Sq + j, (M + 1) * nn,
+                   (long long)(M + 1) * n, (long long)(M + 1)};
+    }
+    Udv alt(int j) const
+    {
+        const long long nn = (long long)n * n;
+        return Udv{AU + j * nn, AD + (long long)j * n, AT + j * nn, Aq + j, (M + 1) * nn,
                    (long long)(M + 1) * n, (long long)(M + 1)};
     }
 };
@@ -737,6 +752,23 @@ static int metro_window(int n, int want, size_t elem)
     return 8;
 }
 
+// prefix stack of the alternating mode; A[0] is the identity and never
+// changes (green_stack_build_prefix and the forward leg store I there,
+// src/green.c:231, src/dqmc.c:652-654)
+static int alt_alloc(Model &m)
+{
+    if (m.AU) return 0;
+    const size_t nn = (size_t)m.n * m.n, S = (size_t)(m.M + 1), R = (size_t)m.nrep;
+    if (dalloc(&m.AU, nn * S * R) || dalloc(&m.AD, (size_t)m.n * S * R) || dalloc(&m.AT, nn * S * R) ||
+        dalloc(&m.Aq, S * R))
+        return 1;
+    m.owned.push_back(m.AU); m.owned.push_back(m.AD); m.owned.push_back(m.AT); m.owned.push_back(m.Aq);
+    const Udv a0 = m.alt(0);
+    hipLaunchKernelGGL(k_udv_identity, grid2((long long)nn, m.nrep), dim3(256), 0, m.w.s, a0.U, a0.sU, a0.D, a0.sD,
+                       a0.T, a0.sU, a0.q, a0.sq, m.n, m.nrep);
+    return launch_check("k_udv_identity");
+}
+
 static int model_alloc(dqmc_gpu_ctx *ctx, Model &m)
 {
     const int n = m.n, R = m.nrep, L = m.L;
@@ -766,6 +798,7 @@ static int model_alloc(dqmc_gpu_ctx *ctx, Model &m)
         dalloc(&m.Sq, S * R))
         return 1;
     own(m.SU); own(m.SD); own(m.ST); own(m.Sq);
+    if (m.alternating && alt_alloc(m)) return 1;
     if (dalloc(&m.LU_, nn * R) || dalloc(&m.LD, (size_t)n * R) || dalloc(&m.LT, nn * R) ||
         dalloc(&m.Lq, R) || dalloc(&m.CU, nn * R) || dalloc(&m.CD, (size_t)n * R) ||
         dalloc(&m.CT, nn * R) || dalloc(&m.Cq, R))
@@ -1008,8 +1041,9 @@ static int bblock_refresh(Model &m)
     return 0;
 }
 
-// left <- B_{lb+len-1} ... B_{lb} left   (udv_lmul_work, src/linalg.c:515-642)
-static int stage_lmul(dqmc_gpu_ctx *ctx, Model &m, const Udv &lf, int lb, int len)
+// out <- B_{lb+len-1} ... B_{lb} in   (udv_lmul_work, src/linalg.c:515-642);
+// `out` may be `in` (in place) or another UDV (the prefix stack's next entry)
+static int stage_lmul(dqmc_gpu_ctx *ctx, Model &m, const Udv &in, const Udv &out, int lb, int len)
 {
     ProfScope ps_(DQMC_GPU_PROF_LMUL, m.w.s);
     const int n = m.n, R = m.nrep;
@@ -1018,8 +1052,8 @@ static int stage_lmul(dqmc_gpu_ctx *ctx, Model &m, const Udv &lf, int lb, int le
     if (chain_f32(ctx)) {
         float *Bf = bblock_f32(m, lb, len);
         hipLaunchKernelGGL(k_f2d, grid2(nn, R), dim3(256), 0, m.w.s, Bf, nn, m.w.W3, nn, nn, R);
-        GemmArgs<double> g = gemm_args<double>(n, n, n, R, m.w.W3, n, nn, 0, lf.U, n, lf.sU, 0, m.w.W1, n, nn);
-        g.bc = vec(lf.D, lf.sD);
+        GemmArgs<double> g = gemm_args<double>(n, n, n, R, m.w.W3, n, nn, 0, in.U, n, in.sU, 0, m.w.W1, n, nn);
+        g.bc = vec(in.D, in.sD);
         if (gemm(m.w, g)) return 1;
         cur = m.w.W1;
     } else if (m.BB && lb % m.stab == 0 && len == std::min(m.stab, m.L - lb)) {
@@ -1028,15 +1062,15 @@ static int stage_lmul(dqmc_gpu_ctx *ctx, Model &m, const Udv &lf, int lb, int le
         const int j = lb / m.stab;
         double *Bj = m.BB + (long long)j * R * nn;
         if (bblock_build(m, lb, len, Bj)) return 1;
-        GemmArgs<double> g = gemm_args<double>(n, n, n, R, Bj, n, nn, 0, lf.U, n, lf.sU, 0, m.w.W1, n, nn);
-        g.bc = vec(lf.D, lf.sD);
+        GemmArgs<double> g = gemm_args<double>(n, n, n, R, Bj, n, nn, 0, in.U, n, in.sU, 0, m.w.W1, n, nn);
+        g.bc = vec(in.D, in.sD);
         if (gemm(m.w, g)) return 1;
         cur = m.w.W1;
     } else {
         // (B U) diag(D) with B applied one slice at a time
-        GemmArgs<double> g = gemm_args<double>(n, n, n, R, m.expK, n, 0, 0, lf.U, n, lf.sU, 0, m.w.W1, n, nn);
+        GemmArgs<double> g = gemm_args<double>(n, n, n, R, m.expK, n, 0, 0, in.U, n, in.sU, 0, m.w.W1, n, nn);
         g.br = vec<double>(m.dvec + (long long)lb * n, sV);
-        g.bc = vec(lf.D, lf.sD);
+        g.bc = vec(in.D, in.sD);
         if (gemm(m.w, g)) return 1;
         cur = m.w.W1;
         double *oth = m.w.W2;
@@ -1048,10 +1082,13 @@ static int stage_lmul(dqmc_gpu_ctx *ctx, Model &m, const Udv &lf, int lb, int le
         }
     }
     // QR: U' = Q, D' = diag R, T' = (D'^-1 R) T
-    if (qr(m.w, cur, lf.U, lf.sU, m.w.W3, nn, lf.D, lf.sD, lf.q, lf.sq)) return 1;
-    GemmArgs<double> g = gemm_args<double>(n, n, n, R, m.w.W3, n, nn, 0, lf.T, n, lf.sU, 0, m.w.W4, n, nn);
+    if (qr(m.w, cur, out.U, out.sU, m.w.W3, nn, out.D, out.sD, out.q, out.sq)) return 1;
+    const bool inplace = in.T == out.T;
+    GemmArgs<double> g =
+        gemm_args<double>(n, n, n, R, m.w.W3, n, nn, 0, in.T, n, in.sU, 0, inplace ? m.w.W4 : out.T, n,
+                          inplace ? nn : out.sU);
     if (gemm(m.w, g)) return 1;
-    return copy_nn(m.w, m.w.W4, nn, lf.T, lf.sU, nn);
+    return inplace ? copy_nn(m.w, m.w.W4, nn, out.T, out.sU, nn) : 0;
 }
 
 // S[j] = S[j+1] B_{b_{j+1}-1} ... B_{b_j}   (green_stack_build_suffix +
@@ -1134,48 +1171,176 @@ static int stage_from_scratch(dqmc_gpu_ctx *ctx, Model &m, int l0)
     int segs[2][2] = {{l0, m.L}, {0, l0}};
     for (auto &sg : segs)
         for (int lb = sg[0]; lb < sg[1]; lb += m.stab)
-            if (stage_lmul(ctx, m, lf, lb, std::min(m.stab, sg[1] - lb))) return 1;
+            if (stage_lmul(ctx, m, lf, lf, lb, std::min(m.stab, sg[1] - lb))) return 1;
     if (udv_inv_one_plus(m.w, lf, m.Gnew, (long long)m.n * m.n)) return 1;
     return stage_accept_rebuild(ctx, m, 0);
 }
 
-// One forward sweep (dqmc_sweep_forward, PH branch, src/dqmc.c:609-813)
-static int sweep_once(dqmc_gpu_ctx *ctx, Model &m)
+// G <- B_l^-1 G B_l = diag(1/d_l) expKinv G expK diag(d_l)
+// (green_wrap_backward, src/green.c:529-546)
+static int stage_wrap_back(dqmc_gpu_ctx *ctx, Model &m, int l)
 {
-    for (int j = m.M - 1; j >= 1; j--)
-        if (stage_rmul(ctx, m, j)) return 1;
+    ProfScope ps_(DQMC_GPU_PROF_WRAP, m.w.s);
+    const int n = m.n, R = m.nrep;
+    const long long nn = (long long)n * n, sV = (long long)m.L * n;
+    if (is_mixed(ctx)) {
+        GemmArgs<float> g = gemm_args<float>(n, n, n, R, m.expKinvf, n, 0, 0, m.Gf, n, nn, 0, m.w.F1, n, nn);
+        g.ar = vec<float>(m.dinvf + (long long)l * n, sV);
+        if (gemm(m.w, g)) return 1;
+        g = gemm_args<float>(n, n, n, R, m.w.F1, n, nn, 0, m.expKf, n, 0, 0, m.Gf, n, nn);
+        g.bc = vec<float>(m.dvecf + (long long)l * n, sV);
+        return gemm(m.w, g);
+    }
+    GemmArgs<double> g = gemm_args<double>(n, n, n, R, m.expKinv, n, 0, 0, m.G, n, nn, 0, m.w.W1, n, nn);
+    g.ar = vec<double>(m.dinv + (long long)l * n, sV);
+    if (gemm(m.w, g)) return 1;
+    g = gemm_args<double>(n, n, n, R, m.w.W1, n, nn, 0, m.expK, n, 0, 0, m.G, n, nn);
+    g.bc = vec<double>(m.dvec + (long long)l * n, sV);
+    return gemm(m.w, g);
+}
+
+// G at a block boundary from a prefix and a suffix UDV
+// (green_from_boundary_factors, src/green.c:308-345), then drift + copy
+static int stage_boundary_rebuild(dqmc_gpu_ctx *ctx, Model &m, const Udv &prefix, const Udv &suffix)
+{
+    const long long nn = (long long)m.n * m.n;
+    if (m.two_sided) {
+        if (udv_inv_two_sided(m.w, prefix, suffix, m.Gnew, nn)) return 1;
+    } else {
+        if (udv_combine(m.w, prefix, suffix, m.comb())) return 1;
+        if (udv_inv_one_plus(m.w, m.comb(), m.Gnew, nn)) return 1;
+    }
+    return stage_accept_rebuild(ctx, m, 1);
+}
+
+// Sweep variants: the plain forward sweep, and the two legs of the
+// alternating mode, each with or without rebuilding the stack it reads.
+enum { SW_FORWARD = 0, SW_FWD_BUILD, SW_FWD_CARRIED, SW_BWD_BUILD, SW_BWD_CARRIED, SW_N };
+
+static int sweep_variant(const Model &m)
+{
+    if (!m.alternating) return SW_FORWARD;
+    if (m.dir == 0) return m.suffix_ok ? SW_FWD_CARRIED : SW_FWD_BUILD;
+    return m.prefix_ok ? SW_BWD_CARRIED : SW_BWD_BUILD;
+}
+
+// direction and carried stacks after a sweep (src/dqmc.c:806-810, 945-948)
+static void sweep_advance(Model &m, int v)
+{
+    if (v == SW_FORWARD) return;
+    const bool fwd = v == SW_FWD_BUILD || v == SW_FWD_CARRIED;
+    m.prefix_ok = fwd;
+    m.suffix_ok = !fwd;
+    m.dir = fwd ? 1 : 0;
+}
+
+// One forward sweep (dqmc_sweep_forward, PH branch, src/dqmc.c:609-813).
+// carry: the alternating mode's forward leg, whose left chain runs through
+// the prefix stack, A[j] = Bblock_{j-1} A[j-1] (stored at every boundary,
+// src/dqmc.c:704-706), so that the next backward sweep reads it.
+static int sweep_forward(dqmc_gpu_ctx *ctx, Model &m, bool build_suffix, bool carry)
+{
+    const long long nn = (long long)m.n * m.n;
+    if (build_suffix)
+        for (int j = m.M - 1; j >= 1; j--)
+            if (stage_rmul(ctx, m, j)) return 1;
     const Udv lf = m.left();
-    if (udv_identity(m, lf)) return 1;
+    if (!carry && udv_identity(m, lf)) return 1;
     int tau_left = 0;
     for (int l = 0; l < m.L; l++) {
         if (stage_metro(ctx, m, l)) return 1;
         if (stage_wrap(ctx, m, l)) return 1;
         if (((l + 1) % m.stab) == 0 && (l + 1) < m.L) {
             const int tau = l + 1, j = tau / m.stab;
-            if (stage_lmul(ctx, m, lf, tau_left, tau - tau_left)) return 1;
+            const Udv cur = carry ? m.alt(j) : lf;
+            if (stage_lmul(ctx, m, carry ? m.alt(j - 1) : lf, cur, tau_left, tau - tau_left)) return 1;
             tau_left = tau;
-            if (m.two_sided) {
-                if (udv_inv_two_sided(m.w, lf, m.stack(j), m.Gnew, (long long)m.n * m.n)) return 1;
-            } else {
-                if (udv_combine(m.w, lf, m.stack(j), m.comb())) return 1;
-                if (udv_inv_one_plus(m.w, m.comb(), m.Gnew, (long long)m.n * m.n)) return 1;
-            }
-            if (stage_accept_rebuild(ctx, m, 1)) return 1;
+            if (stage_boundary_rebuild(ctx, m, cur, m.stack(j))) return 1;
         }
     }
-    if (m.L - tau_left > 0)
-        if (stage_lmul(ctx, m, lf, tau_left, m.L - tau_left)) return 1;
-    if (udv_inv_one_plus(m.w, lf, m.Gnew, (long long)m.n * m.n)) return 1;
+    // tau = L: the whole chain B_{L-1} ... B_0 (A[M] in the alternating mode,
+    // src/dqmc.c:769-771)
+    const Udv fin = carry ? m.alt(m.M) : lf;
+    if (m.L - tau_left > 0) {
+        if (stage_lmul(ctx, m, carry ? m.alt(tau_left / m.stab) : lf, fin, tau_left, m.L - tau_left)) return 1;
+    }
+    if (udv_inv_one_plus(m.w, fin, m.Gnew, nn)) return 1;
     return stage_accept_rebuild(ctx, m, 1);
 }
 
+// S[j] = S[j+1] Bblock_j in a backward sweep, after the Metropolis of
+// block j: its fields are final, so the block product is formed here
+// (green_build_Bblock + udv_rmul, src/dqmc.c:828-830, 924-928)
+static int stage_suffix_block(dqmc_gpu_ctx *ctx, Model &m, int j)
+{
+    if (m.BB && !chain_f32(ctx)) {
+        const long long nn = (long long)m.n * m.n;
+        const int lb = j * m.stab;
+        ProfScope ps_(DQMC_GPU_PROF_STACK, m.w.s);
+        if (bblock_build(m, lb, std::min(m.stab, m.L - lb), m.BB + (long long)j * m.nrep * nn)) return 1;
+    }
+    return stage_rmul(ctx, m, j);
+}
+
+// One backward sweep (dqmc_sweep_backward + dqmc_backward_boundary,
+// src/dqmc.c:815-950): slices L-1 .. 0 with G carried down by the backward
+// wrap; at tau = j s the suffix S[j] = B_{L-1} ... B_tau is accumulated
+// into the suffix stack (read by the next forward sweep) and G is rebuilt
+// from A[j] and S[j]; at tau = 0, G = (1 + S[0])^-1.  S[M] is the identity.
+static int sweep_backward(dqmc_gpu_ctx *ctx, Model &m, bool build_prefix)
+{
+    const long long nn = (long long)m.n * m.n;
+    if (build_prefix) {
+        // green_stack_build_prefix (src/green.c:228-250); A[M] is never read
+        // by the backward sweep
+        for (int j = 1; j < m.M; j++) {
+            const int lb = (j - 1) * m.stab;
+            if (stage_lmul(ctx, m, m.alt(j - 1), m.alt(j), lb, std::min(m.stab, m.L - lb))) return 1;
+        }
+    }
+    if (stage_wrap_back(ctx, m, m.L - 1)) return 1;
+    for (int l = m.L - 1; l >= 0; l--) {
+        if (stage_metro(ctx, m, l)) return 1;
+        if (l == 0) break;
+        if (l % m.stab == 0) {
+            const int j = l / m.stab;
+            if (stage_suffix_block(ctx, m, j)) return 1;
+            if (stage_boundary_rebuild(ctx, m, m.alt(j), m.stack(j))) return 1;
+        }
+        if (stage_wrap_back(ctx, m, l - 1)) return 1;
+    }
+    if (stage_suffix_block(ctx, m, 0)) return 1;
+    if (udv_inv_one_plus(m.w, m.stack(0), m.Gnew, nn)) return 1;
+    return stage_accept_rebuild(ctx, m, 1);
+}
+
+static int sweep_body(dqmc_gpu_ctx *ctx, Model &m, int v)
+{
+    switch (v) {
+    case SW_FORWARD: return sweep_forward(ctx, m, true, false);
+    case SW_FWD_BUILD: return sweep_forward(ctx, m, true, true);
+    case SW_FWD_CARRIED: return sweep_forward(ctx, m, false, true);
+    case SW_BWD_BUILD: return sweep_backward(ctx, m, true);
+    case SW_BWD_CARRIED: return sweep_backward(ctx, m, false);
+    }
+    set_err("unknown sweep variant %d", v);
+    return 1;
+}
+
+// One sweep of a model (dqmc_sweep, src/dqmc.c:952-963); each variant is
+// captured once as a hipGraph and replayed.
 static int launch_sweep(dqmc_gpu_ctx *ctx, Model &m)
 {
     if (m.BB && !m.bb_valid && bblock_refresh(m)) return 1;
-    if (!ctx->opts.use_graph) return sweep_once(ctx, m);
-    if (!m.gexec) {
+    const int v = sweep_variant(m);
+    if (!ctx->opts.use_graph) {
+        if (sweep_body(ctx, m, v)) return 1;
+        sweep_advance(m, v);
+        return 0;
+    }
+    if (!m.gexec[v]) {
         HIPCHK(hipStreamBeginCapture(m.w.s, hipStreamCaptureModeThreadLocal));
-        const int rc = sweep_once(ctx, m);
+        const int rc = sweep_body(ctx, m, v);
         hipGraph_t gr = nullptr;
         const hipError_t e = hipStreamEndCapture(m.w.s, &gr);
         if (rc) {
@@ -1186,10 +1351,11 @@ static int launch_sweep(dqmc_gpu_ctx *ctx, Model &m)
             set_err("hipStreamEndCapture: %s", hipGetErrorString(e));
             return 1;
         }
-        m.graph = gr;
-        HIPCHK(hipGraphInstantiate(&m.gexec, m.graph, nullptr, nullptr, 0));
+        m.graph[v] = gr;
+        HIPCHK(hipGraphInstantiate(&m.gexec[v], m.graph[v], nullptr, nullptr, 0));
     }
-    HIPCHK(hipGraphLaunch(m.gexec, m.w.s));
+    HIPCHK(hipGraphLaunch(m.gexec[v], m.w.s));
+    sweep_advance(m, v);
     return 0;
 }
 
@@ -1417,6 +1583,25 @@ int dqmc_gpu_set_measure(dqmc_gpu_ctx *ctx, int model_id, int nq, const double *
     return 0;
 }
 
+int dqmc_gpu_set_sweep_order(dqmc_gpu_ctx *ctx, int model_id, int order)
+{
+    if (!ctx || model_id < 0 || model_id >= (int)ctx->models.size() ||
+        (order != DQMC_GPU_SWEEP_FORWARD && order != DQMC_GPU_SWEEP_ALTERNATING)) {
+        set_err("invalid set_sweep_order arguments");
+        return 1;
+    }
+    Model &m = *ctx->models[model_id];
+    m.alternating = order == DQMC_GPU_SWEEP_ALTERNATING;
+    m.dir = 0;
+    m.prefix_ok = m.suffix_ok = false;
+    if (m.alternating && m.allocated) {
+        HIPCHK(hipSetDevice(ctx->device));
+        if (alt_alloc(m)) return 1;
+        HIPCHK(hipStreamSynchronize(m.w.s));
+    }
+    return 0;
+}
+
 static int sync_all(dqmc_gpu_ctx *ctx)
 {
     for (Model *m : ctx->models) HIPCHK(hipStreamSynchronize(m->w.s));
@@ -1636,6 +1821,7 @@ int dqmc_gpu_write_field(dqmc_gpu_ctx *ctx, int replica, const signed char *s)
     }
     HIPCHK(hipMemcpy(m->field + Ln * r, s, Ln, hipMemcpyHostToDevice));
     m->bb_valid = false;
+    m->prefix_ok = m->suffix_ok = false; // dqmc_invalidate_carried (src/dqmc.c:31-35)
     HIPCHK(hipMemcpy(m->dvec + Ln * r, dv.data(), Ln * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(m->dinv + Ln * r, di.data(), Ln * 8, hipMemcpyHostToDevice));
     if (m->dvecf) {
@@ -1671,6 +1857,7 @@ int dqmc_gpu_step(dqmc_gpu_ctx *ctx, int step, int arg)
         case DQMC_GPU_STEP_PROPOSE_SLICE:
             rc = stage_metro(ctx, *m, arg);
             m->bb_valid = false;
+            m->prefix_ok = m->suffix_ok = false;
             break;
         case DQMC_GPU_STEP_WRAP: rc = stage_wrap(ctx, *m, arg); break;
         case DQMC_GPU_STEP_REBUILD:
@@ -1702,7 +1889,9 @@ int dqmc_gpu_profile_sweep(dqmc_gpu_ctx *ctx, int nsweep, double *ms, int *launc
         for (Model *m : ctx->models) {
             if (m->nrep == 0) continue;
             rc = (m->BB && !m->bb_valid) ? bblock_refresh(*m) : 0;
-            if (!rc) rc = sweep_once(ctx, *m);
+            const int v = sweep_variant(*m);
+            if (!rc) rc = sweep_body(ctx, *m, v);
+            if (!rc) sweep_advance(*m, v);
             if (!rc && m->nbin > 0) rc = stage_measure(*m, 0);
             if (rc) break;
         }
@@ -1730,8 +1919,10 @@ void dqmc_gpu_destroy(dqmc_gpu_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     for (Model *m : ctx->models) {
         if (m->w.s) (void)hipStreamSynchronize(m->w.s);
-        if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
-        if (m->graph) (void)hipGraphDestroy(m->graph);
+        for (int v = 0; v < SW_N; v++) {
+            if (m->gexec[v]) (void)hipGraphExecDestroy(m->gexec[v]);
+            if (m->graph[v]) (void)hipGraphDestroy(m->graph[v]);
+        }
         for (void *p : m->owned) (void)hipFree(p);
         m->w.release();
         if (m->w.s) (void)hipStreamDestroy(m->w.s);

@@ -27,6 +27,8 @@ MIXED_FP32_CHAIN = 2
 STEP_PROPOSE_SLICE = 0
 STEP_WRAP = 1
 STEP_REBUILD = 2
+SWEEP_FORWARD = 0
+SWEEP_ALTERNATING = 1
 
 _lib = None
 
@@ -39,6 +41,7 @@ SIGNATURES = {
     "dqmc_gpu_add_model": (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _P, _D, _D, _D, _D, _I, _I, C.POINTER(_I)]),
     "dqmc_gpu_add_replicas": (_I, [_P, _I, _I, _P, _P, C.POINTER(_I)]),
     "dqmc_gpu_set_measure": (_I, [_P, _I, _I, _P, _I]),
+    "dqmc_gpu_set_sweep_order": (_I, [_P, _I, _I]),
     "dqmc_gpu_init": (_I, [_P]),
     "dqmc_gpu_sweep": (_I, [_P, _I, _I]),
     "dqmc_gpu_run": (_I, [_P, _I, _I, _I]),
@@ -267,6 +270,11 @@ class Engine:
         _check(lib().dqmc_gpu_set_measure(self.h, model_id, nq, _ptr(ph), nbin), "dqmc_gpu_set_measure")
         m.nq, m.nbin = nq, nbin
         return mq
+
+    def set_sweep_order(self, model_id, order="forward"):
+        """sweep_order = "forward" or "alternating" (src/io.c:284-286)."""
+        code = {"forward": SWEEP_FORWARD, "alternating": SWEEP_ALTERNATING}[order]
+        _check(lib().dqmc_gpu_set_sweep_order(self.h, model_id, code), "dqmc_gpu_set_sweep_order")
 
     def init(self):
         _check(lib().dqmc_gpu_init(self.h), "dqmc_gpu_init")

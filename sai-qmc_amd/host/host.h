@@ -27,6 +27,8 @@ typedef struct {
     char szz_q[256], szz_file[256], sperp_q[256], sperp_file[256];
     char stab_drift_file[256];
     unsigned long long seed;
+    char sweep_order[16];   /* forward | alternating */
+    char green_rebuild[16]; /* combine | two_sided (centered is not run) */
     /* engine keys without a counterpart in the reference */
     char precision[24]; /* fp64 | mixed | mixed_fp32_chain */
     int delay;          /* delayed-update depth, 0 = engine default */

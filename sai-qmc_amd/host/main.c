@@ -219,7 +219,7 @@ int main(int argc, char **argv)
         return 1;
     }
     fprintf(fo, "# lattice=%s n=%d U=%g mu=%g dtau=%g bipartite=%d green_rebuild=%s parallel=%s nrep=%d bins=%d",
-            p.lattice, n, p.U, mu, p.dtau, L.is_bipartite, "combine", p.parallel, p.nrep, p.nrep * p.nbin);
+            p.lattice, n, p.U, mu, p.dtau, L.is_bipartite, p.green_rebuild, p.parallel, p.nrep, p.nrep * p.nbin);
     if (szz.enabled) fprintf(fo, " szz_file=%s szz_nq=%d", p.szz_file, szz.nq);
     if (sperp.enabled) fprintf(fo, " sperp_file=%s sperp_nq=%d", p.sperp_file, sperp.nq);
     fprintf(fo, "\n# T  E_hub dE_hub  E_gc dE_gc  E_ph dE_ph  ntot dN  doublon dD  sign  acceptance dAcceptance\n");
@@ -287,6 +287,10 @@ int main(int argc, char **argv)
         for (int r = 0; r < nrep; r++) seeds[r] = replica_seed(p.seed, b, r);
         if (dqmc_gpu_add_replicas(ctx, mid, nrep, seeds, NULL, &first)) return fail("dqmc_gpu_add_replicas", ctx);
         if (dqmc_gpu_set_measure(ctx, mid, nq, phase, nbin)) return fail("dqmc_gpu_set_measure", ctx);
+        if (dqmc_gpu_set_sweep_order(ctx, mid,
+                                     strcmp(p.sweep_order, "alternating") ? DQMC_GPU_SWEEP_FORWARD
+                                                                          : DQMC_GPU_SWEEP_ALTERNATING))
+            return fail("dqmc_gpu_set_sweep_order", ctx);
         if (dqmc_gpu_init(ctx)) return fail("dqmc_gpu_init", ctx);
         if (dqmc_gpu_run(ctx, p.nwarm, per, nbin)) return fail("dqmc_gpu_run", ctx);
         if (dqmc_gpu_read_status(ctx, st, why, NULL, NULL, NULL)) return fail("dqmc_gpu_read_status", ctx);

@@ -5,9 +5,10 @@
  * params_read (src/io.c:71-103 defaults, 156-418 parsing and checks):
  * '#' starts a comment, one key=value per line, unknown keys are errors.
  * Keys that select reference code paths the GPU engine does not implement
- * (sweep_order=alternating, green_rebuild=two_sided|centered, lattice=file,
- * profile=1, the UDV diagnostics files, replica_log,
- * spin_consistency_file) are accepted only at their default values.
+ * (green_rebuild=centered, lattice=file, profile=1, the UDV diagnostics
+ * files, replica_log, spin_consistency_file) are accepted only at their
+ * default values.  green_rebuild=two_sided selects the boundary rebuild the
+ * engine runs for every mode (the one-QR two-sided inverse).
  * Additional keys: precision=fp64|mixed|mixed_fp32_chain, delay=<k>, device=<id>.
  */
 #include "host.h"
@@ -42,6 +43,8 @@ static void defaults(Params *p)
     strcpy(p->sperp_file, "sperp.dat");
     p->seed = 12345ULL;
     strcpy(p->precision, "fp64");
+    strcpy(p->sweep_order, "forward");  /* src/io.c:92-93 */
+    strcpy(p->green_rebuild, "combine");
     p->delay = 0;
     p->device = 0;
 }
@@ -108,8 +111,7 @@ static int beta_list(Params *p, char *v)
 static int default_only(const char *key, const char *val)
 {
     static const char *fixed[][2] = {
-        {"sweep_order", "forward"}, {"green_rebuild", "combine"}, {"profile", "0"},
-        {"spin_consistency_file", "none"},
+        {"profile", "0"}, {"spin_consistency_file", "none"},
     };
     for (size_t i = 0; i < sizeof fixed / sizeof fixed[0]; i++)
         if (strcmp(key, fixed[i][0]) == 0) {
@@ -223,6 +225,10 @@ int params_read(Params *p, const char *path)
             const int r = beta_list(p, val);
             if (r == 2) BAD("ERROR: beta_list has too many entries (max 64)\n");
             else if (r) BAD("ERROR: beta_list must contain positive numeric values\n");
+        } else if (!strcmp(key, "sweep_order")) {
+            if (copy_str(p->sweep_order, sizeof p->sweep_order, val)) BAD("ERROR: invalid sweep_order\n");
+        } else if (!strcmp(key, "green_rebuild")) {
+            if (copy_str(p->green_rebuild, sizeof p->green_rebuild, val)) BAD("ERROR: invalid green_rebuild\n");
         } else if (!strcmp(key, "precision")) {
             if (copy_str(p->precision, sizeof p->precision, val)) BAD("ERROR: invalid precision\n");
         } else if (!strcmp(key, "delay")) {
@@ -250,6 +256,16 @@ int params_read(Params *p, const char *path)
     else if (!(p->dtau > 0.0)) err = "dtau must be > 0";
     else if (p->nwarm < 0) err = "nwarm must be >= 0";
     else if (p->stab <= 0) err = "stab must be > 0";
+    else if (strcmp(p->sweep_order, "forward") && strcmp(p->sweep_order, "alternating")) {
+        fprintf(stderr, "ERROR: sweep_order must be forward or alternating (got %s)\n", p->sweep_order);
+        return 1;
+    } else if (strcmp(p->green_rebuild, "combine") && strcmp(p->green_rebuild, "two_sided") &&
+               strcmp(p->green_rebuild, "centered")) {
+        fprintf(stderr, "ERROR: green_rebuild must be combine, two_sided, or centered (got %s)\n",
+                p->green_rebuild);
+        return 1;
+    } else if (!strcmp(p->green_rebuild, "centered"))
+        err = "green_rebuild=centered is not supported by the GPU engine";
     else if (strcmp(p->parallel, "serial") && strcmp(p->parallel, "omp") && strcmp(p->parallel, "mpi") &&
              strcmp(p->parallel, "hybrid"))
         err = "parallel must be serial, omp, mpi, or hybrid";

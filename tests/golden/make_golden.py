@@ -29,6 +29,8 @@ Fixtures (all numpy .npz, no pickles):
   c2_observables.txt, c2_szz.txt, c2_sperp.txt (8x8, 64 replicas)
                     the reference's output files for the 4-site chain run
                     (nwarm=300 nmeas=3000 nbin=30 seed=1, szz_q=all, sperp_q=all).
+  c3_observables.txt, c3_szz.txt, c3_sperp.txt (4x4, beta=3, stab=5, 8
+                    replicas, sweep_order=alternating, green_rebuild=two_sided)
 """
 import os
 import subprocess
@@ -232,10 +234,30 @@ def c2_outputs():
                 fo.write(fi.read())
 
 
+C3_INPUT = ("lattice=square\nLx=4\nLy=4\nU=4\ndtau=0.1\nbeta_list=3\nstab=5\nnwarm=200\nnmeas=2000\nnbin=20\n"
+            "seed=3\nparallel=omp\nnrep=8\nsweep_order=alternating\ngreen_rebuild=two_sided\nszz_q=all\nsperp_q=all\n")
+
+
+def c3_outputs():
+    """4x4, beta=3, stab=5, 8 replicas, sweep_order=alternating with the
+    two-sided boundary rebuild: the reference's OpenMP driver."""
+    with tempfile.TemporaryDirectory() as d:
+        inp = os.path.join(d, "c3.in")
+        with open(inp, "w") as f:
+            f.write(C3_INPUT)
+        subprocess.run([pyoracle.REF_OMP, inp], cwd=d, check=True, stdout=subprocess.DEVNULL,
+                       env=dict(os.environ, OMP_NUM_THREADS=str(os.cpu_count() or 1), OPENBLAS_NUM_THREADS="1"))
+        for src, dst in (("observables.dat", "c3_observables.txt"), ("szz.dat", "c3_szz.txt"),
+                         ("sperp.dat", "c3_sperp.txt")):
+            with open(os.path.join(d, src)) as fi, open(os.path.join(HERE, dst), "w") as fo:
+                fo.write(fi.read())
+
+
 if __name__ == "__main__":
     if not pyoracle.ref_available():
         sys.exit("oracle/_ref not built: make -C oracle ref")
-    gens = [traj_chain4, traj_sq8, traj_alt, model, udv_ops, seeds, ref_fixtures, ed_l4, c1_outputs, c2_outputs]
+    gens = [traj_chain4, traj_sq8, traj_alt, model, udv_ops, seeds, ref_fixtures, ed_l4, c1_outputs, c2_outputs,
+            c3_outputs]
     # optional names on the command line select generators (default: all)
     pick = set(sys.argv[1:])
     for g in gens:

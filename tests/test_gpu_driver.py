@@ -8,7 +8,9 @@ if a single marginal accept ever resolved differently.
 
 C1: chain L=4, U=4, beta=4, 300+3000 sweeps, 30 bins, seed 1.
 C2: 8x8, U=4, beta=4, stab 10, 64 replicas (replica_seed(1, 0, r)),
-    200+2000 sweeps, 20 bins."""
+    200+2000 sweeps, 20 bins.
+C3: 4x4, U=4, beta=3, stab 5, 8 replicas, sweep_order=alternating,
+    green_rebuild=two_sided, 200+2000 sweeps, 20 bins."""
 import os
 import subprocess
 
@@ -25,6 +27,8 @@ CASES = {
           "seed=1\nszz_q=all\nsperp_q=all\n",
     "c2": "lattice=square\nLx=8\nLy=8\nU=4\ndtau=0.1\nbeta_list=4\nstab=10\nnwarm=200\n"
           "nmeas=2000\nnbin=20\nseed=1\nparallel=omp\nnrep=64\nszz_q=all\nsperp_q=all\n",
+    "c3": "lattice=square\nLx=4\nLy=4\nU=4\ndtau=0.1\nbeta_list=3\nstab=5\nnwarm=200\nnmeas=2000\nnbin=20\n"
+          "seed=3\nparallel=omp\nnrep=8\nsweep_order=alternating\ngreen_rebuild=two_sided\nszz_q=all\nsperp_q=all\n",
 }
 
 
@@ -41,7 +45,7 @@ def within(a, ea, b, eb, what):
     assert not bad.any(), (what, a[bad], b[bad], tol[bad])
 
 
-@pytest.mark.parametrize("case", ["c1", "c2"])
+@pytest.mark.parametrize("case", ["c1", "c2", "c3"])
 def test_driver_matches_reference_outputs(case, tmp_path):
     if not os.path.exists(os.path.join(GOLD, f"{case}_observables.txt")):
         pytest.skip(f"no reference outputs for {case}")

@@ -40,3 +40,86 @@ def test_fp64_trajectory(orc, eng_mod, kind, Lx, Ly, L, stab, k, seeds, nsw):
             # reference (9.7e-11 at 8x8, beta=4 after 3 sweeps): 5e-10
             assert err < 5e-10, err
     eng.close()
+
+
+@pytest.mark.parametrize("kind,Lx,Ly,L,stab,k,seeds,nsw", [
+    ("chain", 6, 1, 20, 7, 16, [5], 6),
+    ("square", 4, 4, 24, 5, 8, [3, 4], 5),
+    ("square", 8, 8, 40, 10, 16, [1], 4),
+])
+def test_fp64_alternating_trajectory(orc, eng_mod, kind, Lx, Ly, L, stab, k, seeds, nsw):
+    # sweep_order=alternating (src/dqmc.c:815-963): forward legs store the
+    # prefix stack, backward legs the suffix stack; after sweep 2 the host
+    # rewrites the fields, which invalidates both (the next backward leg
+    # rebuilds its prefix stack, src/dqmc.c:867-872)
+    U, dtau = 4.0, 0.1
+    eng = eng_mod.Engine(eng_mod.FP64)
+    mid = eng.add_model(kind, Lx, Ly, U, dtau, L, stab, delay_k=k)
+    eng.set_sweep_order(mid, "alternating")
+    eng.add_replicas(mid, seeds)
+    eng.init()
+    refs = [orc.Replica(kind, Lx, Ly, U, dtau, L, stab, s, alternating=True) for s in seeds]
+    for s in range(nsw):
+        if s == 3:
+            for r, ref in enumerate(refs):
+                eng.set_field(r, ref.field())
+                ref.set_field(ref.field())
+        eng.sweep(1)
+        st = eng.status()
+        for r, ref in enumerate(refs):
+            ref.sweep()
+            assert st["status"][r] == 0
+            assert np.array_equal(eng.field(r), ref.field()), f"sweep {s} replica {r}"
+            acc, att, sign, ds = ref.stats()
+            assert int(st["accepted"][r]) == acc and int(st["attempts"][r]) == att
+            go = ref.green()[0]
+            err = np.abs(eng.green(r) - go).max() / np.abs(go).max()
+            # the backward leg's G at tau = 0 comes from the rmul chain
+            # S[0], whose QRs factor row-graded matrices D T B: the blocked
+            # (32-column WY) Householder QR loses accuracy in their small
+            # rows (5.7e-10 at 8x8 after the first backward sweep; the same
+            # chain with unblocked Householder reaches the oracle's 5e-11
+            # distance from the compiled reference): 1e-9
+            assert err < 1e-9, (s, r, err)
+    eng.close()
+
+
+@pytest.mark.parametrize("kind,Lx,Ly,L,stab,sweeps,seed", [
+    ("chain", 4, 1, 10, 4, 5, 9001),
+    ("chain", 4, 1, 5, 8, 4, 9002),   # one block (stab > L)
+    ("chain", 4, 1, 6, 1, 4, 9003),   # a boundary after every slice
+    ("chain", 4, 1, 4, 4, 4, 9004),
+    ("square", 2, 4, 8, 4, 4, 9011),
+])
+def test_alternating_reference_cases(orc, eng_mod, kind, Lx, Ly, L, stab, sweeps, seed):
+    # the cases of the reference's tests/test_dqmc_alternating.c:182-196:
+    # after every sweep G equals the from-scratch G of the current field
+    # within 1e-9, the sign is 1, the stacks invalidated before sweeps 1
+    # (prefix) and 2 (suffix) are rebuilt (:83-88), and the wrap drift stays
+    # below 1e-7 (:103-107)
+    eng = eng_mod.Engine(eng_mod.FP64, drift=True)
+    mid = eng.add_model(kind, Lx, Ly, 4.0, 0.1, L, stab, delay_k=16)
+    eng.set_sweep_order(mid, "alternating")
+    eng.add_replicas(mid, [seed])
+    eng.init()
+    ref = orc.Replica(kind, Lx, Ly, 4.0, 0.1, L, stab, seed, alternating=True)
+    scratch = orc.Replica(kind, Lx, Ly, 4.0, 0.1, L, stab, seed)
+    for s in range(sweeps):
+        if s in (1, 2):
+            eng.set_field(0, ref.field())
+            ref.set_field(ref.field())
+        eng.sweep(1)
+        assert ref.sweep() == 0
+        st = eng.status()
+        assert st["status"][0] == 0 and st["sign"][0] == 1.0
+        assert np.array_equal(eng.field(0), ref.field()), s
+        g = eng.green(0)
+        scratch.set_field(eng.field(0))
+        assert scratch.rebuild() == 0
+        g0 = scratch.green()[0]
+        assert np.abs(g - g0).max() < 1e-9, s
+        assert np.abs(g - ref.green()[0]).max() < 1e-10, s
+    mx, mn, cnt = eng.drift()
+    assert int(cnt[0]) == sweeps * ((L - 1) // stab + 1)
+    assert mx[0] < 1e-7
+    eng.close()

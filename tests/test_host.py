@@ -21,7 +21,8 @@ class Params(C.Structure):
                 ("output_file", C.c_char * 256), ("parallel", C.c_char * 16), ("nrep", C.c_int),
                 ("szz_q", C.c_char * 256), ("szz_file", C.c_char * 256), ("sperp_q", C.c_char * 256),
                 ("sperp_file", C.c_char * 256), ("stab_drift_file", C.c_char * 256),
-                ("seed", C.c_ulonglong), ("precision", C.c_char * 24), ("delay", C.c_int), ("device", C.c_int)]
+                ("seed", C.c_ulonglong), ("sweep_order", C.c_char * 16), ("green_rebuild", C.c_char * 16),
+                ("precision", C.c_char * 24), ("delay", C.c_int), ("device", C.c_int)]
 
 
 class Lattice(C.Structure):
@@ -123,10 +124,17 @@ def test_params_defaults_and_keys(host, tmp_path):
     assert p.seed == 9 and p.nrep == 3 and p.precision == b"mixed" and p.szz_q == b"all"
     assert p.dtau == 0.1 and p.thop == -1.0 and p.pbc == 1
     assert p.output_file == b"observables.dat" and p.sperp_q == b"none"
+    assert p.sweep_order == b"forward" and p.green_rebuild == b"combine"
+
+
+def test_params_sweep_order_and_rebuild(host, tmp_path):
+    rc, p = read(host, tmp_path, "sweep_order=alternating\ngreen_rebuild=two_sided\n")
+    assert rc == 0 and p.sweep_order == b"alternating" and p.green_rebuild == b"two_sided"
 
 
 @pytest.mark.parametrize("text", ["lattice=hex\n", "Lx=abc\n", "nosuchkey=1\n", "precision=fp16\n",
-                                  "sweep_order=alternating\n", "dtau=-1\n", "beta_list=\n", "Lx\n"])
+                                  "sweep_order=backward\n", "green_rebuild=centered\n", "green_rebuild=lu\n",
+                                  "dtau=-1\n", "beta_list=\n", "Lx\n"])
 def test_params_rejects(host, tmp_path, text):
     rc, _ = read(host, tmp_path, text)
     assert rc != 0
