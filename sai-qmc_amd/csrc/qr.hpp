@@ -869,14 +869,19 @@ __global__ __launch_bounds__(256) void k_trsm_unit_upper(TrsmArgs a)
             const bool iok = i0 + lc < r0 + ib;
             const int ic = min(i0 + lc, n - 1);
             const int kbeg = tr ? 0 : r0 + ib, kend = tr ? r0 : n;
-            for (int k0 = kbeg; k0 < kend; k0 += 64) {
-                double ua[16], xb[16];
+            // U chunks of 64 k are double-buffered: the next chunk's loads
+            // are in flight while this chunk's MFMAs run
+            auto load_u = [&](double (&ua)[16], int k0) {
 #pragma unroll
                 for (int u = 0; u < 16; u++) {
                     const int kc = min(k0 + 4 * u + lr, n - 1);
                     ua[u] = tr ? U[kc + (long long)ic * n] : U[ic + (long long)kc * n];
-                    xb[u] = Xs[cc * ldx + kc];
                 }
+            };
+            auto mma_chunk = [&](double (&ua)[16], int k0) {
+                double xb[16];
+#pragma unroll
+                for (int u = 0; u < 16; u++) xb[u] = Xs[cc * ldx + min(k0 + 4 * u + lr, n - 1)];
 #pragma unroll
                 for (int u = 0; u < 16; u++) {
                     const bool ok = k0 + 4 * u + lr < kend;
@@ -887,6 +892,18 @@ __global__ __launch_bounds__(256) void k_trsm_unit_upper(TrsmArgs a)
                 for (int u = 0; u < 16; u += 2) {
                     acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(ua[u], xb[u], acc0, 0, 0, 0);
                     acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ua[u + 1], xb[u + 1], acc1, 0, 0, 0);
+                }
+            };
+            if (kbeg < kend) {
+                double ua0[16], ua1[16];
+                load_u(ua0, kbeg);
+                for (int k0 = kbeg; k0 < kend; k0 += 128) {
+                    if (k0 + 64 < kend) load_u(ua1, k0 + 64);
+                    mma_chunk(ua0, k0);
+                    if (k0 + 64 < kend) {
+                        if (k0 + 128 < kend) load_u(ua0, k0 + 128);
+                        mma_chunk(ua1, k0 + 64);
+                    }
                 }
             }
 #pragma unroll
