@@ -137,6 +137,12 @@ int dqmc_gpu_read_status(dqmc_gpu_ctx *ctx, int *status, int *failure_reason,
    rebuilt G of the stack is the reference point). */
 int dqmc_gpu_read_drift(dqmc_gpu_ctx *ctx, double *max_inf, double *mean_inf,
                         unsigned long long *samples);
+/* Where each replica's largest drift sample was taken: the slice tau of
+   the rebuild and the number of sweeps completed before it (max_tau,
+   max_sweep of DqmcStabDrift, src/dqmc.h and src/dqmc.c:90-95; the
+   stab_drift_file columns, src/replica_run.c:58-88). */
+int dqmc_gpu_read_drift_at(dqmc_gpu_ctx *ctx, int *max_tau,
+                           unsigned long long *max_sweep);
 
 /* Parity and debug hooks. */
 int dqmc_gpu_read_green(dqmc_gpu_ctx *ctx, int replica, double *g);

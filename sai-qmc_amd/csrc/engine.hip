@@ -357,8 +357,12 @@ __global__ void k_drift_copy(double *G, float *Gf, const double *Gnew, long long
 // per replica: the running sign is reset to the exact +1 at every rebuild
 // (PH path, src/dqmc.c:740-742, 799-801), failures are recorded, the drift
 // sample is accumulated, and the gather slots are cleared for the next call
+// The slice tau of the sample and the model's completed-sweep counter are
+// kept with each new maximum (max_tau, max_sweep of DqmcStabDrift,
+// src/dqmc.c:90-95).
 __global__ void k_drift_finish(int batch, int drift, unsigned long long *tmax, int *tbad, double *dmax, double *dsum,
-                               unsigned long long *dcnt, int *status, int *reason, double *sign)
+                               unsigned long long *dcnt, int *status, int *reason, double *sign, int tau,
+                               const unsigned long long *sweeps, int *dtau, unsigned long long *dsweep)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= batch) return;
@@ -369,13 +373,23 @@ __global__ void k_drift_finish(int batch, int drift, unsigned long long *tmax, i
             reason[b] = DQMC_GPU_FAIL_NONFINITE;
         } else if (drift) {
             const double mm = __longlong_as_double((long long)tmax[b]);
-            dmax[b] = fmax(dmax[b], mm);
+            if (dcnt[b] == 0 || mm > dmax[b]) {
+                dmax[b] = mm;
+                dtau[b] = tau;
+                dsweep[b] = *sweeps;
+            }
             dsum[b] += mm;
             dcnt[b] += 1;
         }
     }
     tmax[b] = 0;
     tbad[b] = 0;
+}
+
+// one completed sweep of a model (the counter behind max_sweep)
+__global__ void k_sweep_tick(unsigned long long *sweeps)
+{
+    if (threadIdx.x == 0) *sweeps += 1;
 }
 
 // ---------------------------------------------------------------- buffers
@@ -673,6 +687,9 @@ struct Model {
     double *sign = nullptr;
     double *dmax = nullptr, *dsum = nullptr;
     unsigned long long *dcnt = nullptr;
+    int *dtaumax = nullptr;                 // tau of each replica's largest drift sample
+    unsigned long long *dsweepmax = nullptr; // ... and the sweep it was taken in
+    unsigned long long *sweeps = nullptr;    // completed sweeps of the model (device counter)
     double *bins = nullptr, *bszz = nullptr, *bsperp = nullptr;
     int *counts = nullptr;
     // sweep_order=alternating (src/dqmc.c:815-963): direction of the next
@@ -819,6 +836,11 @@ static int model_alloc(dqmc_gpu_ctx *ctx, Model &m)
         return 1;
     own(m.acc); own(m.att); own(m.acc_prev); own(m.att_prev); own(m.sign); own(m.dmax); own(m.dsum);
     own(m.dcnt);
+    if (dalloc(&m.dtaumax, R) || dalloc(&m.dsweepmax, R) || dalloc(&m.sweeps, 1)) return 1;
+    own(m.dtaumax); own(m.dsweepmax); own(m.sweeps);
+    HIPCHK(hipMemset(m.dtaumax, 0, sizeof(int) * R));
+    HIPCHK(hipMemset(m.dsweepmax, 0, sizeof(unsigned long long) * R));
+    HIPCHK(hipMemset(m.sweeps, 0, sizeof(unsigned long long)));
     const int nbin = std::max(m.nbin, 1), nq = std::max(m.nq, 1);
     if (dalloc(&m.bins, (size_t)8 * nbin * R) || dalloc(&m.counts, (size_t)nbin * R) ||
         dalloc(&m.bszz, (size_t)nq * nbin * R) || dalloc(&m.bsperp, (size_t)nq * nbin * R))
@@ -1149,7 +1171,7 @@ static int udv_identity(Model &m, const Udv &u)
 }
 
 // G <- Gnew with drift bookkeeping and the FP32 copy
-static int stage_accept_rebuild(dqmc_gpu_ctx *ctx, Model &m, int record_drift)
+static int stage_accept_rebuild(dqmc_gpu_ctx *ctx, Model &m, int record_drift, int tau)
 {
     ProfScope ps_(DQMC_GPU_PROF_ACCEPT, m.w.s);
     const long long nn = (long long)m.n * m.n;
@@ -1158,7 +1180,8 @@ static int stage_accept_rebuild(dqmc_gpu_ctx *ctx, Model &m, int record_drift)
                        drift, is_mixed(ctx) ? 1 : 0, m.dtmax, m.dtbad, m.w.status);
     if (launch_check("k_drift_copy")) return 1;
     hipLaunchKernelGGL(k_drift_finish, dim3((m.nrep + 255) / 256), dim3(256), 0, m.w.s, m.nrep, drift, m.dtmax,
-                       m.dtbad, m.dmax, m.dsum, m.dcnt, m.w.status, m.w.reason, m.sign);
+                       m.dtbad, m.dmax, m.dsum, m.dcnt, m.w.status, m.w.reason, m.sign, tau, m.sweeps, m.dtaumax,
+                       m.dsweepmax);
     return launch_check("k_drift_finish");
 }
 
@@ -1173,7 +1196,7 @@ static int stage_from_scratch(dqmc_gpu_ctx *ctx, Model &m, int l0)
         for (int lb = sg[0]; lb < sg[1]; lb += m.stab)
             if (stage_lmul(ctx, m, lf, lf, lb, std::min(m.stab, sg[1] - lb))) return 1;
     if (udv_inv_one_plus(m.w, lf, m.Gnew, (long long)m.n * m.n)) return 1;
-    return stage_accept_rebuild(ctx, m, 0);
+    return stage_accept_rebuild(ctx, m, 0, l0);
 }
 
 // G <- B_l^-1 G B_l = diag(1/d_l) expKinv G expK diag(d_l)
@@ -1201,7 +1224,7 @@ static int stage_wrap_back(dqmc_gpu_ctx *ctx, Model &m, int l)
 
 // G at a block boundary from a prefix and a suffix UDV
 // (green_from_boundary_factors, src/green.c:308-345), then drift + copy
-static int stage_boundary_rebuild(dqmc_gpu_ctx *ctx, Model &m, const Udv &prefix, const Udv &suffix)
+static int stage_boundary_rebuild(dqmc_gpu_ctx *ctx, Model &m, const Udv &prefix, const Udv &suffix, int tau)
 {
     const long long nn = (long long)m.n * m.n;
     if (m.two_sided) {
@@ -1210,7 +1233,7 @@ static int stage_boundary_rebuild(dqmc_gpu_ctx *ctx, Model &m, const Udv &prefix
         if (udv_combine(m.w, prefix, suffix, m.comb())) return 1;
         if (udv_inv_one_plus(m.w, m.comb(), m.Gnew, nn)) return 1;
     }
-    return stage_accept_rebuild(ctx, m, 1);
+    return stage_accept_rebuild(ctx, m, 1, tau);
 }
 
 // Sweep variants: the plain forward sweep, and the two legs of the
@@ -1255,7 +1278,7 @@ static int sweep_forward(dqmc_gpu_ctx *ctx, Model &m, bool build_suffix, bool ca
             const Udv cur = carry ? m.alt(j) : lf;
             if (stage_lmul(ctx, m, carry ? m.alt(j - 1) : lf, cur, tau_left, tau - tau_left)) return 1;
             tau_left = tau;
-            if (stage_boundary_rebuild(ctx, m, cur, m.stack(j))) return 1;
+            if (stage_boundary_rebuild(ctx, m, cur, m.stack(j), tau)) return 1;
         }
     }
     // tau = L: the whole chain B_{L-1} ... B_0 (A[M] in the alternating mode,
@@ -1265,7 +1288,7 @@ static int sweep_forward(dqmc_gpu_ctx *ctx, Model &m, bool build_suffix, bool ca
         if (stage_lmul(ctx, m, carry ? m.alt(tau_left / m.stab) : lf, fin, tau_left, m.L - tau_left)) return 1;
     }
     if (udv_inv_one_plus(m.w, fin, m.Gnew, nn)) return 1;
-    return stage_accept_rebuild(ctx, m, 1);
+    return stage_accept_rebuild(ctx, m, 1, 0);
 }
 
 // S[j] = S[j+1] Bblock_j in a backward sweep, after the Metropolis of
@@ -1305,26 +1328,29 @@ static int sweep_backward(dqmc_gpu_ctx *ctx, Model &m, bool build_prefix)
         if (l % m.stab == 0) {
             const int j = l / m.stab;
             if (stage_suffix_block(ctx, m, j)) return 1;
-            if (stage_boundary_rebuild(ctx, m, m.alt(j), m.stack(j))) return 1;
+            if (stage_boundary_rebuild(ctx, m, m.alt(j), m.stack(j), l)) return 1;
         }
         if (stage_wrap_back(ctx, m, l - 1)) return 1;
     }
     if (stage_suffix_block(ctx, m, 0)) return 1;
     if (udv_inv_one_plus(m.w, m.stack(0), m.Gnew, nn)) return 1;
-    return stage_accept_rebuild(ctx, m, 1);
+    return stage_accept_rebuild(ctx, m, 1, 0);
 }
 
 static int sweep_body(dqmc_gpu_ctx *ctx, Model &m, int v)
 {
+    int rc;
     switch (v) {
-    case SW_FORWARD: return sweep_forward(ctx, m, true, false);
-    case SW_FWD_BUILD: return sweep_forward(ctx, m, true, true);
-    case SW_FWD_CARRIED: return sweep_forward(ctx, m, false, true);
-    case SW_BWD_BUILD: return sweep_backward(ctx, m, true);
-    case SW_BWD_CARRIED: return sweep_backward(ctx, m, false);
+    case SW_FORWARD: rc = sweep_forward(ctx, m, true, false); break;
+    case SW_FWD_BUILD: rc = sweep_forward(ctx, m, true, true); break;
+    case SW_FWD_CARRIED: rc = sweep_forward(ctx, m, false, true); break;
+    case SW_BWD_BUILD: rc = sweep_backward(ctx, m, true); break;
+    case SW_BWD_CARRIED: rc = sweep_backward(ctx, m, false); break;
+    default: set_err("unknown sweep variant %d", v); return 1;
     }
-    set_err("unknown sweep variant %d", v);
-    return 1;
+    if (rc) return 1;
+    hipLaunchKernelGGL(k_sweep_tick, dim3(1), dim3(64), 0, m.w.s, m.sweeps);
+    return launch_check("k_sweep_tick");
 }
 
 // One sweep of a model (dqmc_sweep, src/dqmc.c:952-963); each variant is
@@ -1750,6 +1776,24 @@ int dqmc_gpu_read_drift(dqmc_gpu_ctx *ctx, double *max_inf, double *mean_inf, un
             if (max_inf) max_inf[f + r] = mx[r];
             if (mean_inf) mean_inf[f + r] = ct[r] ? sm[r] / (double)ct[r] : 0.0;
             if (samples) samples[f + r] = ct[r];
+        }
+    }
+    return 0;
+}
+
+int dqmc_gpu_read_drift_at(dqmc_gpu_ctx *ctx, int *max_tau, unsigned long long *max_sweep)
+{
+    if (require_init(ctx)) return 1;
+    if (sync_all(ctx)) return 1;
+    for (Model *m : ctx->models) {
+        const int R = m->nrep, f = m->first;
+        std::vector<int> tt(R);
+        std::vector<unsigned long long> ss(R);
+        HIPCHK(hipMemcpy(tt.data(), m->dtaumax, R * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ss.data(), m->dsweepmax, R * 8, hipMemcpyDeviceToHost));
+        for (int r = 0; r < R; r++) {
+            if (max_tau) max_tau[f + r] = tt[r];
+            if (max_sweep) max_sweep[f + r] = ss[r];
         }
     }
     return 0;

@@ -48,6 +48,7 @@ SIGNATURES = {
     "dqmc_gpu_read_bins": (_I, [_P, _I, _P, _P, _P, _P]),
     "dqmc_gpu_read_status": (_I, [_P, _P, _P, _P, _P, _P]),
     "dqmc_gpu_read_drift": (_I, [_P, _P, _P, _P]),
+    "dqmc_gpu_read_drift_at": (_I, [_P, _P, _P]),
     "dqmc_gpu_read_green": (_I, [_P, _I, _P]),
     "dqmc_gpu_write_green": (_I, [_P, _I, _P]),
     "dqmc_gpu_read_field": (_I, [_P, _I, _P]),
@@ -352,6 +353,13 @@ class Engine:
         cnt = np.zeros(n, dtype=np.uint64)
         _check(lib().dqmc_gpu_read_drift(self.h, _ptr(mx), _ptr(mn), _ptr(cnt)), "dqmc_gpu_read_drift")
         return mx, mn, cnt
+
+    def drift_at(self):
+        """(max_tau, max_sweep) of each replica's largest drift sample."""
+        tau = np.zeros(self.nrep, dtype=np.int32)
+        sw = np.zeros(self.nrep, dtype=np.uint64)
+        _check(lib().dqmc_gpu_read_drift_at(self.h, _ptr(tau), _ptr(sw)), "dqmc_gpu_read_drift_at")
+        return tau, sw
 
     def bins(self, r):
         m = self._model_of(r)

@@ -27,6 +27,8 @@ typedef struct {
     char szz_q[256], szz_file[256], sperp_q[256], sperp_file[256];
     char stab_drift_file[256];
     unsigned long long seed;
+    char replica_log[256];           /* "" = replicas.dat when nrep > 1, "none" = off */
+    char spin_consistency_file[256]; /* "none" = off */
     char sweep_order[16];   /* forward | alternating */
     char green_rebuild[16]; /* combine | two_sided (centered is not run) */
     /* engine keys without a counterpart in the reference */

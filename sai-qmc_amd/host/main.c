@@ -179,6 +179,22 @@ int main(int argc, char **argv)
         fprintf(stderr, "ERROR: parallel=%s: the GPU driver runs all replicas on one device\n", p.parallel);
         return 1;
     }
+    if (p.stab_drift_file[0] && strcmp(p.parallel, "serial")) {
+        /* src/main.c:724-730 */
+        fprintf(stderr, "ERROR: stab_drift_file currently supports parallel=serial only\n");
+        return 1;
+    }
+    if (p.stab_drift_file[0]) {
+        /* init_stab_drift_file (src/main.c:453-472) */
+        FILE *fd = fopen(p.stab_drift_file, "w");
+        if (!fd) {
+            fprintf(stderr, "ERROR: failed to open stab_drift_file %s\n", p.stab_drift_file);
+            return 1;
+        }
+        fprintf(fd, "# beta_index Ltr replica_id seed U dtau stab use_ph samples max_inf mean_inf max_tau max_sweep "
+                    "failed\n");
+        fclose(fd);
+    }
     Lattice L;
     if (lattice_build(&L, &p)) {
         fprintf(stderr, "ERROR: invalid lattice\n");
@@ -187,6 +203,20 @@ int main(int argc, char **argv)
     if (!L.is_bipartite) {
         fprintf(stderr, "ERROR: the GPU engine runs the particle-hole symmetric path (bipartite lattice)\n");
         return 1;
+    }
+    {
+        /* lattice_dump (src/lattice.c:184-201), written by every run
+           (src/main.c:663-665) */
+        FILE *fh = fopen("hopping_used.txt", "w");
+        if (fh) {
+            fprintf(fh, "%d\n", L.n);
+            for (int i = 0; i < L.n; i++)
+                for (int j = 0; j < L.n; j++)
+                    fprintf(fh, "%.17g%c", L.t[i + (size_t)j * L.n], j == L.n - 1 ? '\n' : ' ');
+            fclose(fh);
+        } else {
+            perror("hopping_used.txt");
+        }
     }
     const int n = L.n;
     const double mu = p.U / 2.0;
@@ -205,6 +235,20 @@ int main(int argc, char **argv)
         fprintf(stderr, "ERROR: invalid szz_q / sperp_q selector\n");
         return 1;
     }
+    /* DeltaSU2 = Sperp/2 - Szz needs the same momenta in both plans
+       (plans_have_same_q, src/main.c:379-392, 807-826) */
+    const int consistency = strcmp(p.spin_consistency_file, "none") != 0;
+    if (consistency) {
+        int same = szz.enabled && sperp.enabled && szz.nq == sperp.nq;
+        for (int q = 0; same && q < szz.nq; q++) same = szz.mx[q] == sperp.mx[q] && szz.my[q] == sperp.my[q];
+        if (!same) {
+            fprintf(stderr,
+                    "ERROR: invalid spin structure-factor setup szz_selector=%s szz_file=%s sperp_selector=%s "
+                    "sperp_file=%s consistency_file=%s lattice=%s Lx=%d Ly=%d\n",
+                    p.szz_q, p.szz_file, p.sperp_q, p.sperp_file, p.spin_consistency_file, p.lattice, L.Lx, L.Ly);
+            return 1;
+        }
+    }
     double *phase;
     int nq;
     union_plans(&L, &szz, &sperp, &phase, &nq);
@@ -222,6 +266,7 @@ int main(int argc, char **argv)
             p.lattice, n, p.U, mu, p.dtau, L.is_bipartite, p.green_rebuild, p.parallel, p.nrep, p.nrep * p.nbin);
     if (szz.enabled) fprintf(fo, " szz_file=%s szz_nq=%d", p.szz_file, szz.nq);
     if (sperp.enabled) fprintf(fo, " sperp_file=%s sperp_nq=%d", p.sperp_file, sperp.nq);
+    if (consistency) fprintf(fo, " spin_consistency_file=%s", p.spin_consistency_file);
     fprintf(fo, "\n# T  E_hub dE_hub  E_gc dE_gc  E_ph dE_ph  ntot dN  doublon dD  sign  acceptance dAcceptance\n");
     fflush(fo);
     FILE *fz = NULL, *fp = NULL;
@@ -257,6 +302,34 @@ int main(int argc, char **argv)
                     "qy_folded_over_pi Sperp dSperp\n");
     }
 
+    /* replica log (src/main.c:829-834, 917-933): replicas.dat by default
+       when nrep > 1, replica_log=<path> always, replica_log=none never */
+    FILE *fr = NULL;
+    if (strcmp(p.replica_log, "none") != 0 && (p.replica_log[0] || p.nrep > 1)) {
+        const char *path = p.replica_log[0] ? p.replica_log : "replicas.dat";
+        if (!(fr = fopen(path, "w"))) {
+            fprintf(stderr, "ERROR: failed to open replica_log %s\n", path);
+            return 1;
+        }
+        fprintf(fr, "# beta T replica_id seed nwarm nmeas nbin status\n");
+    }
+    FILE *fc = NULL;
+    if (consistency) {
+        /* src/main.c:996-1021 */
+        if (!(fc = fopen(p.spin_consistency_file, "w"))) {
+            fprintf(stderr, "ERROR: failed to open spin_consistency_file %s\n", p.spin_consistency_file);
+            return 1;
+        }
+        fprintf(fc, "# definition=DeltaSU2(q)=0.5*Sperp(q)-Szz(q)\n");
+        fprintf(fc, "# error=paired jackknife over the same sign-reweighted replica/bin samples\n");
+        fprintf(fc, "# lattice=%s Lx=%d Ly=%d n=%d pbc=%d t=%.17g U=%.17g mu=%.17g dtau=%.17g\n", p.lattice, L.Lx,
+                L.Ly, n, p.pbc, p.thop, p.U, mu, p.dtau);
+        fprintf(fc, "# szz_q=%s sperp_q=%s seed=%llu parallel=%s nrep=%d bins=%d nbeta=%d\n", p.szz_q, p.sperp_q,
+                p.seed, p.parallel, p.nrep, p.nrep * p.nbin, p.nbeta);
+        fprintf(fc, "# beta_requested beta T q_index mx my qx_over_pi qy_over_pi qx_folded_over_pi "
+                    "qy_folded_over_pi DeltaSU2 dDeltaSU2\n");
+    }
+
     const int nbin = p.nbin, nrep = p.nrep, total = nrep * nbin, per = p.nmeas / p.nbin;
     double *E[7];
     for (int k = 0; k < 7; k++) E[k] = malloc(sizeof(double) * total);
@@ -277,7 +350,7 @@ int main(int argc, char **argv)
             return 1;
         }
         const double beta_eff = Ltr * p.dtau, T = 1.0 / beta_eff;
-        dqmc_gpu_opts o = {precision, 1, 0, 0};
+        dqmc_gpu_opts o = {precision, 1, p.stab_drift_file[0] ? 1 : 0, 0};
         dqmc_gpu_ctx *ctx = NULL;
         if (dqmc_gpu_create(&ctx, p.device, &o)) return fail("dqmc_gpu_create", NULL);
         int mid, first;
@@ -344,6 +417,28 @@ int main(int argc, char **argv)
                 }
             }
         }
+        if (p.stab_drift_file[0]) {
+            /* append_stab_drift_row (src/replica_run.c:58-88), one row per
+               replica; the reference point of the drift is the stack's
+               rebuilt G at the boundary */
+            double *dmx = malloc(sizeof(double) * nrep), *dmn = malloc(sizeof(double) * nrep);
+            unsigned long long *dct = malloc(sizeof(unsigned long long) * nrep),
+                               *dsw = malloc(sizeof(unsigned long long) * nrep);
+            int *dtau = malloc(sizeof(int) * nrep);
+            FILE *fd = fopen(p.stab_drift_file, "a");
+            if (!fd || dqmc_gpu_read_drift(ctx, dmx, dmn, dct) || dqmc_gpu_read_drift_at(ctx, dtau, dsw)) {
+                fprintf(stderr, "ERROR: failed to write stab_drift_file %s\n", p.stab_drift_file);
+                if (fd) fclose(fd);
+                dqmc_gpu_destroy(ctx);
+                return 1;
+            }
+            for (int r = 0; r < nrep; r++)
+                fprintf(fd, "%d %d %d %llu %.17g %.17g %d %d %llu %.17g %.17g %d %llu %d\n", b, Ltr, r, seeds[r], p.U,
+                        p.dtau, p.stab, 1, dct[first + r], dmx[first + r], dmn[first + r], dtau[first + r],
+                        dsw[first + r], 0);
+            fclose(fd);
+            free(dmx); free(dmn); free(dct); free(dsw); free(dtau);
+        }
         dqmc_gpu_destroy(ctx);
         double m[7], e[7];
         for (int k = 0; k < 7; k++) {
@@ -358,10 +453,27 @@ int main(int argc, char **argv)
         fflush(fo);
         if (szz.enabled) write_q(fz, &szz, &L, beta, beta_eff, T, zq, total);
         if (sperp.enabled) write_q(fp, &sperp, &L, beta, beta_eff, T, pq, total);
+        if (fc) {
+            /* write_spin_consistency_beta (src/main.c:319-377): paired
+               jackknife of Sperp/2 - Szz over the same bins */
+            double *dq = malloc(sizeof(double) * (size_t)szz.nq * total);
+            for (size_t k = 0; k < (size_t)szz.nq * total; k++) dq[k] = 0.5 * pq[k] - zq[k];
+            write_q(fc, &szz, &L, beta, beta_eff, T, dq, total);
+            free(dq);
+        }
+        if (fr) {
+            /* src/main.c:1706-1713 */
+            for (int r = 0; r < nrep; r++)
+                fprintf(fr, "%.17g %.17g %d %llu %d %d %d %s\n", beta, T, r, seeds[r], p.nwarm, p.nmeas, p.nbin,
+                        st[r] ? "error" : "ok");
+            fflush(fr);
+        }
     }
     fclose(fo);
     if (fz) fclose(fz);
     if (fp) fclose(fp);
+    if (fc) fclose(fc);
+    if (fr) fclose(fr);
     plan_free(&szz);
     plan_free(&sperp);
     lattice_release(&L);

@@ -6,8 +6,7 @@
  * '#' starts a comment, one key=value per line, unknown keys are errors.
  * Keys that select reference code paths the GPU engine does not implement
  * (green_rebuild=centered, lattice=file, profile=1, the UDV diagnostics
- * files, replica_log, spin_consistency_file) are accepted only at their
- * default values.  green_rebuild=two_sided selects the boundary rebuild the
+ * files) are accepted only at their default values.  green_rebuild=two_sided selects the boundary rebuild the
  * engine runs for every mode (the one-QR two-sided inverse).
  * Additional keys: precision=fp64|mixed|mixed_fp32_chain, delay=<k>, device=<id>.
  */
@@ -43,6 +42,7 @@ static void defaults(Params *p)
     strcpy(p->sperp_file, "sperp.dat");
     p->seed = 12345ULL;
     strcpy(p->precision, "fp64");
+    strcpy(p->spin_consistency_file, "none"); /* src/io.c:96-101 */
     strcpy(p->sweep_order, "forward");  /* src/io.c:92-93 */
     strcpy(p->green_rebuild, "combine");
     p->delay = 0;
@@ -111,7 +111,7 @@ static int beta_list(Params *p, char *v)
 static int default_only(const char *key, const char *val)
 {
     static const char *fixed[][2] = {
-        {"profile", "0"}, {"spin_consistency_file", "none"},
+        {"profile", "0"},
     };
     for (size_t i = 0; i < sizeof fixed / sizeof fixed[0]; i++)
         if (strcmp(key, fixed[i][0]) == 0) {
@@ -123,7 +123,7 @@ static int default_only(const char *key, const char *val)
             return 1;
         }
     static const char *unsupported[] = {"latfile", "profile_file", "udv_scale_file", "udv_scale_diagnostics_file",
-                                        "udv_centered_file", "udv_centered_diagnostics_file", "replica_log"};
+                                        "udv_centered_file", "udv_centered_diagnostics_file"};
     for (size_t i = 0; i < sizeof unsupported / sizeof unsupported[0]; i++)
         if (strcmp(key, unsupported[i]) == 0) {
             fprintf(stderr, "ERROR: %s is not supported by the GPU engine\n", key);
@@ -225,6 +225,11 @@ int params_read(Params *p, const char *path)
             const int r = beta_list(p, val);
             if (r == 2) BAD("ERROR: beta_list has too many entries (max 64)\n");
             else if (r) BAD("ERROR: beta_list must contain positive numeric values\n");
+        } else if (!strcmp(key, "replica_log")) {
+            if (copy_str(p->replica_log, sizeof p->replica_log, val)) BAD("ERROR: invalid replica_log\n");
+        } else if (!strcmp(key, "spin_consistency_file")) {
+            if (copy_str(p->spin_consistency_file, sizeof p->spin_consistency_file, val))
+                BAD("ERROR: invalid spin_consistency_file\n");
         } else if (!strcmp(key, "sweep_order")) {
             if (copy_str(p->sweep_order, sizeof p->sweep_order, val)) BAD("ERROR: invalid sweep_order\n");
         } else if (!strcmp(key, "green_rebuild")) {

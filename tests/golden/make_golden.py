@@ -31,6 +31,10 @@ Fixtures (all numpy .npz, no pickles):
                     (nwarm=300 nmeas=3000 nbin=30 seed=1, szz_q=all, sperp_q=all).
   c3_observables.txt, c3_szz.txt, c3_sperp.txt (4x4, beta=3, stab=5, 8
                     replicas, sweep_order=alternating, green_rebuild=two_sided)
+                    and its replicas.dat / hopping_used.txt (c3_replicas.txt, c3_hopping.txt)
+  c4_*.txt          chain L=6, beta_list=2,3, 2 serial replicas: obs, szz, sperp,
+                    su2 (spin_consistency_file), drift (stab_drift_file), reps
+                    (replica_log) and hopping_used.txt
 """
 import os
 import subprocess
@@ -248,8 +252,29 @@ def c3_outputs():
         subprocess.run([pyoracle.REF_OMP, inp], cwd=d, check=True, stdout=subprocess.DEVNULL,
                        env=dict(os.environ, OMP_NUM_THREADS=str(os.cpu_count() or 1), OPENBLAS_NUM_THREADS="1"))
         for src, dst in (("observables.dat", "c3_observables.txt"), ("szz.dat", "c3_szz.txt"),
-                         ("sperp.dat", "c3_sperp.txt")):
+                         ("sperp.dat", "c3_sperp.txt"), ("replicas.dat", "c3_replicas.txt"),
+                         ("hopping_used.txt", "c3_hopping.txt")):
             with open(os.path.join(d, src)) as fi, open(os.path.join(HERE, dst), "w") as fo:
+                fo.write(fi.read())
+
+
+C4_INPUT = ("lattice=chain\nLx=6\nU=4\ndtau=0.1\nbeta_list=2,3\nstab=5\nnwarm=100\nnmeas=1000\nnbin=10\nseed=5\n"
+            "parallel=serial\nnrep=2\nszz_q=all\nsperp_q=all\nspin_consistency_file=su2.dat\n"
+            "stab_drift_file=drift.dat\nreplica_log=reps.dat\noutput_file=obs.dat\n")
+
+
+def c4_outputs():
+    """chain L=6, beta_list=2,3, 2 serial replicas with every optional output
+    of the driver: spin_consistency_file, stab_drift_file, replica_log."""
+    with tempfile.TemporaryDirectory() as d:
+        inp = os.path.join(d, "c4.in")
+        with open(inp, "w") as f:
+            f.write(C4_INPUT)
+        subprocess.run([pyoracle.REF_OMP, inp], cwd=d, check=True, stdout=subprocess.DEVNULL,
+                       env=dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1"))
+        for src in ("obs.dat", "szz.dat", "sperp.dat", "su2.dat", "drift.dat", "reps.dat", "hopping_used.txt"):
+            with open(os.path.join(d, src)) as fi, open(os.path.join(HERE, "c4_" + src.replace(".dat", ".txt")),
+                                                        "w") as fo:
                 fo.write(fi.read())
 
 
@@ -257,7 +282,7 @@ if __name__ == "__main__":
     if not pyoracle.ref_available():
         sys.exit("oracle/_ref not built: make -C oracle ref")
     gens = [traj_chain4, traj_sq8, traj_alt, model, udv_ops, seeds, ref_fixtures, ed_l4, c1_outputs, c2_outputs,
-            c3_outputs]
+            c3_outputs, c4_outputs]
     # optional names on the command line select generators (default: all)
     pick = set(sys.argv[1:])
     for g in gens:

@@ -10,7 +10,10 @@ C1: chain L=4, U=4, beta=4, 300+3000 sweeps, 30 bins, seed 1.
 C2: 8x8, U=4, beta=4, stab 10, 64 replicas (replica_seed(1, 0, r)),
     200+2000 sweeps, 20 bins.
 C3: 4x4, U=4, beta=3, stab 5, 8 replicas, sweep_order=alternating,
-    green_rebuild=two_sided, 200+2000 sweeps, 20 bins."""
+    green_rebuild=two_sided, 200+2000 sweeps, 20 bins; also replicas.dat
+    and hopping_used.txt.
+C4: chain L=6, beta_list=2,3, 2 serial replicas with the optional outputs
+    spin_consistency_file, stab_drift_file and replica_log."""
 import os
 import subprocess
 
@@ -71,3 +74,45 @@ def test_driver_matches_reference_outputs(case, tmp_path):
         assert s1.shape == s0.shape
         assert np.array_equal(s1[:, :10], s0[:, :10])  # beta, T, q index, mx, my, q/pi columns
         within(s1[:, 10], s1[:, 11], s0[:, 10], s0[:, 11], f)
+    if case == "c3":
+        # replicas.dat (written by default for nrep > 1) and the hopping dump
+        assert (tmp_path / "replicas.dat").read_text() == open(os.path.join(GOLD, "c3_replicas.txt")).read()
+        assert (tmp_path / "hopping_used.txt").read_text() == open(os.path.join(GOLD, "c3_hopping.txt")).read()
+
+
+C4 = ("lattice=chain\nLx=6\nU=4\ndtau=0.1\nbeta_list=2,3\nstab=5\nnwarm=100\nnmeas=1000\nnbin=10\nseed=5\n"
+      "parallel=serial\nnrep=2\nszz_q=all\nsperp_q=all\nspin_consistency_file=su2.dat\n"
+      "stab_drift_file=drift.dat\nreplica_log=reps.dat\noutput_file=obs.dat\n")
+
+
+def test_driver_optional_outputs(tmp_path):
+    # every optional output of the reference driver on a two-beta serial run
+    # (src/main.c:319-377, 453-472, 829-933, src/replica_run.c:58-88)
+    assert os.path.exists(EXE), "dqmc_gpu_main is not built"
+    (tmp_path / "in.txt").write_text(C4)
+    print("running dqmc_gpu_main on c4", flush=True)
+    subprocess.run([EXE, "in.txt"], cwd=tmp_path, check=True, timeout=600)
+    gold = lambda f: os.path.join(GOLD, f"c4_{f}.txt")
+    h0, r0 = split(gold("obs"))
+    h1, r1 = split(tmp_path / "obs.dat")
+    assert h1 == h0 and r1.shape == r0.shape == (2, 14)
+    assert np.array_equal(r1[:, 0], r0[:, 0])
+    for c, name in ((1, "E_hub"), (3, "E_gc"), (5, "E_ph"), (7, "ntot"), (9, "doublon"), (12, "acceptance")):
+        within(r1[:, c], r1[:, c + 1], r0[:, c], r0[:, c + 1], name)
+    for f in ("szz", "sperp", "su2"):
+        g0, s0 = split(gold(f))
+        g1, s1 = split(tmp_path / f"{f}.dat")
+        assert g1 == g0 and s1.shape == s0.shape
+        assert np.array_equal(s1[:, :10], s0[:, :10])
+        within(s1[:, 10], s1[:, 11], s0[:, 10], s0[:, 11], f)
+    assert (tmp_path / "reps.dat").read_text() == open(gold("reps")).read()
+    assert (tmp_path / "hopping_used.txt").read_text() == open(gold("hopping_used")).read()
+    # drift rows: identical identification columns and sample counts; the
+    # drift itself is measured against the stack's rebuilt G here and
+    # against a from-scratch G in the reference, both tiny at this size
+    g0, d0 = split(gold("drift"))
+    g1, d1 = split(tmp_path / "drift.dat")
+    assert g1 == g0 and d1.shape == d0.shape == (4, 14)
+    assert np.array_equal(d1[:, :9], d0[:, :9]) and np.array_equal(d1[:, 13], d0[:, 13])
+    assert (d1[:, 9] < 1e-8).all() and (d1[:, 10] <= d1[:, 9]).all()
+    assert ((d1[:, 11] >= 0) & (d1[:, 11] < d1[:, 1])).all() and (d1[:, 12] < 1100).all()

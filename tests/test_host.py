@@ -21,7 +21,8 @@ class Params(C.Structure):
                 ("output_file", C.c_char * 256), ("parallel", C.c_char * 16), ("nrep", C.c_int),
                 ("szz_q", C.c_char * 256), ("szz_file", C.c_char * 256), ("sperp_q", C.c_char * 256),
                 ("sperp_file", C.c_char * 256), ("stab_drift_file", C.c_char * 256),
-                ("seed", C.c_ulonglong), ("sweep_order", C.c_char * 16), ("green_rebuild", C.c_char * 16),
+                ("seed", C.c_ulonglong), ("replica_log", C.c_char * 256), ("spin_consistency_file", C.c_char * 256),
+                ("sweep_order", C.c_char * 16), ("green_rebuild", C.c_char * 16),
                 ("precision", C.c_char * 24), ("delay", C.c_int), ("device", C.c_int)]
 
 
@@ -125,6 +126,7 @@ def test_params_defaults_and_keys(host, tmp_path):
     assert p.dtau == 0.1 and p.thop == -1.0 and p.pbc == 1
     assert p.output_file == b"observables.dat" and p.sperp_q == b"none"
     assert p.sweep_order == b"forward" and p.green_rebuild == b"combine"
+    assert p.replica_log == b"" and p.spin_consistency_file == b"none"
 
 
 def test_params_sweep_order_and_rebuild(host, tmp_path):
