@@ -188,13 +188,21 @@ void orc_matmul(int n, const double *A, const double *B, double *C)
 void orc_geqrf(int n, double *A, double *tau)
 {
     for (int j = 0; j < n; j++) {
-        double xnorm2 = 0.0;
-        for (int i = j + 1; i < n; i++) xnorm2 += A[IDX(i, j, n)] * A[IDX(i, j, n)];
+        /* ||x|| scaled by max|x_i| so that entries near the double range
+           neither overflow nor underflow when squared (dnrm2, dlapy2) */
+        double scale = 0.0, ss = 0.0;
+        for (int i = j + 1; i < n; i++) scale = fmax(scale, fabs(A[IDX(i, j, n)]));
+        if (scale > 0.0)
+            for (int i = j + 1; i < n; i++) {
+                const double t = A[IDX(i, j, n)] / scale;
+                ss += t * t;
+            }
+        const double xnorm = scale * sqrt(ss);
         const double alpha = A[IDX(j, j, n)];
-        if (xnorm2 == 0.0) {
+        if (xnorm == 0.0) {
             tau[j] = 0.0;
         } else {
-            const double beta = -copysign(sqrt(alpha * alpha + xnorm2), alpha);
+            const double beta = -copysign(hypot(alpha, xnorm), alpha);
             tau[j] = (beta - alpha) / beta;
             const double sc = 1.0 / (alpha - beta);
             for (int i = j + 1; i < n; i++) A[IDX(i, j, n)] *= sc;
@@ -421,25 +429,51 @@ out:
     return rc;
 }
 
-/* udv_inv_one_plus_work, non-centered (src/linalg.c:911-1115):
+/* The big/small split of one D entry (src/linalg.c:966-986, 1190-1215):
+   without an offset, Db = D if |D| > 1 else 1 and Ds = D / Db; with a log
+   offset the true log-magnitude ell = log|D| + offset decides, and the
+   INVERSE of the big part is kept (Dbi = sgn D e^-ell) so that nothing
+   overflows.  Returns 0, or 3 (LINALG_FAILURE_EFFECTIVE_LOG_RANGE) when
+   -|ell| is below log(DBL_TRUE_MIN) (:166-182).  *big as the sign rule
+   (:1046-1052) reads it. */
+static int split_entry(double d, double off, double *dbi, double *ds, int *big)
+{
+    if (off == 0.0) {
+        *big = fabs(d) > 1.0;
+        *dbi = *big ? 1.0 / d : 1.0;
+        *ds = *big ? 1.0 : d;
+        return 0;
+    }
+    const double ell = log(fabs(d)) + off;
+    if (!isfinite(ell) || -fabs(ell) < log(4.9406564584124654e-324)) return 3;
+    *big = ell > 0.0;
+    *dbi = *big ? copysign(exp(-ell), d) : 1.0;
+    *ds = *big ? 1.0 : copysign(exp(ell), d);
+    return 0;
+}
+
+/* udv_inv_one_plus_work (src/linalg.c:911-1115), with the UDV's log
+   offset (0 = not centered):
    G = (1 + U D T)^-1 = T^-1 (Db^-1 U^T T^-1 + Ds)^-1 Db^-1 U^T,
-   sign = prod_{|D_i|>1} sgn D_i * sgn det U * sgn det M. */
-int orc_udv_inv_one_plus(int n, const double *U, const double *D,
-                         const double *T, double *g, int *sign)
+   sign = prod_{big} sgn D_i * sgn det U * sgn det M. */
+int orc_udv_inv_one_plus_off(int n, const double *U, const double *D,
+                             const double *T, double off, double *g,
+                             int *sign)
 {
     const size_t nn = (size_t)n * (size_t)n;
-    double *Db = malloc(sizeof(double) * (size_t)n);
+    double *Dbi = malloc(sizeof(double) * (size_t)n);
     double *Ds = malloc(sizeof(double) * (size_t)n);
+    int *big = malloc(sizeof(int) * (size_t)n);
     double *Ti = malloc(sizeof(double) * nn), *M = malloc(sizeof(double) * nn);
     double *Mi = malloc(sizeof(double) * nn), *W = malloc(sizeof(double) * nn);
     double *W2 = malloc(sizeof(double) * nn);
     int rc = 1;
     if (sign) *sign = 0;
-    if (!all_finite(nn, U) || !all_finite((size_t)n, D) || !all_finite(nn, T)) goto out;
+    if (!all_finite(nn, U) || !all_finite((size_t)n, D) || !all_finite(nn, T) || !isfinite(off)) goto out;
     for (int i = 0; i < n; i++) {
-        if (fabs(D[i]) > 1.0) { Db[i] = D[i]; Ds[i] = 1.0; }
-        else { Db[i] = 1.0; Ds[i] = D[i]; }
-        if (Db[i] == 0.0 || Ds[i] == 0.0) goto out;
+        const int e = split_entry(D[i], off, &Dbi[i], &Ds[i], &big[i]);
+        if (e) { rc = e; goto out; }
+        if (!isfinite(Dbi[i]) || !isfinite(Ds[i]) || Dbi[i] == 0.0 || Ds[i] == 0.0) goto out;
     }
     memcpy(Ti, T, sizeof(double) * nn);
     orc_trtri_unit_upper(n, Ti);
@@ -448,7 +482,7 @@ int orc_udv_inv_one_plus(int n, const double *U, const double *D,
         for (int i = 0; i < n; i++) {
             double acc = 0.0;
             for (int k = 0; k < n; k++) acc += U[IDX(k, i, n)] * Ti[IDX(k, j, n)];
-            M[IDX(i, j, n)] = acc / Db[i];
+            M[IDX(i, j, n)] = off != 0.0 ? acc * Dbi[i] : (big[i] ? acc / D[i] : acc);
         }
     for (int i = 0; i < n; i++) M[IDX(i, i, n)] += Ds[i];
     if (!all_finite(nn, M)) goto out;
@@ -456,7 +490,7 @@ int orc_udv_inv_one_plus(int n, const double *U, const double *D,
     {
         int s = 1, ok1, ok2;
         for (int i = 0; i < n; i++)
-            if (fabs(D[i]) > 1.0 && D[i] < 0.0) s = -s;
+            if (big[i] && D[i] < 0.0) s = -s;
         memcpy(W, U, sizeof(double) * nn);
         const int su = det_sign_lu(n, W, &ok1);
         memcpy(W, M, sizeof(double) * nn);
@@ -464,19 +498,206 @@ int orc_udv_inv_one_plus(int n, const double *U, const double *D,
         if (!ok1 || !ok2) goto out;
         if (sign) *sign = s * su * sm;
     }
-    /* W = (Mi Db^-1) U^T ; g = Ti W */
+    /* W2 = (Mi Db^-1) U^T ; g = Ti W2 */
     for (int j = 0; j < n; j++)
         for (int i = 0; i < n; i++) {
             double acc = 0.0;
-            for (int k = 0; k < n; k++) acc += (Mi[IDX(i, k, n)] / Db[k]) * U[IDX(j, k, n)];
+            for (int k = 0; k < n; k++) {
+                const double mk = off != 0.0 ? Mi[IDX(i, k, n)] * Dbi[k]
+                                             : (big[k] ? Mi[IDX(i, k, n)] / D[k] : Mi[IDX(i, k, n)]);
+                acc += mk * U[IDX(j, k, n)];
+            }
             W2[IDX(i, j, n)] = acc;
         }
     orc_matmul(n, Ti, W2, g);
     rc = all_finite(nn, g) ? 0 : 1;
 out:
     if (rc && sign) *sign = 0;
-    free(Db); free(Ds); free(Ti); free(M); free(Mi); free(W); free(W2);
+    free(Dbi); free(Ds); free(big); free(Ti); free(M); free(Mi); free(W); free(W2);
     return rc;
+}
+
+int orc_udv_inv_one_plus(int n, const double *U, const double *D,
+                         const double *T, double *g, int *sign)
+{
+    return orc_udv_inv_one_plus_off(n, U, D, T, 0.0, g, sign);
+}
+
+/* udv_inv_one_plus_two_sided_work (src/linalg.c:1117-1389):
+   1 + L R with L = Ul Dl Tl, R = Ur Dr Tr (log offsets offl, offr; 0 = not
+   centered) as Ul Dlb H Drb Tr,
+   H = Dlb^-1 (Ul^T Tr^-1) Drb^-1 + Dls (Tl Ur) Drs,
+   G = Tr^-1 Drb^-1 H^-1 Dlb^-1 Ul^T,
+   sign = prod_{big} sgn Dl * prod_{big} sgn Dr * sgn det Ul * sgn det H. */
+int orc_udv_inv_two_sided(int n, const double *Ul, const double *Dl,
+                          const double *Tl, double offl, const double *Ur,
+                          const double *Dr, const double *Tr, double offr,
+                          double *g, int *sign)
+{
+    const size_t nn = (size_t)n * (size_t)n;
+    double *lbi = malloc(sizeof(double) * (size_t)n), *ls = malloc(sizeof(double) * (size_t)n);
+    double *rbi = malloc(sizeof(double) * (size_t)n), *rs = malloc(sizeof(double) * (size_t)n);
+    int *lb = malloc(sizeof(int) * (size_t)n), *rb = malloc(sizeof(int) * (size_t)n);
+    double *TLUR = malloc(sizeof(double) * nn), *Ti = malloc(sizeof(double) * nn);
+    double *H = malloc(sizeof(double) * nn), *Hi = malloc(sizeof(double) * nn);
+    double *W = malloc(sizeof(double) * nn), *W2 = malloc(sizeof(double) * nn);
+    int rc = 1;
+    if (sign) *sign = 0;
+    if (!all_finite(nn, Ul) || !all_finite((size_t)n, Dl) || !all_finite(nn, Tl) || !all_finite(nn, Ur) ||
+        !all_finite((size_t)n, Dr) || !all_finite(nn, Tr) || !isfinite(offl) || !isfinite(offr))
+        goto out;
+    for (int i = 0; i < n; i++) {
+        int e = split_entry(Dl[i], offl, &lbi[i], &ls[i], &lb[i]);
+        if (!e) e = split_entry(Dr[i], offr, &rbi[i], &rs[i], &rb[i]);
+        if (e) { rc = e; goto out; }
+        if (lbi[i] == 0.0 || rbi[i] == 0.0 || ls[i] == 0.0 || rs[i] == 0.0) goto out;
+    }
+    orc_matmul(n, Tl, Ur, TLUR);
+    memcpy(Ti, Tr, sizeof(double) * nn);
+    orc_trtri_unit_upper(n, Ti);
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+            double acc = 0.0; /* (Ul^T Tr^-1)(i, j) */
+            for (int k = 0; k < n; k++) acc += Ul[IDX(k, i, n)] * Ti[IDX(k, j, n)];
+            double large = offl != 0.0 ? acc * lbi[i] : (lb[i] ? acc / Dl[i] : acc);
+            large = offr != 0.0 ? large * rbi[j] : (rb[j] ? large / Dr[j] : large);
+            H[IDX(i, j, n)] = large + ls[i] * TLUR[IDX(i, j, n)] * rs[j];
+        }
+    if (!all_finite(nn, H)) goto out;
+    if (orc_inverse(n, H, Hi) != 0 || !all_finite(nn, Hi)) goto out;
+    {
+        int s = 1, ok1, ok2;
+        for (int i = 0; i < n; i++) {
+            if (lb[i] && Dl[i] < 0.0) s = -s;
+            if (rb[i] && Dr[i] < 0.0) s = -s;
+        }
+        memcpy(W, Ul, sizeof(double) * nn);
+        const int su = det_sign_lu(n, W, &ok1);
+        memcpy(W, H, sizeof(double) * nn);
+        const int sh = det_sign_lu(n, W, &ok2);
+        if (!ok1 || !ok2) goto out;
+        if (sign) *sign = s * su * sh;
+    }
+    /* W2 = (Hi Dlb^-1) Ul^T, rows scaled by Drb^-1; g = Tr^-1 W2 */
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+            double acc = 0.0;
+            for (int k = 0; k < n; k++) {
+                const double hk = offl != 0.0 ? Hi[IDX(i, k, n)] * lbi[k]
+                                              : (lb[k] ? Hi[IDX(i, k, n)] / Dl[k] : Hi[IDX(i, k, n)]);
+                acc += hk * Ul[IDX(j, k, n)];
+            }
+            W2[IDX(i, j, n)] = offr != 0.0 ? acc * rbi[i] : (rb[i] ? acc / Dr[i] : acc);
+        }
+    orc_matmul(n, Ti, W2, g);
+    rc = all_finite(nn, g) ? 0 : 1;
+out:
+    if (rc && sign) *sign = 0;
+    free(lbi); free(ls); free(rbi); free(rs); free(lb); free(rb);
+    free(TLUR); free(Ti); free(H); free(Hi); free(W); free(W2);
+    return rc;
+}
+
+/* udv_recenter (src/linalg.c:453-513) with the hard margin 8 of the
+   centered updates: D /= e^c, offset += c for the centre c of the log|D|
+   range; 1 (nothing changed) when the radius leaves less than the margin
+   below log(DBL_MAX) or a rescaled entry is not representable. */
+int orc_udv_recenter(int n, double *D, double *off)
+{
+    double lo = 0.0, hi = 0.0;
+    if (!isfinite(*off)) return 1;
+    for (int i = 0; i < n; i++) {
+        if (!isfinite(D[i]) || D[i] == 0.0) return 1;
+        const double ld = log(fabs(D[i]));
+        if (i == 0 || ld < lo) lo = ld;
+        if (i == 0 || ld > hi) hi = ld;
+    }
+    const double c = lo + 0.5 * (hi - lo), r = 0.5 * (hi - lo);
+    const double margin = log(1.7976931348623157e308) - r;
+    if (!isfinite(c) || !isfinite(r) || !isfinite(margin) || margin < 8.0 || !isfinite(*off + c)) return 1;
+    for (int i = 0; i < n; i++) {
+        const double v = exp(log(fabs(D[i])) - c);
+        if (!isfinite(v) || v == 0.0) return 1;
+    }
+    for (int i = 0; i < n; i++) D[i] = copysign(exp(log(fabs(D[i])) - c), D[i]);
+    *off += c;
+    return 0;
+}
+
+static double logaddexp2(double a, double b)
+{
+    if (a == -INFINITY) return b;
+    if (b == -INFINITY) return a;
+    const double hi = a > b ? a : b, lo = a > b ? b : a;
+    return hi + log1p(exp(lo - hi));
+}
+
+/* centered_update_has_margin (src/linalg.c:119-164): a bound on the
+   largest QR input entry of the update, below log(DBL_MAX) - 8. */
+static int update_has_margin(int n, const double *D, const double *T, const double *B, int left)
+{
+    const double limit = log(1.7976931348623157e308) - 8.0;
+    double worst = -INFINITY;
+    if (left) {
+        double lb = -INFINITY;
+        for (int i = 0; i < n; i++) {
+            double row = -INFINITY;
+            for (int k = 0; k < n; k++)
+                if (B[IDX(i, k, n)] != 0.0) row = logaddexp2(row, log(fabs(B[IDX(i, k, n)])));
+            if (row > lb) lb = row;
+        }
+        for (int j = 0; j < n; j++) {
+            const double c = log(fabs(D[j])) + lb;
+            if (c > worst) worst = c;
+        }
+    } else {
+        for (int i = 0; i < n; i++) {
+            double row = -INFINITY;
+            for (int k = 0; k < n; k++) {
+                double bmax = 0.0;
+                for (int j = 0; j < n; j++) bmax = fmax(bmax, fabs(B[IDX(k, j, n)]));
+                const double t = fabs(T[IDX(i, k, n)]);
+                if (t != 0.0 && bmax != 0.0) row = logaddexp2(row, log(t) + log(bmax));
+            }
+            const double c = log(fabs(D[i])) + row;
+            if (c > worst) worst = c;
+        }
+    }
+    return isfinite(worst) && worst <= limit;
+}
+
+/* udv_lmul_centered_work / udv_rmul_centered (src/linalg.c:515-791,
+   centered = 1): recentre, check the update's margin (inputs restored on
+   failure), update, recentre.  0 ok, 1 LINALG_FAILURE_CENTERED_RADIUS,
+   2 LINALG_FAILURE_CENTERED_UPDATE_MARGIN, 4 other failure. */
+static int udv_mul_centered(int n, double *U, double *D, double *T, double *off, const double *B, int left)
+{
+    double *D0 = malloc(sizeof(double) * (size_t)n);
+    const double off0 = *off;
+    int rc = 0;
+    memcpy(D0, D, sizeof(double) * (size_t)n);
+    if (orc_udv_recenter(n, D, off)) { rc = 1; goto out; }
+    if (!update_has_margin(n, D, T, B, left)) {
+        memcpy(D, D0, sizeof(double) * (size_t)n);
+        *off = off0;
+        rc = 2;
+        goto out;
+    }
+    if (left ? orc_udv_lmul(n, U, D, T, B) : orc_udv_rmul(n, U, D, T, B)) { rc = 4; goto out; }
+    if (orc_udv_recenter(n, D, off)) rc = 1;
+out:
+    free(D0);
+    return rc;
+}
+
+int orc_udv_lmul_centered(int n, double *U, double *D, double *T, double *off, const double *B)
+{
+    return udv_mul_centered(n, U, D, T, off, B, 1);
+}
+
+int orc_udv_rmul_centered(int n, double *U, double *D, double *T, double *off, const double *B)
+{
+    return udv_mul_centered(n, U, D, T, off, B, 0);
 }
 
 /* ------------------------------------------------------ B and the wrap */
@@ -667,6 +888,7 @@ void orc_jackknife(const double *x, int N, double *mean, double *err)
 
 typedef struct {
     double *U, *D, *T;
+    double off; /* log offset of a centered UDV (0 otherwise) */
 } Udv;
 
 struct orc_replica {
@@ -684,6 +906,8 @@ struct orc_replica {
     Udv *S, left, comb;
     Udv *A;             /* prefix stack of the alternating mode (M + 1) */
     int alternating, dir, prefix_ok, suffix_ok;
+    int rebuild;        /* 0 combine, 1 two_sided, 2 centered (src/green.h:10-14) */
+    int reason;         /* LINALG_FAILURE_* of a failed replica */
 };
 
 static void udv_alloc(Udv *u, int n)
@@ -699,6 +923,7 @@ static void udv_eye(Udv *u, int n)
     memset(u->U, 0, sizeof(double) * (size_t)n * n);
     memset(u->T, 0, sizeof(double) * (size_t)n * n);
     for (int i = 0; i < n; i++) { u->U[IDX(i, i, n)] = 1.0; u->T[IDX(i, i, n)] = 1.0; u->D[i] = 1.0; }
+    u->off = 0.0;
 }
 
 static void udv_cp(Udv *d, const Udv *s, int n)
@@ -706,6 +931,28 @@ static void udv_cp(Udv *d, const Udv *s, int n)
     memcpy(d->U, s->U, sizeof(double) * (size_t)n * n);
     memcpy(d->T, s->T, sizeof(double) * (size_t)n * n);
     memcpy(d->D, s->D, sizeof(double) * (size_t)n);
+    d->off = s->off;
+}
+
+/* green_udv_lmul / green_udv_rmul / dqmc_udv_lmul / dqmc_udv_rmul
+   (src/green.c:22-38, src/dqmc.c:37-53): the centered variants in the
+   centered rebuild mode.  0 ok; on failure the replica's reason is set. */
+static int rep_lmul(orc_replica *R, Udv *u, const double *B)
+{
+    int rc;
+    if (R->rebuild == 2) rc = orc_udv_lmul_centered(R->n, u->U, u->D, u->T, &u->off, B);
+    else rc = orc_udv_lmul(R->n, u->U, u->D, u->T, B) ? 4 : 0;
+    if (rc && rc < 4) R->reason = rc;
+    return rc;
+}
+
+static int rep_rmul(orc_replica *R, Udv *u, const double *B)
+{
+    int rc;
+    if (R->rebuild == 2) rc = orc_udv_rmul_centered(R->n, u->U, u->D, u->T, &u->off, B);
+    else rc = orc_udv_rmul(R->n, u->U, u->D, u->T, B) ? 4 : 0;
+    if (rc && rc < 4) R->reason = rc;
+    return rc;
 }
 
 static void udv_release(Udv *u)
@@ -725,9 +972,12 @@ static int from_scratch(orc_replica *R, int l0, double *out, int *sgn)
     for (int k = 0; k < R->L && rc == 0; k++) {
         const int l = (l0 + k) % R->L;
         orc_build_B(n, R->expK, R->expv, R->s + (size_t)l * n, B);
-        rc = orc_udv_lmul(n, u.U, u.D, u.T, B);
+        rc = rep_lmul(R, &u, B);
     }
-    if (rc == 0) rc = orc_udv_inv_one_plus(n, u.U, u.D, u.T, out, sgn);
+    if (rc == 0) {
+        rc = orc_udv_inv_one_plus_off(n, u.U, u.D, u.T, u.off, out, sgn);
+        if (rc == 3) R->reason = 3;
+    }
     free(B);
     udv_release(&u);
     return rc;
@@ -879,15 +1129,22 @@ static void metro_slice(orc_replica *R, int l)
     delay_flush(R);
 }
 
-/* green_from_boundary_factors in combine mode (src/green.c:308-345) followed
-   by the PH map and the sign reset (src/dqmc.c:731-738, 843-846). */
+/* green_from_boundary_factors (src/green.c:308-345): combine +
+   inv_one_plus, or the two-sided inverse in the two_sided and centered
+   modes; then the PH map and the sign reset (src/dqmc.c:731-738, 843-846). */
 static int rebuild_from_factors(orc_replica *R, const Udv *prefix, const Udv *suffix)
 {
     const int n = R->n;
-    int sg = 0;
-    if (orc_udv_combine(n, prefix->U, prefix->D, prefix->T, suffix->U, suffix->D, suffix->T, R->comb.U,
-                        R->comb.D, R->comb.T) ||
-        orc_udv_inv_one_plus(n, R->comb.U, R->comb.D, R->comb.T, R->g, &sg) || sg == 0) {
+    int sg = 0, rc;
+    if (R->rebuild == 0)
+        rc = orc_udv_combine(n, prefix->U, prefix->D, prefix->T, suffix->U, suffix->D, suffix->T, R->comb.U,
+                             R->comb.D, R->comb.T) ||
+             orc_udv_inv_one_plus(n, R->comb.U, R->comb.D, R->comb.T, R->g, &sg);
+    else
+        rc = orc_udv_inv_two_sided(n, prefix->U, prefix->D, prefix->T, prefix->off, suffix->U, suffix->D,
+                                   suffix->T, suffix->off, R->g, &sg);
+    if (rc == 3) R->reason = 3;
+    if (rc || sg == 0) {
         R->status = 1;
         R->det_sign = 0;
         return 1;
@@ -902,7 +1159,9 @@ static int rebuild_from_factors(orc_replica *R, const Udv *prefix, const Udv *su
 static int rebuild_from_left(orc_replica *R, const Udv *u)
 {
     int sg = 0;
-    if (orc_udv_inv_one_plus(R->n, u->U, u->D, u->T, R->g, &sg) || sg == 0) {
+    const int rc = orc_udv_inv_one_plus_off(R->n, u->U, u->D, u->T, u->off, R->g, &sg);
+    if (rc == 3) R->reason = 3;
+    if (rc || sg == 0) {
         R->status = 1;
         R->det_sign = 0;
         return 1;
@@ -930,7 +1189,7 @@ static int sweep_forward(orc_replica *R, int carry)
         for (int j = M - 1; j >= 1; j--) {
             orc_build_Bblock(n, R->expK, R->expv, R->s, R->b[j], R->b[j + 1] - R->b[j], Bb);
             udv_cp(&R->S[j], &R->S[j + 1], n);
-            if (orc_udv_rmul(n, R->S[j].U, R->S[j].D, R->S[j].T, Bb)) { R->status = 1; goto done; }
+            if (rep_rmul(R, &R->S[j], Bb)) { R->status = 1; goto done; }
         }
     }
     udv_eye(&R->left, n);
@@ -943,7 +1202,7 @@ static int sweep_forward(orc_replica *R, int carry)
             const int tau = l + 1, j = tau / stab, len = tau - tau_left;
             if (record_drift(R, tau)) { R->status = 1; goto done; }
             orc_build_Bblock(n, R->expK, R->expv, R->s, tau_left, len, Bb);
-            if (orc_udv_lmul(n, R->left.U, R->left.D, R->left.T, Bb)) { R->status = 1; goto done; }
+            if (rep_lmul(R, &R->left, Bb)) { R->status = 1; goto done; }
             tau_left = tau;
             if (carry) udv_cp(&R->A[j], &R->left, n);
             if (rebuild_from_factors(R, &R->left, &R->S[j])) goto done;
@@ -954,7 +1213,7 @@ static int sweep_forward(orc_replica *R, int carry)
         if (record_drift(R, 0)) { R->status = 1; goto done; }
         if (len > 0) {
             orc_build_Bblock(n, R->expK, R->expv, R->s, tau_left, len, Bb);
-            if (orc_udv_lmul(n, R->left.U, R->left.D, R->left.T, Bb)) { R->status = 1; goto done; }
+            if (rep_lmul(R, &R->left, Bb)) { R->status = 1; goto done; }
         }
         if (carry) udv_cp(&R->A[M], &R->left, n);
         if (rebuild_from_left(R, &R->left)) goto done;
@@ -987,7 +1246,7 @@ static int sweep_backward(orc_replica *R)
         for (int j = 1; j <= M; j++) {
             orc_build_Bblock(n, R->expK, R->expv, R->s, R->b[j - 1], R->b[j] - R->b[j - 1], Bb);
             udv_cp(&R->A[j], &R->A[j - 1], n);
-            if (orc_udv_lmul(n, R->A[j].U, R->A[j].D, R->A[j].T, Bb)) { R->status = 1; goto done; }
+            if (rep_lmul(R, &R->A[j], Bb)) { R->status = 1; goto done; }
         }
     }
     udv_eye(&R->left, n);
@@ -1001,7 +1260,7 @@ static int sweep_backward(orc_replica *R)
             const int j = l / stab;
             if (record_drift(R, l)) { R->status = 1; goto done; }
             orc_build_Bblock(n, R->expK, R->expv, R->s, R->b[j], R->b[j + 1] - R->b[j], Bb);
-            if (orc_udv_rmul(n, R->left.U, R->left.D, R->left.T, Bb)) { R->status = 1; goto done; }
+            if (rep_rmul(R, &R->left, Bb)) { R->status = 1; goto done; }
             udv_cp(&R->S[j], &R->left, n);
             if (rebuild_from_factors(R, &R->A[j], &R->left)) goto done;
         }
@@ -1012,7 +1271,7 @@ static int sweep_backward(orc_replica *R)
         if (record_drift(R, 0)) { R->status = 1; goto done; }
         if (len > 0) {
             orc_build_Bblock(n, R->expK, R->expv, R->s, 0, len, Bb);
-            if (orc_udv_rmul(n, R->left.U, R->left.D, R->left.T, Bb)) { R->status = 1; goto done; }
+            if (rep_rmul(R, &R->left, Bb)) { R->status = 1; goto done; }
         }
         if (rebuild_from_left(R, &R->left)) goto done;
     }
@@ -1033,6 +1292,19 @@ int orc_replica_sweep(orc_replica *R)
     if (!R->alternating) return sweep_forward(R, 0);
     return R->dir ? sweep_backward(R) : sweep_forward(R, 1);
 }
+
+/* green_rebuild = combine (0), two_sided (1) or centered (2)
+   (src/replica_run.c:37-45, src/green.c:116-119); the initial G is rebuilt
+   in the new mode as dqmc_init_modes does (src/dqmc.c:464-466). */
+int orc_replica_set_rebuild(orc_replica *R, int mode)
+{
+    R->rebuild = mode;
+    R->status = 0;
+    R->reason = 0;
+    return orc_replica_rebuild(R);
+}
+
+int orc_replica_reason(const orc_replica *R) { return R->reason; }
 
 /* sweep_order=alternating (src/dqmc.c:431-462): allocate the prefix stack;
    the next sweep is forward and rebuilds both stacks as it needs them. */

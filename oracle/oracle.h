@@ -57,6 +57,18 @@ int orc_udv_combine(int n, const double *Ul, const double *Dl,
                     const double *Tr, double *Uo, double *Do, double *To);
 int orc_udv_inv_one_plus(int n, const double *U, const double *D,
                          const double *T, double *g, int *sign);
+int orc_udv_inv_one_plus_off(int n, const double *U, const double *D,
+                             const double *T, double off, double *g,
+                             int *sign);           /* src/linalg.c:911-1115 */
+int orc_udv_inv_two_sided(int n, const double *Ul, const double *Dl,
+                          const double *Tl, double offl, const double *Ur,
+                          const double *Dr, const double *Tr, double offr,
+                          double *g, int *sign);   /* src/linalg.c:1117-1389 */
+int orc_udv_recenter(int n, double *D, double *off); /* src/linalg.c:453-513 */
+int orc_udv_lmul_centered(int n, double *U, double *D, double *T, double *off,
+                          const double *B);        /* src/linalg.c:515-653 */
+int orc_udv_rmul_centered(int n, double *U, double *D, double *T, double *off,
+                          const double *B);        /* src/linalg.c:662-794 */
 
 /* ---- B matrices and wrap (src/green.c) ---- */
 /* expv[0] for s=-1, expv[1] for s=+1 (src/green.c:54-67, sigma=+1). */
@@ -98,6 +110,8 @@ orc_replica *orc_replica_new(int n, int L, int stab, const double *thop,
 int orc_replica_status(const orc_replica *R);
 int orc_replica_sweep(orc_replica *R);
 void orc_replica_set_alternating(orc_replica *R, int on);
+int orc_replica_set_rebuild(orc_replica *R, int mode);
+int orc_replica_reason(const orc_replica *R);
 void orc_replica_green(const orc_replica *R, double *gu, double *gd);
 void orc_replica_field(const orc_replica *R, signed char *s);
 void orc_replica_set_field(orc_replica *R, const signed char *s);

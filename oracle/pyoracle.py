@@ -24,6 +24,7 @@ def _ptr(a):
 
 
 _orc = None
+REBUILD = {"combine": 0, "two_sided": 1, "centered": 2}
 
 
 def orc():
@@ -62,6 +63,13 @@ def orc():
             "orc_replica_status": (_I, [_P]),
             "orc_replica_sweep": (_I, [_P]),
             "orc_replica_set_alternating": (None, [_P, _I]),
+            "orc_replica_set_rebuild": (_I, [_P, _I]),
+            "orc_replica_reason": (_I, [_P]),
+            "orc_udv_inv_one_plus_off": (_I, [_I, _P, _P, _P, _D, _P, _P]),
+            "orc_udv_inv_two_sided": (_I, [_I, _P, _P, _P, _D, _P, _P, _P, _D, _P, _P]),
+            "orc_udv_recenter": (_I, [_I, _P, _P]),
+            "orc_udv_lmul_centered": (_I, [_I, _P, _P, _P, _P, _P]),
+            "orc_udv_rmul_centered": (_I, [_I, _P, _P, _P, _P, _P]),
             "orc_replica_green": (None, [_P, _P, _P]),
             "orc_replica_field": (None, [_P, _P]),
             "orc_replica_set_field": (None, [_P, _P]),
@@ -102,7 +110,7 @@ class Replica:
     forward/backward sweeps with carried stacks)."""
 
     def __init__(self, kind, Lx, Ly, U, dtau, L, stab, seed, delay_cap=16, drift=False, t=-1.0,
-                 alternating=False):
+                 alternating=False, rebuild="combine"):
         th, bp, _ = lattice(kind, Lx, Ly, t)
         self.n = th.shape[0]
         self.L = L
@@ -110,6 +118,8 @@ class Replica:
         self.U = U
         self.h = C.c_void_p(orc().orc_replica_new(self.n, L, stab, _ptr(np.asfortranarray(th)), _ptr(bp), U, dtau,
                                                   seed, delay_cap, 1 if drift else 0))
+        if rebuild != "combine":
+            orc().orc_replica_set_rebuild(self.h, REBUILD[rebuild])
         if alternating:
             orc().orc_replica_set_alternating(self.h, 1)
 
@@ -236,12 +246,12 @@ class RefReplica:
     """The compiled reference engine for one replica (oracle/ref_trace.c)."""
 
     def __init__(self, kind, Lx, Ly, U, dtau, L, stab, seed, drift=False, t=-1.0, alternating=False,
-                 two_sided=False):
+                 rebuild="combine"):
         self.n = Lx * (Ly if kind == "square" else 1)
         self.L = L
         self.h = C.c_void_p(ref().ref_replica_new_mode(0 if kind == "chain" else 1, Lx, Ly, 1, t, U, dtau, L, stab,
                                                        seed, 1 if drift else 0, 1 if alternating else 0,
-                                                       1 if two_sided else 0))
+                                                       REBUILD[rebuild]))
 
     def __del__(self):
         if getattr(self, "h", None):

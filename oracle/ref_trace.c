@@ -36,11 +36,12 @@ typedef struct {
 
 /* lattice_type: 0 = chain (src/lattice.c:26-44), 1 = square (:46-82).
    alternating selects DQMC_SWEEP_ALTERNATING (src/dqmc.h:45-48),
-   two_sided GREEN_REBUILD_TWO_SIDED (src/green.h:10-14). */
+   rebuild the GreenRebuildMode (src/green.h:10-14: 0 combine, 1 two_sided,
+   2 centered). */
 void *ref_replica_new_mode(int lattice_type, int Lx, int Ly, int pbc,
                            double thop, double U, double dtau, int Ltr,
                            int stab, unsigned long long seed, int drift,
-                           int alternating, int two_sided)
+                           int alternating, int rebuild)
 {
     RefReplica *R = calloc(1, sizeof(*R));
     if (R == NULL) return NULL;
@@ -56,7 +57,7 @@ void *ref_replica_new_mode(int lattice_type, int Lx, int Ly, int pbc,
     R->init_rc = dqmc_init_modes(
         &R->D, &R->m, &R->f, &R->r, stab, NULL,
         alternating ? DQMC_SWEEP_ALTERNATING : DQMC_SWEEP_FORWARD,
-        two_sided ? GREEN_REBUILD_TWO_SIDED : GREEN_REBUILD_COMBINE);
+        (GreenRebuildMode)rebuild);
     if (drift) {
         dqmc_enable_stab_drift(&R->D, 1);
     }

@@ -15,6 +15,9 @@ Fixtures (all numpy .npz, no pickles):
                     chain L=6 Ltr=20 stab=7 seed=5 and 4x4 Ltr=24 stab=5
                     seed=3 (both with a shorter last block), U=4, dtau=0.1:
                     6 sweeps of field, accept counts, sign and G_up.
+  traj_modes.npz    green_rebuild=two_sided / centered trajectories, 4x4: U=8
+                    L=80 stab=8 (two_sided; centered; centered alternating)
+                    and U=16 L=400 stab=10 (centered, D beyond 1e300).
   model.npz         expK, expKinv, lambda of the reference's model_init for
                     the 4-site chain and the 4x4 square lattice.
   udv_ops_n16.npz   seeded random UDV inputs and the reference's udv_lmul,
@@ -111,6 +114,31 @@ def traj_alt():
         out[f"sign_{c}"] = np.array(sg)
         out[f"green_{c}"] = np.array(gs)
     np.savez_compressed(os.path.join(HERE, "traj_alt.npz"), **out)
+
+
+MODE_CASES = (("two_sided", False, 8.0, 80, 8, 3, 4), ("centered", False, 8.0, 80, 8, 3, 4),
+              ("centered", True, 8.0, 80, 8, 4, 4), ("centered", False, 16.0, 400, 10, 3, 2))
+
+
+def traj_modes():
+    """green_rebuild=two_sided and centered trajectories of the compiled
+    reference, 4x4, dtau=0.1: fields, accept counts, sign and G per sweep."""
+    out = {}
+    for c, (mode, alt, U, L, stab, seed, nsw) in enumerate(MODE_CASES):
+        r = pyoracle.RefReplica("square", 4, 4, U, 0.1, L, stab, seed, alternating=alt, rebuild=mode)
+        fs, acc, sg, gs = [], [], [], []
+        for _ in range(nsw):
+            assert r.sweep() == 0
+            fs.append(r.field())
+            st = r.stats()
+            acc.append(st[0])
+            sg.append(st[2])
+            gs.append(r.green()[0])
+        out[f"fields_{c}"] = np.array(fs)
+        out[f"accepted_{c}"] = np.array(acc, dtype=np.uint64)
+        out[f"sign_{c}"] = np.array(sg)
+        out[f"green_{c}"] = np.array(gs)
+    np.savez_compressed(os.path.join(HERE, "traj_modes.npz"), **out)
 
 
 def model():
@@ -281,7 +309,7 @@ def c4_outputs():
 if __name__ == "__main__":
     if not pyoracle.ref_available():
         sys.exit("oracle/_ref not built: make -C oracle ref")
-    gens = [traj_chain4, traj_sq8, traj_alt, model, udv_ops, seeds, ref_fixtures, ed_l4, c1_outputs, c2_outputs,
+    gens = [traj_chain4, traj_sq8, traj_alt, traj_modes, model, udv_ops, seeds, ref_fixtures, ed_l4, c1_outputs, c2_outputs,
             c3_outputs, c4_outputs]
     # optional names on the command line select generators (default: all)
     pick = set(sys.argv[1:])

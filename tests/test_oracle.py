@@ -252,6 +252,85 @@ def test_alternating_green_is_the_rebuilt_green(orc):
     assert not np.array_equal(f.field(), r.field())
 
 
+MODE_CASES = (("two_sided", False, 8.0, 80, 8, 3, 4), ("centered", False, 8.0, 80, 8, 3, 4),
+              ("centered", True, 8.0, 80, 8, 4, 4), ("centered", False, 16.0, 400, 10, 3, 2))
+
+
+def test_rebuild_modes_match_reference(orc):
+    # green_rebuild=two_sided and centered (src/green.c:308-345,
+    # src/linalg.c:453-794, 1117-1389) against the compiled reference's
+    # trajectories; the last case (U=16, beta=40) carries D beyond 1e300
+    g = gold("traj_modes.npz")
+    for c, (mode, alt, U, L, stab, seed, nsw) in enumerate(MODE_CASES):
+        r = orc.Replica("square", 4, 4, U, 0.1, L, stab, seed, alternating=alt, rebuild=mode)
+        for s in range(nsw):
+            assert r.sweep() == 0
+            assert np.array_equal(r.field(), g[f"fields_{c}"][s]), (c, s)
+            acc, _, sign, _ = r.stats()
+            assert acc == int(g[f"accepted_{c}"][s]) and sign == g[f"sign_{c}"][s]
+            ref = g[f"green_{c}"][s]
+            assert np.abs(r.green()[0] - ref).max() / np.abs(ref).max() < 1e-9, (c, s)
+
+
+def _udv_args(orc, U, D, T):
+    return orc._ptr(U), orc._ptr(D), orc._ptr(T)
+
+
+def test_log_offset_is_a_scale_of_d(orc):
+    # tests/test_udv.c:173-196: a UDV whose D is scaled by e^-2 with
+    # log_offset 2 is the same matrix: same G and sign
+    O = orc.orc()
+    rng = np.random.default_rng(11)
+    n = 3
+    U, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    U = np.asfortranarray(U)
+    T = np.asfortranarray(np.triu(rng.standard_normal((n, n)), 1) + np.eye(n))
+    D = np.array([-3.0, 0.5, 2.0])
+    g0, g1 = np.zeros((n, n), order="F"), np.zeros((n, n), order="F")
+    s0, s1 = orc.C.c_int(), orc.C.c_int()
+    assert O.orc_udv_inv_one_plus_off(n, *_udv_args(orc, U, D, T), 0.0, orc._ptr(g0), orc.C.byref(s0)) == 0
+    Ds = D * np.exp(-2.0)
+    assert abs(Ds[0]) < 1.0
+    assert O.orc_udv_inv_one_plus_off(n, *_udv_args(orc, U, Ds, T), 2.0, orc._ptr(g1), orc.C.byref(s1)) == 0
+    assert s0.value == s1.value and np.abs(g0 - g1).max() < 1e-12
+    assert np.abs(g0 - np.linalg.inv(np.eye(n) + U @ np.diag(D) @ T)).max() < 1e-12
+    # the two-sided inverse with offsets on both sides is the same matrix too
+    g2, s2 = np.zeros((n, n), order="F"), orc.C.c_int()
+    I = np.asfortranarray(np.eye(n))
+    assert O.orc_udv_inv_two_sided(n, *_udv_args(orc, U, Ds, T), 2.0, *_udv_args(orc, I, np.ones(n), I), 0.0,
+                                   orc._ptr(g2), orc.C.byref(s2)) == 0
+    assert s2.value == s0.value and np.abs(g2 - g0).max() < 1e-12
+    # log_offset = 800 leaves no representable split (tests/test_udv.c:198-208)
+    eye = np.asfortranarray(np.eye(n))
+    assert O.orc_udv_inv_one_plus_off(n, *_udv_args(orc, eye, np.ones(n), eye), 800.0, orc._ptr(g0),
+                                      orc.C.byref(s0)) == 3
+    assert s0.value == 0
+
+
+def test_recenter_and_update_margin(orc):
+    O = orc.orc()
+    n = 4
+    # recentring keeps D e^offset and centres log|D| (tests/test_udv_stack.c:196-212)
+    D = np.array([np.exp(30.0), -np.exp(-10.0), 2.0, 1e-3])
+    off = orc.C.c_double(1.5)
+    Dc = D.copy()
+    assert O.orc_udv_recenter(n, orc._ptr(Dc), orc.C.byref(off)) == 0
+    ld = np.log(np.abs(Dc))
+    assert abs(ld.min() + ld.max()) < 1e-9 and off.value == pytest.approx(1.5 + 10.0, abs=1e-12)
+    assert np.allclose(Dc * np.exp(off.value - 1.5), D, rtol=1e-12) and np.array_equal(np.sign(Dc), np.sign(D))
+    # an update that would leave the double range fails with the inputs
+    # untouched (tests/test_udv_stack.c:214-252)
+    for fn in (O.orc_udv_lmul_centered, O.orc_udv_rmul_centered):
+        U = np.asfortranarray(np.eye(n))
+        T = np.asfortranarray(np.eye(n))
+        Dw = np.array([np.exp(700.0), np.exp(-700.0), 1.0, 1.0])
+        amp = np.asfortranarray(np.exp(2.0) * np.eye(n))
+        o = orc.C.c_double(0.0)
+        assert fn(n, *_udv_args(orc, U, Dw, T), orc.C.byref(o), orc._ptr(amp)) == 2
+        assert np.array_equal(U, np.eye(n)) and np.array_equal(T, np.eye(n)) and o.value == 0.0
+        assert np.array_equal(Dw, [np.exp(700.0), np.exp(-700.0), 1.0, 1.0])
+
+
 def test_u0_green_closed_form(orc):
     # U = 0: B_l = expK for every slice, G = (1 + exp(-beta K))^-1
     # (tests/test_green_init.c:29-46)
