@@ -37,6 +37,10 @@ extern "C" {
 
 /* Failure reasons: 0..3 as LINALG_FAILURE_* (src/linalg.h:48-53). */
 #define DQMC_GPU_FAIL_NONE 0
+/* LINALG_FAILURE_* of the centered rebuild (src/linalg.h:48-53) */
+#define DQMC_GPU_FAIL_CENTERED_RADIUS 1        /* log|D| range too wide to recentre */
+#define DQMC_GPU_FAIL_CENTERED_UPDATE_MARGIN 2 /* the update would leave the double range */
+#define DQMC_GPU_FAIL_EFFECTIVE_LOG_RANGE 3    /* e^-|log|D| + offset| underflows */
 #define DQMC_GPU_FAIL_NONFINITE 16 /* non-finite matrix at a stage */
 #define DQMC_GPU_FAIL_ZERO_D 17    /* zero diagonal in a QR (D_i == 0) */
 #define DQMC_GPU_FAIL_SIGN 18      /* det sign evaluated to 0 */
@@ -101,6 +105,20 @@ int dqmc_gpu_set_measure(dqmc_gpu_ctx *ctx, int model_id, int nq,
 #define DQMC_GPU_SWEEP_FORWARD 0
 #define DQMC_GPU_SWEEP_ALTERNATING 1
 int dqmc_gpu_set_sweep_order(dqmc_gpu_ctx *ctx, int model_id, int order);
+
+/* Boundary rebuild of a model (the green_rebuild key, src/io.c:287-289;
+   GreenRebuildMode, src/green.h:10-14), before dqmc_gpu_init.  COMBINE and
+   TWO_SIDED both run the one-QR two-sided inverse (the reference's
+   two_sided formula, src/linalg.c:1117-1389); CENTERED in addition keeps
+   every UDV's D recentred around a per-replica log offset
+   (udv_lmul_centered_work / udv_rmul_centered / udv_recenter,
+   src/linalg.c:453-794), so that D may exceed the double range at low
+   temperature, and fails a replica with DQMC_GPU_FAIL_CENTERED_* or
+   _EFFECTIVE_LOG_RANGE as the reference does. */
+#define DQMC_GPU_REBUILD_COMBINE 0
+#define DQMC_GPU_REBUILD_TWO_SIDED 1
+#define DQMC_GPU_REBUILD_CENTERED 2
+int dqmc_gpu_set_rebuild(dqmc_gpu_ctx *ctx, int model_id, int mode);
 
 /* Initial stabilised Green's function at tau = 0 for every replica
    (dqmc_init_modes -> green_from_scratch(Gu, 0), src/dqmc.c:418-483). */

@@ -51,7 +51,7 @@ static void set_err(const char *fmt, ...)
 // ------------------------------------------------------------ small kernels
 
 __global__ void k_udv_identity(double *U, long long sU, double *D, long long sD, double *T,
-                               long long sT, int *q, long long sq, int n, int batch)
+                               long long sT, int *q, long long sq, int n, int batch, double *off, long long so)
 {
     const int b = blockIdx.y;
     if (b >= batch) return;
@@ -63,7 +63,10 @@ __global__ void k_udv_identity(double *U, long long sU, double *D, long long sD,
         T[b * sT + idx] = v;
         if (j == 0) D[b * sD + i] = 1.0;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) q[b * sq] = 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        q[b * sq] = 1;
+        if (off) off[b * so] = 0.0; // udv_identity (src/linalg.c:432-441)
+    }
 }
 
 __global__ void k_copy_d(const double *src, long long sS, double *dst, long long sD, long long len,
@@ -155,12 +158,34 @@ __global__ void k_scale_cols_f(const float *expK, const float *d, long long sdv,
 // both the U reads and the N, X, T accesses are coalesced.  Block (0, b)
 // also records the sign of prod_{|D_i| > 1} D_i.
 #define BN_T 32
+// The big/small split of D_i (src/linalg.c:966-986, 1190-1215): without a
+// log offset Db = D where |D| > 1 (else 1) and Ds = D / Db; with one, the
+// true log-magnitude ell = log|D| + offset decides and the inverse of the
+// big part is formed directly (sgn D e^-ell), so nothing overflows.  bi is
+// the factor that divides by Db, s the small part; false when -|ell| is
+// below log(DBL_TRUE_MIN) (src/linalg.c:166-182).
+__device__ __forceinline__ bool d_split(double d, double off, double &bi, double &sm, bool &big)
+{
+    if (off == 0.0) {
+        big = fabs(d) > 1.0;
+        bi = big ? 1.0 / d : 1.0;
+        sm = big ? 1.0 : d;
+        return true;
+    }
+    const double ell = log(fabs(d)) + off;
+    big = ell > 0.0;
+    bi = big ? copysign(exp(-ell), d) : 1.0;
+    sm = big ? 1.0 : copysign(exp(ell), d);
+    return isfinite(ell) && -fabs(ell) >= -744.44007192138122; // log(4.9e-324)
+}
+
 __global__ __launch_bounds__(256) void k_build_N(const double *U, const double *D, const double *T, long long sU,
-                                                 long long sD, double *N, double *X, int *bigsign, int n, int batch,
-                                                 const int *status)
+                                                 long long sD, const double *off, long long so, double *N, double *X,
+                                                 int *bigsign, int n, int batch, int *status, int *reason)
 {
     const int b = blockIdx.y;
     if (b >= batch || status[b]) return;
+    const double ob = off ? off[b * so] : 0.0;
     const double *Ub = U + b * sU, *Tb = T + b * sU, *Db = D + b * sD;
     double *Nb = N + (long long)b * n * n, *Xb = X + (long long)b * n * n;
     const int nt = (n + BN_T - 1) / BN_T;
@@ -178,7 +203,9 @@ __global__ __launch_bounds__(256) void k_build_N(const double *U, const double *
     __syncthreads();
     const int i = i0 + tx;
     const double d = Db[min(i, n - 1)];
-    const bool big = fabs(d) > 1.0;
+    double dbi, dsm;
+    bool big;
+    (void)d_split(d, ob, dbi, dsm, big);
     double t[BN_T / 8];
 #pragma unroll
     for (int k = 0; k < BN_T / 8; k++) t[k] = Tb[min(i, n - 1) + (long long)min(j0 + ty + 8 * k, n - 1) * n];
@@ -187,44 +214,208 @@ __global__ __launch_bounds__(256) void k_build_N(const double *U, const double *
         const int j = j0 + ty + 8 * k;
         if (i < n && j < n) {
             const double uu = tl[tx][ty + 8 * k];
-            const double x = big ? uu / d : uu;
+            const double x = big ? (ob == 0.0 ? uu / d : uu * dbi) : uu;
             Xb[i + (long long)j * n] = x;
-            Nb[i + (long long)j * n] = x + (big ? 1.0 : d) * t[k];
+            Nb[i + (long long)j * n] = x + dsm * t[k];
         }
     }
     if (blockIdx.x == 0) {
-        __shared__ int neg;
-        if (threadIdx.x == 0) neg = 0;
+        __shared__ int neg, bad;
+        if (threadIdx.x == 0) {
+            neg = 0;
+            bad = 0;
+        }
         __syncthreads();
-        for (int r = threadIdx.x; r < n; r += blockDim.x)
-            if (fabs(Db[r]) > 1.0 && Db[r] < 0.0) atomicXor(&neg, 1);
+        for (int r = threadIdx.x; r < n; r += blockDim.x) {
+            double bi, sm;
+            bool bg;
+            if (!d_split(Db[r], ob, bi, sm, bg)) bad = 1;
+            if (bg && Db[r] < 0.0) atomicXor(&neg, 1);
+        }
         __syncthreads();
-        if (threadIdx.x == 0) bigsign[b] = neg ? -1 : 1;
+        if (threadIdx.x == 0) {
+            bigsign[b] = neg ? -1 : 1;
+            if (bad) {
+                status[b] = 1;
+                reason[b] = DQMC_GPU_FAIL_EFFECTIVE_LOG_RANGE;
+            }
+        }
     }
 }
 
 // Db^-1 and Ds of the left and right D (Db = D where |D| > 1, else 1;
 // Ds = D / Db) and the sign of prod_{|D_i| > 1} D_i over both, one
 // workgroup per replica.
-__global__ void k_split2(const double *Dl, long long sDl, const double *Dr, long long sDr, int n, double *lbi,
-                         double *ls, double *rbi, double *rs_, int *bigsign, int batch, const int *status)
+__global__ void k_split2(const double *Dl, long long sDl, const double *offl, long long sol, const double *Dr,
+                         long long sDr, const double *offr, long long sor, int n, double *lbi, double *ls, double *rbi,
+                         double *rs_, int *bigsign, int batch, int *status, int *reason)
 {
     const int b = blockIdx.x;
     if (b >= batch || status[b]) return;
-    __shared__ int neg;
-    if (threadIdx.x == 0) neg = 0;
+    const double ol = offl ? offl[b * sol] : 0.0, orr = offr ? offr[b * sor] : 0.0;
+    __shared__ int neg, bad;
+    if (threadIdx.x == 0) {
+        neg = 0;
+        bad = 0;
+    }
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const double dl = Dl[b * sDl + i], dr = Dr[b * sDr + i];
-        const bool bl = fabs(dl) > 1.0, br = fabs(dr) > 1.0;
-        lbi[(long long)b * n + i] = bl ? 1.0 / dl : 1.0;
-        ls[(long long)b * n + i] = bl ? 1.0 : dl;
-        rbi[(long long)b * n + i] = br ? 1.0 / dr : 1.0;
-        rs_[(long long)b * n + i] = br ? 1.0 : dr;
+        double li, lsv, ri, rsv;
+        bool bl, br;
+        const bool okl = d_split(dl, ol, li, lsv, bl), okr = d_split(dr, orr, ri, rsv, br);
+        if (!okl || !okr) bad = 1;
+        lbi[(long long)b * n + i] = li;
+        ls[(long long)b * n + i] = lsv;
+        rbi[(long long)b * n + i] = ri;
+        rs_[(long long)b * n + i] = rsv;
         if ((bl && dl < 0.0) != (br && dr < 0.0)) atomicXor(&neg, 1);
     }
     __syncthreads();
-    if (threadIdx.x == 0) bigsign[b] = neg ? -1 : 1;
+    if (threadIdx.x == 0) {
+        bigsign[b] = neg ? -1 : 1;
+        if (bad) {
+            status[b] = 1;
+            reason[b] = DQMC_GPU_FAIL_EFFECTIVE_LOG_RANGE;
+        }
+    }
+}
+
+// udv_recenter (src/linalg.c:453-513) with the hard margin 8 of the
+// centered updates, one workgroup per replica: c = centre of the log|D|
+// range, Dout = D / e^c, offout = offin + c.  The output may be the input.
+// A range whose half-width leaves less than the margin below log(DBL_MAX),
+// or an entry that would not survive the rescaling, fails the replica
+// (LINALG_FAILURE_CENTERED_RADIUS) and writes nothing.
+__global__ __launch_bounds__(256) void k_udv_recenter(const double *Din, long long sDin, const double *oin,
+                                                      long long soin, double *Dout, long long sDout, double *oout,
+                                                      long long soout, int n, int batch, int *status, int *reason)
+{
+    const int b = blockIdx.x;
+    if (b >= batch || status[b]) return;
+    __shared__ double wlo[4], whi[4];
+    __shared__ int bad;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) bad = 0;
+    double d[2], ld[2];
+    double lo = INFINITY, hi = -INFINITY;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int i = threadIdx.x + 256 * k;
+        d[k] = i < n ? Din[b * sDin + i] : 1.0;
+        ld[k] = log(fabs(d[k]));
+        if (i < n) {
+            lo = fmin(lo, ld[k]);
+            hi = fmax(hi, ld[k]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+        if (threadIdx.x + 256 * k < n && (!isfinite(d[k]) || d[k] == 0.0 || !isfinite(ld[k]))) bad = 1;
+    lo = -wave_max(-lo);
+    hi = wave_max(hi);
+    if (lane == 0) {
+        wlo[wave] = lo;
+        whi[wave] = hi;
+    }
+    __syncthreads();
+    lo = fmin(fmin(wlo[0], wlo[1]), fmin(wlo[2], wlo[3]));
+    hi = fmax(fmax(whi[0], whi[1]), fmax(whi[2], whi[3]));
+    const double off = oin ? oin[b * soin] : 0.0;
+    const double c = lo + 0.5 * (hi - lo), r = 0.5 * (hi - lo);
+    const double margin = 709.78271289338397 - r; // log(DBL_MAX) - r
+    double v[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        v[k] = exp(ld[k] - c);
+        if (threadIdx.x + 256 * k < n && (!isfinite(v[k]) || v[k] == 0.0)) bad = 1;
+    }
+    __syncthreads();
+    const bool fail = bad || !isfinite(c) || !isfinite(r) || !isfinite(margin) || margin < 8.0 || !isfinite(off + c);
+    if (fail) {
+        if (threadIdx.x == 0) {
+            status[b] = 1;
+            reason[b] = DQMC_GPU_FAIL_CENTERED_RADIUS;
+        }
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int i = threadIdx.x + 256 * k;
+        if (i < n) Dout[b * sDout + i] = copysign(v[k], d[k]);
+    }
+    if (threadIdx.x == 0) oout[b * soout] = off + c;
+}
+
+// centered_update_has_margin (src/linalg.c:119-164), one workgroup per
+// replica: a bound on the largest QR input entry of the update, which
+// must stay below log(DBL_MAX) - 8; left: log|D_j| + max_i log sum_k |B_ik|;
+// right: max_i log|D_i| + log sum_k |T_ik| max_j |B_kj|.  Sums in the log
+// domain shifted by their largest term.  A replica without the margin fails
+// (LINALG_FAILURE_CENTERED_UPDATE_MARGIN).
+__global__ __launch_bounds__(256) void k_update_margin(const double *B, long long sB, const double *T, long long sT,
+                                                       const double *D, long long sD, int n, int left, int batch,
+                                                       int *status, int *reason)
+{
+    const int b = blockIdx.x;
+    if (b >= batch || status[b]) return;
+    __shared__ double bmx[METRO_MAX_N];
+    __shared__ double wmax[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const double *Bb = B + b * sB, *Tb = T ? T + b * sT : nullptr, *Db = D + b * sD;
+    double worst = -INFINITY;
+    if (left) {
+        // row i of B: max_k |B_ik| then sum_k |B_ik| / max, rows over threads
+        double lb = -INFINITY;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            double mx = 0.0;
+            for (int k = 0; k < n; k++) mx = fmax(mx, fabs(Bb[i + (long long)k * n]));
+            double sm = 0.0;
+            if (mx > 0.0)
+                for (int k = 0; k < n; k++) sm += fabs(Bb[i + (long long)k * n]) / mx;
+            if (mx > 0.0) lb = fmax(lb, log(mx) + log(sm));
+        }
+        lb = wave_max(lb);
+        if (lane == 0) wmax[wave] = lb;
+        __syncthreads();
+        lb = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+        __syncthreads();
+        for (int j = threadIdx.x; j < n; j += blockDim.x) worst = fmax(worst, log(fabs(Db[j])) + lb);
+    } else {
+        for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            double mx = 0.0;
+            for (int j = 0; j < n; j++) mx = fmax(mx, fabs(Bb[k + (long long)j * n]));
+            bmx[k] = mx;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            double top = -INFINITY;
+            for (int k = 0; k < n; k++) {
+                const double t = fabs(Tb[i + (long long)k * n]);
+                if (t != 0.0 && bmx[k] != 0.0) top = fmax(top, log(t) + log(bmx[k]));
+            }
+            double sm = 0.0;
+            if (top > -INFINITY)
+                for (int k = 0; k < n; k++) {
+                    const double t = fabs(Tb[i + (long long)k * n]);
+                    if (t != 0.0 && bmx[k] != 0.0) sm += exp(log(t) + log(bmx[k]) - top);
+                }
+            worst = fmax(worst, log(fabs(Db[i])) + top + log(sm));
+        }
+    }
+    // NaN (a non-finite input) counts as no margin
+    double w = isnan(worst) ? INFINITY : worst;
+    w = wave_max(w);
+    if (lane == 0) wmax[wave] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double ww = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+        if (!(isfinite(ww) && ww <= 709.78271289338397 - 8.0)) {
+            status[b] = 1;
+            reason[b] = DQMC_GPU_FAIL_CENTERED_UPDATE_MARGIN;
+        }
+    }
 }
 
 // M = Dlb^-1 Z^T Drb^-1 + Dls X Drs per 32 x 32 tile, Z read through an
@@ -406,6 +597,8 @@ struct Udv {
     double *U, *D, *T;
     int *q;
     long long sU, sD, sq; // per-replica strides
+    double *off = nullptr; // log offset of a centered UDV per replica (null: 0)
+    long long so = 0;
 };
 
 struct Work {
@@ -596,8 +789,8 @@ static int udv_inv_one_plus(Work &w, const Udv &u, double *Gout, long long sG)
     const int n = w.n;
     const long long nn = (long long)n * n;
     const int bnt = (n + BN_T - 1) / BN_T;
-    hipLaunchKernelGGL(k_build_N, dim3(bnt * bnt, w.nb), dim3(256), 0, w.s, u.U, u.D, u.T, u.sU, u.sD, w.W1, w.W4, w.I1, n,
-                       w.nb, w.status);
+    hipLaunchKernelGGL(k_build_N, dim3(bnt * bnt, w.nb), dim3(256), 0, w.s, u.U, u.D, u.T, u.sU, u.sD, u.off, u.so,
+                       w.W1, w.W4, w.I1, n, w.nb, w.status, w.reason);
     if (launch_check("k_build_N")) return 1;                         // W1 = N, W4 = X
     if (qr(w, w.W1, w.W2, nn, w.W3, nn, w.V3, n, w.I2, 1)) return 1; // W2 = Q_N, V3 = D_N, W3 = Rn_N
     GemmArgs<double> g = gemm_args<double>(n, n, n, w.nb, w.W2, n, nn, 1, w.W4, n, nn, 0, w.W1, n, nn);
@@ -623,8 +816,8 @@ static int udv_inv_two_sided(Work &w, const Udv &l, const Udv &r, double *Gout, 
     ProfScope ps_(DQMC_GPU_PROF_INV, w.s);
     const int n = w.n;
     const long long nn = (long long)n * n;
-    hipLaunchKernelGGL(k_split2, dim3(w.nb), dim3(256), 0, w.s, l.D, l.sD, r.D, r.sD, n, w.V1, w.V2, w.V4, w.V5,
-                       w.I1, w.nb, w.status);
+    hipLaunchKernelGGL(k_split2, dim3(w.nb), dim3(256), 0, w.s, l.D, l.sD, l.off, l.so, r.D, r.sD, r.off, r.so, n,
+                       w.V1, w.V2, w.V4, w.V5, w.I1, w.nb, w.status, w.reason);
     if (launch_check("k_split2")) return 1;
     if (trsm(w, r.T, r.sU, l.U, l.sU, w.W1, nn, 1)) return 1; // W1 = Tr^-T Ul = (Ul^T Tr^-1)^T
     GemmArgs<double> g = gemm_args<double>(n, n, n, w.nb, l.T, n, l.sU, 0, r.U, n, r.sU, 0, w.W2, n, nn);
@@ -671,6 +864,10 @@ struct Model {
     uint64_t *rng = nullptr;
     double *SU = nullptr, *SD = nullptr, *ST = nullptr;
     int *Sq = nullptr;
+    // log offsets of the centered rebuild mode (green_rebuild=centered):
+    // suffix stack, prefix stack, left and combined UDVs, all 0 otherwise
+    double *SO = nullptr, *AO = nullptr, *LO = nullptr, *CO = nullptr;
+    bool centered = false;
     double *LU_ = nullptr, *LD = nullptr, *LT = nullptr;
     int *Lq = nullptr;
     double *CU = nullptr, *CD = nullptr, *CT = nullptr;
@@ -709,23 +906,23 @@ struct Model {
 
     Udv left() const
     {
-        return Udv{LU_, LD, LT, Lq, (long long)n * n, (long long)n, 1};
+        return Udv{LU_, LD, LT, Lq, (long long)n * n, (long long)n, 1, LO, 1};
     }
     Udv comb() const
     {
-        return Udv{CU, CD, CT, Cq, (long long)n * n, (long long)n, 1};
+        return Udv{CU, CD, CT, Cq, (long long)n * n, (long long)n, 1, CO, 1};
     }
     Udv stack(int j) const
     {
         const long long nn = (long long)n * n;
         return Udv{SU + j * nn, SD + (long long)j * n, ST + j * nn, Sq + j, (M + 1) * nn,
-                   (long long)(M + 1) * n, (long long)(M + 1)};
+                   (long long)(M + 1) * n, (long long)(M + 1), SO ? SO + j : nullptr, (long long)(M + 1)};
     }
     Udv alt(int j) const
     {
         const long long nn = (long long)n * n;
         return Udv{AU + j * nn, AD + (long long)j * n, AT + j * nn, Aq + j, (M + 1) * nn,
-                   (long long)(M + 1) * n, (long long)(M + 1)};
+                   (long long)(M + 1) * n, (long long)(M + 1), AO ? AO + j : nullptr, (long long)(M + 1)};
     }
 };
 
@@ -777,12 +974,14 @@ static int alt_alloc(Model &m)
     if (m.AU) return 0;
     const size_t nn = (size_t)m.n * m.n, S = (size_t)(m.M + 1), R = (size_t)m.nrep;
     if (dalloc(&m.AU, nn * S * R) || dalloc(&m.AD, (size_t)m.n * S * R) || dalloc(&m.AT, nn * S * R) ||
-        dalloc(&m.Aq, S * R))
+        dalloc(&m.Aq, S * R) || dalloc(&m.AO, S * R))
         return 1;
     m.owned.push_back(m.AU); m.owned.push_back(m.AD); m.owned.push_back(m.AT); m.owned.push_back(m.Aq);
+    m.owned.push_back(m.AO);
+    HIPCHK(hipMemset(m.AO, 0, sizeof(double) * S * R));
     const Udv a0 = m.alt(0);
     hipLaunchKernelGGL(k_udv_identity, grid2((long long)nn, m.nrep), dim3(256), 0, m.w.s, a0.U, a0.sU, a0.D, a0.sD,
-                       a0.T, a0.sU, a0.q, a0.sq, m.n, m.nrep);
+                       a0.T, a0.sU, a0.q, a0.sq, m.n, m.nrep, a0.off, a0.so);
     return launch_check("k_udv_identity");
 }
 
@@ -815,6 +1014,11 @@ static int model_alloc(dqmc_gpu_ctx *ctx, Model &m)
         dalloc(&m.Sq, S * R))
         return 1;
     own(m.SU); own(m.SD); own(m.ST); own(m.Sq);
+    if (dalloc(&m.SO, S * R) || dalloc(&m.LO, (size_t)R) || dalloc(&m.CO, (size_t)R)) return 1;
+    own(m.SO); own(m.LO); own(m.CO);
+    HIPCHK(hipMemset(m.SO, 0, sizeof(double) * S * R));
+    HIPCHK(hipMemset(m.LO, 0, sizeof(double) * R));
+    HIPCHK(hipMemset(m.CO, 0, sizeof(double) * R));
     if (m.alternating && alt_alloc(m)) return 1;
     if (dalloc(&m.LU_, nn * R) || dalloc(&m.LD, (size_t)n * R) || dalloc(&m.LT, nn * R) ||
         dalloc(&m.Lq, R) || dalloc(&m.CU, nn * R) || dalloc(&m.CD, (size_t)n * R) ||
@@ -893,7 +1097,7 @@ static int model_alloc(dqmc_gpu_ctx *ctx, Model &m)
     // the last stack entry is the identity and never changes (src/green.c:205)
     const Udv sm = m.stack(m.M);
     hipLaunchKernelGGL(k_udv_identity, grid2(nn, R), dim3(256), 0, m.w.s, sm.U, sm.sU, sm.D, sm.sD, sm.T,
-                       sm.sU, sm.q, sm.sq, n, R);
+                       sm.sU, sm.q, sm.sq, n, R, sm.off, sm.so);
     if (launch_check("k_udv_identity")) return 1;
     HIPCHK(hipStreamSynchronize(m.w.s));
     m.allocated = true;
@@ -1063,6 +1267,23 @@ static int bblock_refresh(Model &m)
     return 0;
 }
 
+// recentre a UDV's D (into `to`, which may be `from`): udv_recenter
+static int udv_recenter(Model &m, const Udv &from, const Udv &to)
+{
+    hipLaunchKernelGGL(k_udv_recenter, dim3(m.nrep), dim3(256), 0, m.w.s, from.D, from.sD, from.off, from.so, to.D,
+                       to.sD, to.off, to.so, m.n, m.nrep, m.w.status, m.w.reason);
+    return launch_check("k_udv_recenter");
+}
+
+// the update's margin check of the centered mode on B (and T for rmul)
+static int update_margin(Model &m, const double *B, long long sB, const double *T, long long sT, const double *D,
+                         long long sD, int left)
+{
+    hipLaunchKernelGGL(k_update_margin, dim3(m.nrep), dim3(256), 0, m.w.s, B, sB, T, sT, D, sD, m.n, left, m.nrep,
+                       m.w.status, m.w.reason);
+    return launch_check("k_update_margin");
+}
+
 // out <- B_{lb+len-1} ... B_{lb} in   (udv_lmul_work, src/linalg.c:515-642);
 // `out` may be `in` (in place) or another UDV (the prefix stack's next entry)
 static int stage_lmul(dqmc_gpu_ctx *ctx, Model &m, const Udv &in, const Udv &out, int lb, int len)
@@ -1071,21 +1292,31 @@ static int stage_lmul(dqmc_gpu_ctx *ctx, Model &m, const Udv &in, const Udv &out
     const int n = m.n, R = m.nrep;
     const long long nn = (long long)n * n, sV = (long long)m.L * n;
     double *cur;
+    // centered mode (udv_lmul_centered_work, src/linalg.c:540-560): the
+    // input's D is recentred into `out` and the update reads it from there
+    Udv src = in;
+    if (m.centered) {
+        if (udv_recenter(m, in, out)) return 1;
+        src.D = out.D;
+        src.sD = out.sD;
+    }
+    const bool aligned = lb % m.stab == 0 && len == std::min(m.stab, m.L - lb);
     if (chain_f32(ctx)) {
         float *Bf = bblock_f32(m, lb, len);
         hipLaunchKernelGGL(k_f2d, grid2(nn, R), dim3(256), 0, m.w.s, Bf, nn, m.w.W3, nn, nn, R);
         GemmArgs<double> g = gemm_args<double>(n, n, n, R, m.w.W3, n, nn, 0, in.U, n, in.sU, 0, m.w.W1, n, nn);
-        g.bc = vec(in.D, in.sD);
+        g.bc = vec(src.D, src.sD);
         if (gemm(m.w, g)) return 1;
         cur = m.w.W1;
-    } else if (m.BB && lb % m.stab == 0 && len == std::min(m.stab, m.L - lb)) {
+    } else if (m.BB && (aligned || m.centered)) {
         // Bblock_j = B_{lb+len-1} ... B_{lb}, left-accumulated (src/green.c:151-166),
-        // kept for the next sweep's suffix stack; then (Bblock U) diag(D)
-        const int j = lb / m.stab;
-        double *Bj = m.BB + (long long)j * R * nn;
+        // kept for the next sweep's suffix stack (a block not on the stack
+        // grid, centered only, goes to W3); then (Bblock U) diag(D)
+        double *Bj = aligned ? m.BB + (long long)(lb / m.stab) * R * nn : m.w.W3;
         if (bblock_build(m, lb, len, Bj)) return 1;
+        if (m.centered && update_margin(m, Bj, nn, nullptr, 0, src.D, src.sD, 1)) return 1;
         GemmArgs<double> g = gemm_args<double>(n, n, n, R, Bj, n, nn, 0, in.U, n, in.sU, 0, m.w.W1, n, nn);
-        g.bc = vec(in.D, in.sD);
+        g.bc = vec(src.D, src.sD);
         if (gemm(m.w, g)) return 1;
         cur = m.w.W1;
     } else {
@@ -1110,7 +1341,8 @@ static int stage_lmul(dqmc_gpu_ctx *ctx, Model &m, const Udv &in, const Udv &out
         gemm_args<double>(n, n, n, R, m.w.W3, n, nn, 0, in.T, n, in.sU, 0, inplace ? m.w.W4 : out.T, n,
                           inplace ? nn : out.sU);
     if (gemm(m.w, g)) return 1;
-    return inplace ? copy_nn(m.w, m.w.W4, nn, out.T, out.sU, nn) : 0;
+    if (inplace && copy_nn(m.w, m.w.W4, nn, out.T, out.sU, nn)) return 1;
+    return m.centered ? udv_recenter(m, out, out) : 0;
 }
 
 // S[j] = S[j+1] B_{b_{j+1}-1} ... B_{b_j}   (green_stack_build_suffix +
@@ -1122,26 +1354,40 @@ static int stage_rmul(dqmc_gpu_ctx *ctx, Model &m, int j)
     const long long nn = (long long)n * n, sV = (long long)m.L * n;
     const int bj = std::min(j * m.stab, m.L), bj1 = std::min((j + 1) * m.stab, m.L);
     const Udv s1 = m.stack(j + 1), s0 = m.stack(j);
+    // centered mode (udv_rmul_centered, src/linalg.c:687-704): S[j+1]'s D
+    // recentred into S[j], from where the update reads it
+    const double *Dsrc = s1.D;
+    long long sDsrc = s1.sD;
+    if (m.centered) {
+        if (!(m.BB && m.bb_valid)) {
+            set_err("centered rebuild needs the block-product cache");
+            return 1;
+        }
+        if (udv_recenter(m, s1, s0)) return 1;
+        Dsrc = s0.D;
+        sDsrc = s0.sD;
+        if (update_margin(m, m.BB + (long long)j * R * nn, nn, s1.T, s1.sU, Dsrc, sDsrc, 0)) return 1;
+    }
     double *cur;
     if (chain_f32(ctx)) {
         float *Bf = bblock_f32(m, bj, bj1 - bj);
         hipLaunchKernelGGL(k_f2d, grid2(nn, R), dim3(256), 0, m.w.s, Bf, nn, m.w.W3, nn, nn, R);
         GemmArgs<double> g = gemm_args<double>(n, n, n, R, s1.T, n, s1.sU, 0, m.w.W3, n, nn, 0, m.w.W1, n, nn);
-        g.ar = vec(s1.D, s1.sD);
+        g.ar = vec(Dsrc, sDsrc);
         if (gemm(m.w, g)) return 1;
         cur = m.w.W1;
     } else if (m.BB && m.bb_valid) {
         // diag(D) T Bblock_j with the block product the left chain formed
         GemmArgs<double> g = gemm_args<double>(n, n, n, R, s1.T, n, s1.sU, 0, m.BB + (long long)j * R * nn, n, nn, 0,
                                                m.w.W1, n, nn);
-        g.ar = vec(s1.D, s1.sD);
+        g.ar = vec(Dsrc, sDsrc);
         if (gemm(m.w, g)) return 1;
         cur = m.w.W1;
     } else {
         // diag(D) T B_top ... B_bj, B applied one slice at a time from the right
         const int top = bj1 - 1;
         GemmArgs<double> g = gemm_args<double>(n, n, n, R, s1.T, n, s1.sU, 0, m.expK, n, 0, 0, m.w.W1, n, nn);
-        g.ar = vec(s1.D, s1.sD);
+        g.ar = vec(Dsrc, sDsrc);
         g.cc = vec<double>(m.dvec + (long long)top * n, sV);
         if (gemm(m.w, g)) return 1;
         cur = m.w.W1;
@@ -1159,14 +1405,15 @@ static int stage_rmul(dqmc_gpu_ctx *ctx, Model &m, int j)
     if (gemm(m.w, g)) return 1;
     hipLaunchKernelGGL(k_mul_q, dim3((R + 255) / 256), dim3(256), 0, m.w.s, s1.q, s1.sq, m.w.I1, 1LL, s0.q,
                        s0.sq, R, m.w.status);
-    return launch_check("k_mul_q");
+    if (launch_check("k_mul_q")) return 1;
+    return m.centered ? udv_recenter(m, s0, s0) : 0;
 }
 
 static int udv_identity(Model &m, const Udv &u)
 {
     const long long nn = (long long)m.n * m.n;
     hipLaunchKernelGGL(k_udv_identity, grid2(nn, m.nrep), dim3(256), 0, m.w.s, u.U, u.sU, u.D, u.sD, u.T,
-                       u.sU, u.q, u.sq, m.n, m.nrep);
+                       u.sU, u.q, u.sq, m.n, m.nrep, u.off, u.so);
     return launch_check("k_udv_identity");
 }
 
@@ -1625,6 +1872,28 @@ int dqmc_gpu_set_sweep_order(dqmc_gpu_ctx *ctx, int model_id, int order)
         if (alt_alloc(m)) return 1;
         HIPCHK(hipStreamSynchronize(m.w.s));
     }
+    return 0;
+}
+
+int dqmc_gpu_set_rebuild(dqmc_gpu_ctx *ctx, int model_id, int mode)
+{
+    if (!ctx || model_id < 0 || model_id >= (int)ctx->models.size() ||
+        (mode != DQMC_GPU_REBUILD_COMBINE && mode != DQMC_GPU_REBUILD_TWO_SIDED &&
+         mode != DQMC_GPU_REBUILD_CENTERED)) {
+        set_err("invalid set_rebuild arguments");
+        return 1;
+    }
+    if (ctx->initialised) {
+        set_err("the rebuild mode must be set before dqmc_gpu_init");
+        return 1;
+    }
+    if (mode == DQMC_GPU_REBUILD_CENTERED && ctx->opts.precision == DQMC_GPU_MIXED_FP32_CHAIN) {
+        set_err("the centered rebuild needs the FP64 B-chain");
+        return 1;
+    }
+    Model &m = *ctx->models[model_id];
+    m.centered = mode == DQMC_GPU_REBUILD_CENTERED;
+    if (m.centered) m.two_sided = true; // centered factors meet in the two-sided inverse
     return 0;
 }
 

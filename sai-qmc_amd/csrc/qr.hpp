@@ -263,7 +263,7 @@ __device__ __forceinline__ void pub_wait(const int *p, int need, int *bad)
 
 template <int MR>
 __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int ld, int *s_col, int *s_pub,
-                         double *s_tau, double *s_sc, double *s_beta, double *Tl, int *bad)
+                         double *s_tau, double *s_sc, double *s_beta, int *s_big, double *Tl, int *bad)
 {
     constexpr int NS = QR_NB / QR_WAVES; // column slots per wave
     const int lane = threadIdx.x & 63;
@@ -299,11 +299,46 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
         q = wave_sum(q);
         const double alpha = bcast_lane(col[0], c2);
         double tau, sc, beta;
-        reflector(alpha, q, tau, sc, beta);
+        // squares of entries beyond ~1e150 overflow and those below ~1e-140
+        // lose digits: then ||x|| is taken scaled by max|x_i| (dnrm2,
+        // dlapy2), and the column is marked big so that its dot products
+        // use the scaled reflector (wave-uniform branches)
+        const bool plain = q <= 1e300 && q >= 1e-280 && fabs(alpha) <= 1e150;
+        if (plain) {
+            reflector(alpha, q, tau, sc, beta);
+        } else {
+            double mx = 0.0;
+#pragma unroll
+            for (int s = 0; s < MR; s++) {
+                const int r = lane + 64 * s;
+                if (r > c2 && r < m) mx = fmax(mx, fabs(col[s]));
+            }
+            mx = wave_max(mx);
+            double qs = 0.0;
+            if (mx > 0.0) {
+#pragma unroll
+                for (int s = 0; s < MR; s++) {
+                    const int r = lane + 64 * s;
+                    const double t = col[s] / mx;
+                    if (r > c2 && r < m) qs += t * t;
+                }
+            }
+            qs = wave_sum(qs);
+            const double xn = mx * sqrt(qs);
+            tau = 0.0;
+            sc = 0.0;
+            beta = alpha;
+            if (xn != 0.0) {
+                beta = -copysign(hypot(alpha, xn), alpha);
+                tau = (beta - alpha) / beta;
+                sc = 1.0 / (alpha - beta);
+            }
+        }
         if (lane == 0) {
             s_tau[c2] = tau;
             s_sc[c2] = sc;
             s_beta[c2] = beta;
+            s_big[c2] = plain ? 0 : 1;
             Tl[c2 * (QR_NB + 1)] = tau;
             if (!isfinite(tau) || !isfinite(sc)) *bad = 1;
         }
@@ -331,14 +366,20 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
         const bool owner = c + 1 < nb && ((c + 1) % QR_WAVES) == wave;
         const int k1 = (c + 1) / QR_WAVES;
         double tau = 0.0, sc = 0.0;
-        // column c2 <- column c2 - w v_c, w = tau (pcc + sc v_c^T x_below), v_c = e_c + sc x
+        int bigc = 0;
+        // column c2 <- column c2 - w v_c, w = tau (pcc + sc v_c^T x_below), v_c = e_c + sc x;
+        // for a big column x holds sc x already (dsum = v_c^T x_below)
         auto apply = [&](double (&col)[MR], double dsum) {
             const double pcc = bcast_lane(col[0], cl);
-            const double w = tau * (pcc + sc * dsum);
-            const double ws = w * sc;
+            const double w = tau * (pcc + (bigc ? dsum : sc * dsum));
+            const double ws = bigc ? w : w * sc;
 #pragma unroll
             for (int s = 0; s < MR; s++) col[s] -= ws * x[s];
             if (lane == cl) col[0] -= w;
+        };
+        auto scale_x = [&]() {
+#pragma unroll
+            for (int s = 0; s < MR; s++) x[s] *= sc;
         };
         if (owner) {
             // lookahead: the reduction x_c^T a_{c+1} as soon as column c is
@@ -353,10 +394,18 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
                     double acc = 0.0;
 #pragma unroll
                     for (int s = 0; s < MR; s++) acc += x[s] * pc[k][s];
-                    const double dsum = wave_sum(acc);
+                    double dsum = wave_sum(acc);
                     pub_wait(s_pub, c + 1, bad);
                     tau = s_tau[c];
                     sc = s_sc[c];
+                    bigc = __builtin_amdgcn_readfirstlane(s_big[c]);
+                    if (bigc) {
+                        scale_x();
+                        acc = 0.0;
+#pragma unroll
+                        for (int s = 0; s < MR; s++) acc += x[s] * pc[k][s];
+                        dsum = wave_sum(acc);
+                    }
                     if (tau != 0.0) apply(pc[k], dsum);
                     publish(pc[k], c + 1);
                 }
@@ -367,6 +416,8 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
             read_x();
             tau = s_tau[c];
             sc = s_sc[c];
+            bigc = __builtin_amdgcn_readfirstlane(s_big[c]);
+            if (bigc) scale_x();
         }
         if (tau != 0.0) {
             double d[NS];
@@ -514,6 +565,7 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_udv(QrArgs a)
     double *s_sc = s_tau + QR_NB;
     double *s_beta = s_sc + QR_NB;
     __shared__ int s_bad, s_ntau, s_col, s_pub;
+    __shared__ int s_big[QR_NB];
     if (tid == 0) { s_bad = 0; s_ntau = 0; }
     __syncthreads();
 
@@ -528,7 +580,7 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_udv(QrArgs a)
             s_pub = 0;
         }
         __syncthreads();
-        qr_panel<MR>(A, n, p0, nb, m, P, ld, &s_col, &s_pub, s_tau, s_sc, s_beta, Tl, &s_bad);
+        qr_panel<MR>(A, n, p0, nb, m, P, ld, &s_col, &s_pub, s_tau, s_sc, s_beta, s_big, Tl, &s_bad);
         if (wave == 0) { // non-trivial reflectors, counted with one ballot
             const bool nz = (tid < nb) && s_tau[min(tid, QR_NB - 1)] != 0.0;
             const unsigned long long bal = __ballot(nz);

@@ -29,6 +29,7 @@ STEP_WRAP = 1
 STEP_REBUILD = 2
 SWEEP_FORWARD = 0
 SWEEP_ALTERNATING = 1
+REBUILD = {"combine": 0, "two_sided": 1, "centered": 2}
 
 _lib = None
 
@@ -42,6 +43,7 @@ SIGNATURES = {
     "dqmc_gpu_add_replicas": (_I, [_P, _I, _I, _P, _P, C.POINTER(_I)]),
     "dqmc_gpu_set_measure": (_I, [_P, _I, _I, _P, _I]),
     "dqmc_gpu_set_sweep_order": (_I, [_P, _I, _I]),
+    "dqmc_gpu_set_rebuild": (_I, [_P, _I, _I]),
     "dqmc_gpu_init": (_I, [_P]),
     "dqmc_gpu_sweep": (_I, [_P, _I, _I]),
     "dqmc_gpu_run": (_I, [_P, _I, _I, _I]),
@@ -276,6 +278,10 @@ class Engine:
         """sweep_order = "forward" or "alternating" (src/io.c:284-286)."""
         code = {"forward": SWEEP_FORWARD, "alternating": SWEEP_ALTERNATING}[order]
         _check(lib().dqmc_gpu_set_sweep_order(self.h, model_id, code), "dqmc_gpu_set_sweep_order")
+
+    def set_rebuild(self, model_id, mode="combine"):
+        """green_rebuild = "combine", "two_sided" or "centered" (src/io.c:287-289)."""
+        _check(lib().dqmc_gpu_set_rebuild(self.h, model_id, REBUILD[mode]), "dqmc_gpu_set_rebuild")
 
     def init(self):
         _check(lib().dqmc_gpu_init(self.h), "dqmc_gpu_init")

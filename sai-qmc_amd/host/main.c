@@ -364,6 +364,10 @@ int main(int argc, char **argv)
                                      strcmp(p.sweep_order, "alternating") ? DQMC_GPU_SWEEP_FORWARD
                                                                           : DQMC_GPU_SWEEP_ALTERNATING))
             return fail("dqmc_gpu_set_sweep_order", ctx);
+        const int rebuild = !strcmp(p.green_rebuild, "centered")    ? DQMC_GPU_REBUILD_CENTERED
+                            : !strcmp(p.green_rebuild, "two_sided") ? DQMC_GPU_REBUILD_TWO_SIDED
+                                                                    : DQMC_GPU_REBUILD_COMBINE;
+        if (dqmc_gpu_set_rebuild(ctx, mid, rebuild)) return fail("dqmc_gpu_set_rebuild", ctx);
         if (dqmc_gpu_init(ctx)) return fail("dqmc_gpu_init", ctx);
         if (dqmc_gpu_run(ctx, p.nwarm, per, nbin)) return fail("dqmc_gpu_run", ctx);
         if (dqmc_gpu_read_status(ctx, st, why, NULL, NULL, NULL)) return fail("dqmc_gpu_read_status", ctx);

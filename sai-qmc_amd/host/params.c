@@ -5,9 +5,8 @@
  * params_read (src/io.c:71-103 defaults, 156-418 parsing and checks):
  * '#' starts a comment, one key=value per line, unknown keys are errors.
  * Keys that select reference code paths the GPU engine does not implement
- * (green_rebuild=centered, lattice=file, profile=1, the UDV diagnostics
- * files) are accepted only at their default values.  green_rebuild=two_sided selects the boundary rebuild the
- * engine runs for every mode (the one-QR two-sided inverse).
+ * (lattice=file, profile=1, the UDV diagnostics files) are accepted only
+ * at their default values.
  * Additional keys: precision=fp64|mixed|mixed_fp32_chain, delay=<k>, device=<id>.
  */
 #include "host.h"
@@ -269,8 +268,8 @@ int params_read(Params *p, const char *path)
         fprintf(stderr, "ERROR: green_rebuild must be combine, two_sided, or centered (got %s)\n",
                 p->green_rebuild);
         return 1;
-    } else if (!strcmp(p->green_rebuild, "centered"))
-        err = "green_rebuild=centered is not supported by the GPU engine";
+    } else if (!strcmp(p->green_rebuild, "centered") && !strcmp(p->precision, "mixed_fp32_chain"))
+        err = "green_rebuild=centered needs precision=fp64 or mixed";
     else if (strcmp(p->parallel, "serial") && strcmp(p->parallel, "omp") && strcmp(p->parallel, "mpi") &&
              strcmp(p->parallel, "hybrid"))
         err = "parallel must be serial, omp, mpi, or hybrid";

@@ -123,3 +123,35 @@ def test_alternating_reference_cases(orc, eng_mod, kind, Lx, Ly, L, stab, sweeps
     assert int(cnt[0]) == sweeps * ((L - 1) // stab + 1)
     assert mx[0] < 1e-7
     eng.close()
+
+
+GOLD = __import__("os").path.join(__import__("os").path.dirname(__import__("os").path.abspath(__file__)), "golden")
+MODE_CASES = (("two_sided", False, 8.0, 80, 8, 3, 4), ("centered", False, 8.0, 80, 8, 3, 4),
+              ("centered", True, 8.0, 80, 8, 4, 4), ("centered", False, 16.0, 400, 10, 3, 2))
+
+
+@pytest.mark.parametrize("case", range(len(MODE_CASES)))
+def test_rebuild_modes_trajectory(eng_mod, case):
+    # green_rebuild=two_sided / centered against the compiled reference's
+    # own trajectories (tests/golden/traj_modes.npz); the last case (U=16,
+    # beta=40) has D beyond 1e300: only the centered factors and the
+    # overflow-safe reflector norms keep it finite
+    mode, alt, U, L, stab, seed, nsw = MODE_CASES[case]
+    g = np.load(__import__("os").path.join(GOLD, "traj_modes.npz"))
+    eng = eng_mod.Engine(eng_mod.FP64)
+    mid = eng.add_model("square", 4, 4, U, 0.1, L, stab, delay_k=16)
+    eng.set_rebuild(mid, mode)
+    if alt:
+        eng.set_sweep_order(mid, "alternating")
+    eng.add_replicas(mid, [seed])
+    eng.init()
+    for s in range(nsw):
+        eng.sweep(1)
+        st = eng.status()
+        assert st["status"][0] == 0, (s, st["reason"][0])
+        assert np.array_equal(eng.field(0), g[f"fields_{case}"][s]), s
+        assert int(st["accepted"][0]) == int(g[f"accepted_{case}"][s]) and st["sign"][0] == g[f"sign_{case}"][s]
+        ref = g[f"green_{case}"][s]
+        err = np.abs(eng.green(0) - ref).max() / np.abs(ref).max()
+        assert err < 1e-9, (s, err)
+    eng.close()

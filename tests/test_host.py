@@ -132,10 +132,14 @@ def test_params_defaults_and_keys(host, tmp_path):
 def test_params_sweep_order_and_rebuild(host, tmp_path):
     rc, p = read(host, tmp_path, "sweep_order=alternating\ngreen_rebuild=two_sided\n")
     assert rc == 0 and p.sweep_order == b"alternating" and p.green_rebuild == b"two_sided"
+    rc, p = read(host, tmp_path, "green_rebuild=centered\nreplica_log=none\nspin_consistency_file=d.dat\n")
+    assert rc == 0 and p.green_rebuild == b"centered" and p.replica_log == b"none"
+    assert p.spin_consistency_file == b"d.dat"
 
 
 @pytest.mark.parametrize("text", ["lattice=hex\n", "Lx=abc\n", "nosuchkey=1\n", "precision=fp16\n",
-                                  "sweep_order=backward\n", "green_rebuild=centered\n", "green_rebuild=lu\n",
+                                  "sweep_order=backward\n", "green_rebuild=centered\nprecision=mixed_fp32_chain\n",
+                                  "green_rebuild=lu\n",
                                   "dtau=-1\n", "beta_list=\n", "Lx\n"])
 def test_params_rejects(host, tmp_path, text):
     rc, _ = read(host, tmp_path, text)
