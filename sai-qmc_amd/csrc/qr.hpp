@@ -369,10 +369,11 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
         int bigc = 0;
         // column c2 <- column c2 - w v_c, w = tau (pcc + sc v_c^T x_below), v_c = e_c + sc x;
         // for a big column x holds sc x already (dsum = v_c^T x_below)
+        double scd = 0.0; // sc, or 1 for a big column whose x is scaled already
         auto apply = [&](double (&col)[MR], double dsum) {
             const double pcc = bcast_lane(col[0], cl);
-            const double w = tau * (pcc + (bigc ? dsum : sc * dsum));
-            const double ws = bigc ? w : w * sc;
+            const double w = tau * (pcc + scd * dsum);
+            const double ws = w * scd;
 #pragma unroll
             for (int s = 0; s < MR; s++) col[s] -= ws * x[s];
             if (lane == cl) col[0] -= w;
@@ -399,6 +400,7 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
                     tau = s_tau[c];
                     sc = s_sc[c];
                     bigc = __builtin_amdgcn_readfirstlane(s_big[c]);
+                    scd = bigc ? 1.0 : sc;
                     if (bigc) {
                         scale_x();
                         acc = 0.0;
@@ -417,6 +419,7 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
             tau = s_tau[c];
             sc = s_sc[c];
             bigc = __builtin_amdgcn_readfirstlane(s_big[c]);
+            scd = bigc ? 1.0 : sc;
             if (bigc) scale_x();
         }
         if (tau != 0.0) {
