@@ -51,6 +51,30 @@ template <int CTRL, int RMASK> __device__ __forceinline__ float dpp_move(float x
 {
     return __int_as_float(dpp_move_i<CTRL, RMASK>(__float_as_int(x)));
 }
+// lane l of every 16-lane row to the whole row (gfx950 DPP row_newbcast);
+// l folds to a constant in unrolled code
+__device__ __forceinline__ double row_bcast(double v, int l)
+{
+    switch (l & 15) {
+    case 0: return dpp_move<0x150, 0xF>(v);
+    case 1: return dpp_move<0x151, 0xF>(v);
+    case 2: return dpp_move<0x152, 0xF>(v);
+    case 3: return dpp_move<0x153, 0xF>(v);
+    case 4: return dpp_move<0x154, 0xF>(v);
+    case 5: return dpp_move<0x155, 0xF>(v);
+    case 6: return dpp_move<0x156, 0xF>(v);
+    case 7: return dpp_move<0x157, 0xF>(v);
+    case 8: return dpp_move<0x158, 0xF>(v);
+    case 9: return dpp_move<0x159, 0xF>(v);
+    case 10: return dpp_move<0x15A, 0xF>(v);
+    case 11: return dpp_move<0x15B, 0xF>(v);
+    case 12: return dpp_move<0x15C, 0xF>(v);
+    case 13: return dpp_move<0x15D, 0xF>(v);
+    case 14: return dpp_move<0x15E, 0xF>(v);
+    default: return dpp_move<0x15F, 0xF>(v);
+    }
+}
+
 __device__ __forceinline__ double lane63(double v)
 {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);

@@ -974,35 +974,51 @@ __global__ __launch_bounds__(256) void k_trsm_unit_upper(TrsmArgs a)
             Ud[i * (TS_NB + 1) + k] = (i < ib && k < ib) ? ud[u] : 0.0;
         }
         __syncthreads();
-        if (wave == 0 && lane < TS_NB) {
-            // column `lane`, right-looking: once x_i = s_i is final it is
-            // removed from every s_k, k < i, so the dependent chain is one
-            // FMA per row
-            double *col = Xs + lane * ldx + r0;
-            double sv[TS_NB];
+        {
+            // the unit solve of the 32 columns: every 16-lane row takes one
+            // column (4 per wave, two passes over the 32), lane l of the row
+            // holds rows l and 16 + l; right-looking, so once x_i is final
+            // it is broadcast along the row (DPP) and removed from the rows
+            // still open, with each entry updated in the same order as a
+            // column-per-lane substitution
+            const int rl = lane & 15;
 #pragma unroll
-            for (int i = 0; i < TS_NB; i++) sv[i] = col[min(i, ib - 1)];
-            if (!tr) {
+            for (int pass = 0; pass < 2; pass++) {
+                const int c = pass * 16 + wave * 4 + (lane >> 4);
+                double *col = Xs + c * ldx + r0;
+                double sv[2];
 #pragma unroll
-                for (int i = TS_NB - 1; i >= 1; i--) {
-                    if (i < ib) {
+                for (int h = 0; h < 2; h++) sv[h] = col[min(16 * h + rl, ib - 1)];
+                if (!tr) {
 #pragma unroll
-                        for (int k = 0; k < i; k++) sv[k] -= Ud[k * (TS_NB + 1) + i] * sv[i];
+                    for (int i = TS_NB - 1; i >= 1; i--) {
+                        if (i < ib) {
+                            const double xi = row_bcast(sv[i >> 4], i);
+#pragma unroll
+                            for (int h = 0; h < 2; h++) {
+                                const int k = 16 * h + rl;
+                                if (k < i) sv[h] -= Ud[k * (TS_NB + 1) + i] * xi;
+                            }
+                        }
+                    }
+                } else {
+                    // Ud holds (U^T)_II, unit lower: forward
+#pragma unroll
+                    for (int i = 0; i < TS_NB - 1; i++) {
+                        if (i < ib) {
+                            const double xi = row_bcast(sv[i >> 4], i);
+#pragma unroll
+                            for (int h = 0; h < 2; h++) {
+                                const int k = 16 * h + rl;
+                                if (k > i) sv[h] -= Ud[k * (TS_NB + 1) + i] * xi;
+                            }
+                        }
                     }
                 }
-            } else {
-                // Ud holds (U^T)_II, unit lower: forward, right-looking
 #pragma unroll
-                for (int i = 0; i < TS_NB - 1; i++) {
-                    if (i < ib) {
-#pragma unroll
-                        for (int k = i + 1; k < TS_NB; k++) sv[k] -= Ud[k * (TS_NB + 1) + i] * sv[i];
-                    }
-                }
+                for (int h = 0; h < 2; h++)
+                    if (16 * h + rl < ib && c < TS_NB) col[16 * h + rl] = sv[h];
             }
-#pragma unroll
-            for (int i = 0; i < TS_NB; i++)
-                if (i < ib) col[i] = sv[i];
         }
         __syncthreads();
     }
