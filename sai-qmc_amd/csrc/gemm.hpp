@@ -543,26 +543,39 @@ __global__ __launch_bounds__(256, 2) void k_gemm_dl(GemmArgs<double> g)
             const int j = n0 + wc * 64 + jt * 16 + Mfma<double>::row(lane, v);
             sc[jt][v] = g.cc.fix(g.cc.raw(b, j)) * g.bc.fix(g.bc.raw(b, j));
         }
-#pragma unroll
-    for (int it = 0; it < 4; it++) {
+    auto store_block = [&](int it, int jt, const double (&old)[4], bool with_old) {
         const int i = m0 + wr * 64 + it * 16 + lc;
 #pragma unroll
-        for (int jt = 0; jt < 4; jt++) {
-            double old[4] = {0.0, 0.0, 0.0, 0.0};
-            if (g.beta != 0.0) {
-#pragma unroll
-                for (int v = 0; v < 4; v++)
-                    old[v] = C[i + (long long)(n0 + wc * 64 + jt * 16 + Mfma<double>::row(lane, v)) * g.ldc];
-            }
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int j = n0 + wc * 64 + jt * 16 + Mfma<double>::row(lane, v);
-                double r = g.alpha * acc[it][jt][v] * sr[it] * sc[jt][v];
-                if (g.dg.p && i == j) r += g.dg.at(b, i);
-                if (g.beta != 0.0) r += g.beta * old[v];
-                C[i + (long long)j * g.ldc] = r;
-            }
+        for (int v = 0; v < 4; v++) {
+            const int j = n0 + wc * 64 + jt * 16 + Mfma<double>::row(lane, v);
+            double r = g.alpha * acc[it][jt][v] * sr[it] * sc[jt][v];
+            if (g.dg.p && i == j) r += g.dg.at(b, i);
+            if (with_old) r += g.beta * old[v];
+            C[i + (long long)j * g.ldc] = r;
         }
+    };
+    if (g.beta != 0.0) {
+        // C += ...: the old values of the next 16 x 16 block are loaded
+        // while this block is combined and stored
+        auto load_block = [&](int it, int jt, double (&old)[4]) {
+            const int i = m0 + wr * 64 + it * 16 + lc;
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                old[v] = C[i + (long long)(n0 + wc * 64 + jt * 16 + Mfma<double>::row(lane, v)) * g.ldc];
+        };
+        double old[2][4];
+        load_block(0, 0, old[0]);
+#pragma unroll
+        for (int blk = 0; blk < 16; blk++) {
+            if (blk + 1 < 16) load_block((blk + 1) >> 2, (blk + 1) & 3, old[(blk + 1) & 1]);
+            store_block(blk >> 2, blk & 3, old[blk & 1], true);
+        }
+    } else {
+        const double none[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int it = 0; it < 4; it++)
+#pragma unroll
+            for (int jt = 0; jt < 4; jt++) store_block(it, jt, none, false);
     }
 }
 
