@@ -29,6 +29,8 @@
 #pragma once
 #include "common.hpp"
 
+#define METRO_THREADS 512
+
 template <typename T> struct MetroArgs {
     int n, L, W, l, batch;
     int s0, s1;         // sites [s0, s1) of the slice in this launch
@@ -51,7 +53,6 @@ template <typename T> struct MetroArgs {
     int *pcount;        // pending pair count per replica (the flush GEMM's K)
 };
 
-#define METRO_THREADS 512
 #define METRO_MAX_N 512
 
 // LDS: corrected columns and rows (2 x W x (n+1)), the window slices of C
@@ -80,19 +81,17 @@ __device__ __forceinline__ double bcast_any(double v, int lane)
 //   Sc[w][j] = G[j][i0+w] - sum_q C[q][j] V[q][i0+w]
 //   Sr[w][j] = G[i0+w][j] - sum_q C[q][i0+w] V[q][j]
 // Sc, Sr are W x (n+1) in LDS; Vw[q*W + w] = V[q][i0+w] and Cw[q*W + w] =
-// C[q][i0+w] are the window slices staged in LDS.  Each wave takes two
+// C[q][i0+w] are the window slices staged in LDS.  Each wave takes JT
 // 16-row blocks of j for both outputs at once, streaming C and V from
-// global memory eight k-steps (32 loads per lane) at a time.  Each
-// accumulator row of a tile
-// covers 16 consecutive rows of one column of G, so the G loads are
-// 128-byte segments.
-template <typename T, int W>
+// global memory UB k-steps (4*JT*UB loads per lane) at a time; the kernel
+// runs JT = 2, UB = 8 on 8 waves.  Each accumulator row of a tile covers 16
+// consecutive rows of one column of G, so the G loads are 128-byte segments.
+template <typename T, int W, int JT, int UB>
 __device__ void window_correct(const T *G, int n, int i0, int wn, const T *C, const T *V, int P, const T *Cw,
                                const T *Vw, T *Sc, T *Sr)
 {
     typedef typename Mfma<T>::acc_t acc_t;
     constexpr int NTW = (W + 15) / 16;
-    constexpr int JT = 2; // j blocks per wave per pass
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int lc = lane & 15, lr = lane >> 4;
     const int ntj = (n + 15) >> 4;
@@ -135,15 +134,15 @@ __device__ void window_correct(const T *G, int n, int i0, int wn, const T *C, co
                     ar[t][tw][v] = (w0 + lc < wn && j0[t] + rr < n) ? ar[t][tw][v] : T(0);
                 }
             }
-        // big operands C[q][j0+lc] (cols) and V[q][j0+lc] (rows): 8 k-steps of
-        // both j blocks (32 loads per lane) in flight per batch; the second
-        // wave on the SIMD computes while this one waits
-        for (int q0 = 0; q0 < P; q0 += 32) {
-            T cc[JT][8], cv[JT][8];
+        // big operands C[q][j0+lc] (cols) and V[q][j0+lc] (rows): UB k-steps
+        // of the JT j blocks in flight per batch; another wave on the SIMD
+        // computes while this one waits
+        for (int q0 = 0; q0 < P; q0 += 4 * UB) {
+            T cc[JT][UB], cv[JT][UB];
             // loads from clamped (always valid) addresses, zeroed afterwards:
             // no divergent branch between them
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
+            for (int u = 0; u < UB; u++) {
                 const int q = min(q0 + 4 * u + lr, P - 1);
 #pragma unroll
                 for (int t = 0; t < JT; t++) {
@@ -153,7 +152,7 @@ __device__ void window_correct(const T *G, int n, int i0, int wn, const T *C, co
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
+            for (int u = 0; u < UB; u++) {
                 const bool qok = q0 + 4 * u + lr < P;
 #pragma unroll
                 for (int t = 0; t < JT; t++) {
@@ -162,7 +161,7 @@ __device__ void window_correct(const T *G, int n, int i0, int wn, const T *C, co
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
+            for (int u = 0; u < UB; u++) {
                 const int q = q0 + 4 * u + lr;
 #pragma unroll
                 for (int tw = 0; tw < NTW; tw++) {
@@ -200,6 +199,7 @@ template <typename T, int W>
 __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
 {
     static_assert(W == 8 || W == 16 || W == 32, "window of 8, 16 or 32 sites");
+    constexpr int TH = METRO_THREADS;
     constexpr int RPL = W * W / 64; // rows of the window block per lane of wave 0
     const int b = blockIdx.x;
     if (b >= a.batch) return;
@@ -246,14 +246,29 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
     T gc[W];
     for (int i0 = a.s0; i0 < a.s1; i0 += W) {
         const int wn = min(W, a.s1 - i0);
-        // 1. corrected window columns -> Sc, rows -> Sr, the window block -> Bw
-        for (int idx = threadIdx.x; idx < P * W; idx += blockDim.x) {
-            const int q = idx / W, w = idx % W;
-            Cw[idx] = w < wn ? Cp[(long long)q * n + i0 + w] : T(0);
-            Vw[idx] = w < wn ? Vp[(long long)q * n + i0 + w] : T(0);
+        // 1. corrected window columns -> Sc, rows -> Sr, the window block -> Bw.
+        // The window slices of C and V are staged four entries per thread
+        // at a time, all loads issued before the first LDS store.
+        for (int idx0 = threadIdx.x; idx0 < P * W; idx0 += 4 * TH) {
+            T xc[4], xv[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int idx = min(idx0 + k * TH, P * W - 1), q = idx / W, w = min(idx % W, wn - 1);
+                xc[k] = Cp[(long long)q * n + i0 + w];
+                xv[k] = Vp[(long long)q * n + i0 + w];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int idx = idx0 + k * TH;
+                if (idx < P * W) {
+                    const bool ok = idx % W < wn;
+                    Cw[idx] = ok ? xc[k] : T(0);
+                    Vw[idx] = ok ? xv[k] : T(0);
+                }
+            }
         }
         __syncthreads();
-        window_correct<T, W>(G, n, i0, wn, Cp, Vp, P, Cw, Vw, Sc, Sr);
+        window_correct<T, W, 2, 8>(G, n, i0, wn, Cp, Vp, P, Cw, Vw, Sc, Sr);
         __syncthreads();
         for (int idx = threadIdx.x; idx < W * W; idx += blockDim.x) {
             const int r = idx / W, c = idx % W;
