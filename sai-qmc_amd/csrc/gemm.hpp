@@ -483,13 +483,16 @@ __global__ __launch_bounds__(256, 2) void k_gemm_dl(GemmArgs<double> g)
             }
         }
     };
-    // k-scales of this lane's four k rows of a tile (0 past K)
+    // k-scales of this lane's four k rows of a tile (0 past K): the raw
+    // entries are loaded with the tile, one k-step ahead, and combined
+    // once the wait for the tile has covered them
     const bool ksc = g.ac.p || g.br.p || (K % GB_K) != 0;
-    auto kscale = [&](int k0, double (&sk)[4]) {
+    auto kraw = [&](int k0, double (&ra)[4], double (&rb)[4]) {
 #pragma unroll
         for (int kk = 0; kk < 4; kk++) {
-            const int k = k0 + 4 * kk + lr, kc = min(k, K - 1);
-            sk[kk] = k < K ? g.ac.fix(g.ac.raw(b, kc)) * g.br.fix(g.br.raw(b, kc)) : 0.0;
+            const int kc = min(k0 + 4 * kk + lr, K - 1);
+            ra[kk] = g.ac.raw(b, kc);
+            rb[kk] = g.br.raw(b, kc);
         }
     };
 
@@ -499,14 +502,27 @@ __global__ __launch_bounds__(256, 2) void k_gemm_dl(GemmArgs<double> g)
 #pragma unroll
         for (int jt = 0; jt < 4; jt++) acc[it][jt] = f64x4{0, 0, 0, 0};
 
-    if (K > 0) issue(0, 0);
+    double ra[4] = {1.0, 1.0, 1.0, 1.0}, rb[4] = {1.0, 1.0, 1.0, 1.0};
+    if (K > 0) {
+        if (ksc) kraw(0, ra, rb);
+        issue(0, 0);
+    }
     int u = 0;
     for (int k0 = 0; k0 < K; k0 += GB_K) {
-        double sk[4] = {1.0, 1.0, 1.0, 1.0};
-        if (ksc) kscale(k0, sk);
-        __builtin_amdgcn_s_waitcnt(0); // this wave's tile loads have landed
+        __builtin_amdgcn_s_waitcnt(0); // this wave's tile loads (and k-scales) have landed
         __syncthreads();               // ... and everyone's; the other buffer is free
-        if (k0 + GB_K < K) issue(u ^ 1, k0 + GB_K);
+        double sk[4] = {1.0, 1.0, 1.0, 1.0};
+        if (ksc) {
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) {
+                const int k = k0 + 4 * kk + lr;
+                sk[kk] = k < K ? g.ac.fix(ra[kk]) * g.br.fix(rb[kk]) : 0.0;
+            }
+        }
+        if (k0 + GB_K < K) {
+            if (ksc) kraw(k0 + GB_K, ra, rb);
+            issue(u ^ 1, k0 + GB_K);
+        }
         const double *As = lds + u * GD_BUF;
         const double *Bs = As + GD_ABUF;
 #pragma unroll
