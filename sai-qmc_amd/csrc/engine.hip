@@ -898,6 +898,11 @@ struct Model {
     double *AU = nullptr, *AD = nullptr, *AT = nullptr;
     int *Aq = nullptr;
     Work w;
+    // the suffix stack of a forward sweep is built on w2's stream while
+    // the main stream runs block 0 (fork at the sweep start, join before
+    // the first boundary rebuild)
+    Work w2;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool allocated = false;
     // one captured graph per sweep variant (SW_*)
     hipGraph_t graph[5] = {};
@@ -992,6 +997,16 @@ static int model_alloc(dqmc_gpu_ctx *ctx, Model &m)
     const int mixed = ctx->opts.precision != DQMC_GPU_FP64;
     HIPCHK(hipStreamCreateWithFlags(&m.w.s, hipStreamNonBlocking));
     if (m.w.alloc(n, R, mixed)) return 1;
+    const char *ov = getenv("DQMC_GPU_OVERLAP");
+    if (!(ov && ov[0] == '0')) {
+        HIPCHK(hipStreamCreateWithFlags(&m.w2.s, hipStreamNonBlocking));
+        if (m.w2.alloc(n, R, 0)) return 1;
+        // a replica that fails on either stream is skipped on both
+        m.w2.status = m.w.status;
+        m.w2.reason = m.w.reason;
+        HIPCHK(hipEventCreateWithFlags(&m.ev_fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&m.ev_join, hipEventDisableTiming));
+    }
     auto own = [&](void *p) {
         if (p) m.owned.push_back(p);
     };
@@ -1347,9 +1362,9 @@ static int stage_lmul(dqmc_gpu_ctx *ctx, Model &m, const Udv &in, const Udv &out
 
 // S[j] = S[j+1] B_{b_{j+1}-1} ... B_{b_j}   (green_stack_build_suffix +
 // udv_rmul, src/green.c:203-226, src/linalg.c:662-784)
-static int stage_rmul(dqmc_gpu_ctx *ctx, Model &m, int j)
+static int stage_rmul(dqmc_gpu_ctx *ctx, Model &m, int j, Work &w)
 {
-    ProfScope ps_(DQMC_GPU_PROF_STACK, m.w.s);
+    ProfScope ps_(DQMC_GPU_PROF_STACK, w.s);
     const int n = m.n, R = m.nrep;
     const long long nn = (long long)n * n, sV = (long long)m.L * n;
     const int bj = std::min(j * m.stab, m.L), bj1 = std::min((j + 1) * m.stab, m.L);
@@ -1371,40 +1386,40 @@ static int stage_rmul(dqmc_gpu_ctx *ctx, Model &m, int j)
     double *cur;
     if (chain_f32(ctx)) {
         float *Bf = bblock_f32(m, bj, bj1 - bj);
-        hipLaunchKernelGGL(k_f2d, grid2(nn, R), dim3(256), 0, m.w.s, Bf, nn, m.w.W3, nn, nn, R);
-        GemmArgs<double> g = gemm_args<double>(n, n, n, R, s1.T, n, s1.sU, 0, m.w.W3, n, nn, 0, m.w.W1, n, nn);
+        hipLaunchKernelGGL(k_f2d, grid2(nn, R), dim3(256), 0, w.s, Bf, nn, w.W3, nn, nn, R);
+        GemmArgs<double> g = gemm_args<double>(n, n, n, R, s1.T, n, s1.sU, 0, w.W3, n, nn, 0, w.W1, n, nn);
         g.ar = vec(Dsrc, sDsrc);
-        if (gemm(m.w, g)) return 1;
-        cur = m.w.W1;
+        if (gemm(w, g)) return 1;
+        cur = w.W1;
     } else if (m.BB && m.bb_valid) {
         // diag(D) T Bblock_j with the block product the left chain formed
         GemmArgs<double> g = gemm_args<double>(n, n, n, R, s1.T, n, s1.sU, 0, m.BB + (long long)j * R * nn, n, nn, 0,
-                                               m.w.W1, n, nn);
+                                               w.W1, n, nn);
         g.ar = vec(Dsrc, sDsrc);
-        if (gemm(m.w, g)) return 1;
-        cur = m.w.W1;
+        if (gemm(w, g)) return 1;
+        cur = w.W1;
     } else {
         // diag(D) T B_top ... B_bj, B applied one slice at a time from the right
         const int top = bj1 - 1;
-        GemmArgs<double> g = gemm_args<double>(n, n, n, R, s1.T, n, s1.sU, 0, m.expK, n, 0, 0, m.w.W1, n, nn);
+        GemmArgs<double> g = gemm_args<double>(n, n, n, R, s1.T, n, s1.sU, 0, m.expK, n, 0, 0, w.W1, n, nn);
         g.ar = vec(Dsrc, sDsrc);
         g.cc = vec<double>(m.dvec + (long long)top * n, sV);
-        if (gemm(m.w, g)) return 1;
-        cur = m.w.W1;
-        double *oth = m.w.W2;
+        if (gemm(w, g)) return 1;
+        cur = w.W1;
+        double *oth = w.W2;
         for (int l = top - 1; l >= bj; l--) {
             g = gemm_args<double>(n, n, n, R, cur, n, nn, 0, m.expK, n, 0, 0, oth, n, nn);
             g.cc = vec<double>(m.dvec + (long long)l * n, sV);
-            if (gemm(m.w, g)) return 1;
+            if (gemm(w, g)) return 1;
             std::swap(cur, oth);
         }
     }
-    double *Qb = (cur == m.w.W1) ? m.w.W2 : m.w.W1;
-    if (qr(m.w, cur, Qb, nn, s0.T, s0.sU, s0.D, s0.sD, m.w.I1, 1)) return 1;
+    double *Qb = (cur == w.W1) ? w.W2 : w.W1;
+    if (qr(w, cur, Qb, nn, s0.T, s0.sU, s0.D, s0.sD, w.I1, 1)) return 1;
     GemmArgs<double> g = gemm_args<double>(n, n, n, R, s1.U, n, s1.sU, 0, Qb, n, nn, 0, s0.U, n, s0.sU);
-    if (gemm(m.w, g)) return 1;
-    hipLaunchKernelGGL(k_mul_q, dim3((R + 255) / 256), dim3(256), 0, m.w.s, s1.q, s1.sq, m.w.I1, 1LL, s0.q,
-                       s0.sq, R, m.w.status);
+    if (gemm(w, g)) return 1;
+    hipLaunchKernelGGL(k_mul_q, dim3((R + 255) / 256), dim3(256), 0, w.s, s1.q, s1.sq, w.I1, 1LL, s0.q,
+                       s0.sq, R, w.status);
     if (launch_check("k_mul_q")) return 1;
     return m.centered ? udv_recenter(m, s0, s0) : 0;
 }
@@ -1511,9 +1526,25 @@ static void sweep_advance(Model &m, int v)
 static int sweep_forward(dqmc_gpu_ctx *ctx, Model &m, bool build_suffix, bool carry)
 {
     const long long nn = (long long)m.n * m.n;
-    if (build_suffix)
+    // The suffix stack S[M-1] .. S[1] reads only the cached block products
+    // Bblock_j, j >= 1, and S[M], and nothing of it is read before the first
+    // boundary rebuild; block 0's Metropolis, wraps and left update touch
+    // G, the fields, the left UDV and Bblock_0 only.  So the stack goes to
+    // the second stream (its own scratch) and runs beside block 0.  Not in
+    // centered or FP32-chain modes (their helpers use the main scratch), nor
+    // in a profiled sweep, whose per-stage events assume one stream.
+    const bool ovl = build_suffix && !carry && m.w2.s && !m.centered && !chain_f32(ctx) && m.BB && m.bb_valid &&
+                     m.M >= 2 && g_prof == nullptr;
+    if (build_suffix) {
+        if (ovl) {
+            HIPCHK(hipEventRecord(m.ev_fork, m.w.s));
+            HIPCHK(hipStreamWaitEvent(m.w2.s, m.ev_fork, 0));
+        }
         for (int j = m.M - 1; j >= 1; j--)
-            if (stage_rmul(ctx, m, j)) return 1;
+            if (stage_rmul(ctx, m, j, ovl ? m.w2 : m.w)) return 1;
+        if (ovl) HIPCHK(hipEventRecord(m.ev_join, m.w2.s));
+    }
+    bool joined = !ovl;
     const Udv lf = m.left();
     if (!carry && udv_identity(m, lf)) return 1;
     int tau_left = 0;
@@ -1525,9 +1556,14 @@ static int sweep_forward(dqmc_gpu_ctx *ctx, Model &m, bool build_suffix, bool ca
             const Udv cur = carry ? m.alt(j) : lf;
             if (stage_lmul(ctx, m, carry ? m.alt(j - 1) : lf, cur, tau_left, tau - tau_left)) return 1;
             tau_left = tau;
+            if (!joined) {
+                HIPCHK(hipStreamWaitEvent(m.w.s, m.ev_join, 0));
+                joined = true;
+            }
             if (stage_boundary_rebuild(ctx, m, cur, m.stack(j), tau)) return 1;
         }
     }
+    if (!joined) HIPCHK(hipStreamWaitEvent(m.w.s, m.ev_join, 0));
     // tau = L: the whole chain B_{L-1} ... B_0 (A[M] in the alternating mode,
     // src/dqmc.c:769-771)
     const Udv fin = carry ? m.alt(m.M) : lf;
@@ -1549,7 +1585,7 @@ static int stage_suffix_block(dqmc_gpu_ctx *ctx, Model &m, int j)
         ProfScope ps_(DQMC_GPU_PROF_STACK, m.w.s);
         if (bblock_build(m, lb, std::min(m.stab, m.L - lb), m.BB + (long long)j * m.nrep * nn)) return 1;
     }
-    return stage_rmul(ctx, m, j);
+    return stage_rmul(ctx, m, j, m.w);
 }
 
 // One backward sweep (dqmc_sweep_backward + dqmc_backward_boundary,
@@ -2237,8 +2273,13 @@ void dqmc_gpu_destroy(dqmc_gpu_ctx *ctx)
             if (m->graph[v]) (void)hipGraphDestroy(m->graph[v]);
         }
         for (void *p : m->owned) (void)hipFree(p);
+        if (m->w2.s) (void)hipStreamSynchronize(m->w2.s);
         m->w.release();
         if (m->w.s) (void)hipStreamDestroy(m->w.s);
+        m->w2.release();
+        if (m->w2.s) (void)hipStreamDestroy(m->w2.s);
+        if (m->ev_fork) (void)hipEventDestroy(m->ev_fork);
+        if (m->ev_join) (void)hipEventDestroy(m->ev_join);
         delete m;
     }
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
