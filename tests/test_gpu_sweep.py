@@ -155,3 +155,34 @@ def test_rebuild_modes_trajectory(eng_mod, case):
         err = np.abs(eng.green(0) - ref).max() / np.abs(ref).max()
         assert err < 1e-9, (s, err)
     eng.close()
+
+
+def test_two_stream_schedule_bit_identical(eng_mod, monkeypatch):
+    # A forward sweep builds its suffix stack on a second stream beside
+    # block 0 and joins before the first boundary rebuild.  The schedule
+    # must not change a bit: fields, accept counts and G after every sweep
+    # equal those of the one-stream schedule (DQMC_GPU_OVERLAP=0).
+    kind, Lx, Ly, L, stab, k, nsw = "square", 16, 16, 40, 5, 16, 3
+    seeds = list(range(1, 17))
+    runs = []
+    for ov in ("1", "0"):
+        monkeypatch.setenv("DQMC_GPU_OVERLAP", ov)
+        print(f"two-stream schedule check, DQMC_GPU_OVERLAP={ov}", flush=True)
+        eng = eng_mod.Engine(eng_mod.FP64)
+        mid = eng.add_model(kind, Lx, Ly, 4.0, 0.1, L, stab, delay_k=k)
+        eng.add_replicas(mid, seeds)
+        eng.init()
+        traj = []
+        for _ in range(nsw):
+            eng.sweep(1)
+            st = eng.status()
+            assert (st["status"] == 0).all()
+            traj.append(([eng.field(r).copy() for r in range(len(seeds))], st["accepted"].copy(),
+                         [eng.green(r).copy() for r in range(len(seeds))]))
+        eng.close()
+        runs.append(traj)
+    for s, (a, b) in enumerate(zip(*runs)):
+        assert np.array_equal(a[1], b[1]), f"accept counts differ after sweep {s}"
+        for r in range(len(seeds)):
+            assert np.array_equal(a[0][r], b[0][r]), f"fields differ: sweep {s} replica {r}"
+            assert np.array_equal(a[2][r], b[2][r]), f"G differs: sweep {s} replica {r}"
