@@ -31,6 +31,7 @@ sys.path.insert(0, os.path.join(ROOT, "sai-qmc_amd"))
 METRIC = "replica-sweeps/s, 16x16 U=4 β=8: mixed-FP32 vs FP64 GPU at G drift ≤1e-6"
 LX, LY, U, DTAU, BETA, STAB, DELAY, NREP = 16, 16, 4.0, 0.1, 8.0, 8, 32, 128
 P32, P64 = 157.3e12, 78.6e12  # MI355X dense FP32 / FP64 matrix peaks
+HBM = 8.0e12  # MI355X HBM3E bytes/s
 
 
 def run_engine(dq, precision, steps, warmup, drift, groups=1, delay=DELAY, order="forward"):
@@ -76,11 +77,31 @@ def traffic_from_profile(kernel):
     return None
 
 
+def roofline_metro_kernel(prof, acc, mixed, nrep_launch=NREP):
+    """Roofline of the dominant kernel by total time, the Metropolis kernel
+    (one launch per half-slice site segment).  Algorithmic bytes per replica
+    and launch: the window columns and rows of G it corrects, n^2 values for
+    the n/2 sites of a segment, plus the pending pairs it writes, a n/2
+    pairs of 2n values (a = acceptance).  Launch duration from HIP events of
+    a profiled sweep: the Metropolis stage minus its flush GEMMs, over the
+    number of flushes (one per launch)."""
+    n = LX * LY
+    ms, _ = prof["metro"]
+    fms, fcnt = prof["flush"]
+    t = (ms - fms) / fcnt / 1e3
+    w = 4 if mixed else 8
+    byts = (n * n + acc * (n // 2) * 2 * n) * w * nrep_launch
+    return {"bound": "hbm", "kernel": "k_metro_slice<float, 32>" if mixed else "k_metro_slice<double, 16>",
+            "achieved": round(byts / t / 1e9, 1), "peak": round(HBM / 1e9, 1), "unit": "GB/s",
+            "frac": round(byts / t / HBM, 4),
+            "traffic": None if mixed else traffic_from_profile("k_metro_slice"),
+            "avg_launch_ms": round(t * 1e3, 4), "launches_per_sweep": fcnt, "bytes_per_launch": round(byts)}
+
+
 def roofline_gemm(prof, mixed, nrep_launch=NREP):
-    """Roofline of the dominant kernel by total time: the batched n x n x n
-    FP64 GEMM (k_gemm_dl; the few transposed ones run on k_gemm_big), 2 n^3
-    flops per replica per launch, average launch duration from HIP events
-    around every such launch of a profiled sweep."""
+    """The batched n x n x n FP64 GEMM (k_gemm_dl; the few transposed ones
+    run on k_gemm_big), 2 n^3 flops per replica per launch, average launch
+    duration from HIP events around every such launch of a profiled sweep."""
     n = LX * LY
     ms, cnt = prof["gemm"]
     t = ms / cnt / 1e3
@@ -203,7 +224,9 @@ def main():
         "drift_max": main_run["drift_max"],
         "drift_mean": main_run["drift_mean"],
         "acceptance": round(main_run["acc"], 4),
-        "roofline": roofline_gemm(main_run["profile"], main_name != "fp64", NREP // a.groups),
+        "roofline": roofline_metro_kernel(main_run["profile"], main_run["acc"], main_name != "fp64",
+                                          NREP // a.groups),
+        "roofline_gemm": roofline_gemm(main_run["profile"], main_name != "fp64", NREP // a.groups),
         "roofline_metro": roofline_metro(main_run["profile"], main_run["acc"], NREP // a.groups),
         "profile_ms_per_sweep": {k: round(v[0], 3) for k, v in main_run["profile"].items()},
     }

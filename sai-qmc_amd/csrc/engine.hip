@@ -20,6 +20,7 @@
 #include "../../include/dqmc_gpu.h"
 #include "common.hpp"
 #include "gemm.hpp"
+#include "kron.hpp"
 #include "measure.hpp"
 #include "metro.hpp"
 #include "qr.hpp"
@@ -848,6 +849,11 @@ struct Model {
     double U = 0, dtau = 0, mu = 0, lambda = 0;
     double expv[2], expvinv[2], nu[2], nd[2];
     std::vector<double> h_expK, h_expKinv, h_thop, h_phase;
+    // expK = E_y (x) E_x and expKinv = F_y (x) F_x on a 16 x 16 lattice:
+    // factor pairs (E_x, E_y), (E_x^T, E_y^T), (F_x, F_y), (F_x^T, F_y^T),
+    // 16 x 16 row-major each; empty when expK does not split
+    std::vector<double> h_kron;
+    double *kronF = nullptr;
     std::vector<int> h_bipart;
     std::vector<unsigned long long> seeds;
     std::vector<signed char> h_fields;
@@ -1073,6 +1079,11 @@ static int model_alloc(dqmc_gpu_ctx *ctx, Model &m)
     // uploads
     HIPCHK(hipMemcpy(m.expK, m.h_expK.data(), nn * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(m.expKinv, m.h_expKinv.data(), nn * 8, hipMemcpyHostToDevice));
+    if (!m.h_kron.empty()) {
+        if (dalloc(&m.kronF, m.h_kron.size())) return 1;
+        m.owned.push_back(m.kronF);
+        HIPCHK(hipMemcpy(m.kronF, m.h_kron.data(), m.h_kron.size() * 8, hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMemcpy(m.thop, m.h_thop.data(), nn * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(m.bipart, m.h_bipart.data(), n * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(m.field, m.h_fields.data(), (size_t)L * n * R, hipMemcpyHostToDevice));
@@ -1206,6 +1217,30 @@ static int stage_metro(dqmc_gpu_ctx *ctx, Model &m, int l)
     return 0;
 }
 
+// out = (A_y (x) A_x) applied to the columns (rows = 0) or the rows
+// (rows = 1) of in, with the scales of k_kron16; f selects the factor pair:
+// 0 expK from the left, 1 expK from the right, 2 expKinv from the left,
+// 3 expKinv from the right
+static int kron(Model &m, Work &w, int rows, int f, const double *in, double *out, Vec<double> pre,
+                Vec<double> post, Vec<double> oth)
+{
+    KronArgs a;
+    a.batch = m.nrep;
+    a.rows = rows;
+    a.in = in;
+    a.sIn = (long long)m.n * m.n;
+    a.out = out;
+    a.sOut = a.sIn;
+    a.ax = m.kronF + (size_t)f * 2 * KRON_N;
+    a.ay = a.ax + KRON_N;
+    a.pre = pre;
+    a.post = post;
+    a.oth = oth;
+    a.status = w.status;
+    hipLaunchKernelGGL(k_kron16, dim3(KRON_L * m.nrep), dim3(256), 0, w.s, a);
+    return launch_check("k_kron16");
+}
+
 // G <- B_l G B_l^-1 = expK diag(d_l) G diag(1/d_l) expKinv (src/green.c:509-527)
 static int stage_wrap(dqmc_gpu_ctx *ctx, Model &m, int l)
 {
@@ -1219,6 +1254,13 @@ static int stage_wrap(dqmc_gpu_ctx *ctx, Model &m, int l)
         if (gemm(m.w, g)) return 1;
         g = gemm_args<float>(n, n, n, R, m.w.F1, n, nn, 0, m.expKinvf, n, 0, 0, m.Gf, n, nn);
         return gemm(m.w, g);
+    }
+    if (m.kronF) {
+        // W1 = expK diag(d) G diag(1/d) column by column, then G = W1 expKinv row by row
+        if (kron(m, m.w, 0, 0, m.G, m.w.W1, vec<double>(m.dvec + (long long)l * n, sV), vnone<double>(),
+                 vec<double>(m.dinv + (long long)l * n, sV)))
+            return 1;
+        return kron(m, m.w, 1, 3, m.w.W1, m.G, vnone<double>(), vnone<double>(), vnone<double>());
     }
     GemmArgs<double> g = gemm_args<double>(n, n, n, R, m.expK, n, 0, 0, m.G, n, nn, 0, m.w.W1, n, nn);
     g.br = vec<double>(m.dvec + (long long)l * n, sV);
@@ -1260,9 +1302,16 @@ static int bblock_build(Model &m, int lb, int len, double *out)
                        bufs[cur], nn, n, R);
     if (launch_check("k_scale_cols_d")) return 1;
     for (int l = lb + 1; l < lb + len; l++) {
-        GemmArgs<double> g = gemm_args<double>(n, n, n, R, m.expK, n, 0, 0, bufs[cur], n, nn, 0, bufs[cur ^ 1], n, nn);
-        g.br = vec<double>(m.dvec + (long long)l * n, sV);
-        if (gemm(m.w, g)) return 1;
+        if (m.kronF) {
+            if (kron(m, m.w, 0, 0, bufs[cur], bufs[cur ^ 1], vec<double>(m.dvec + (long long)l * n, sV),
+                     vnone<double>(), vnone<double>()))
+                return 1;
+        } else {
+            GemmArgs<double> g =
+                gemm_args<double>(n, n, n, R, m.expK, n, 0, 0, bufs[cur], n, nn, 0, bufs[cur ^ 1], n, nn);
+            g.br = vec<double>(m.dvec + (long long)l * n, sV);
+            if (gemm(m.w, g)) return 1;
+        }
         cur ^= 1;
     }
     return 0;
@@ -1475,6 +1524,14 @@ static int stage_wrap_back(dqmc_gpu_ctx *ctx, Model &m, int l)
         g = gemm_args<float>(n, n, n, R, m.w.F1, n, nn, 0, m.expKf, n, 0, 0, m.Gf, n, nn);
         g.bc = vec<float>(m.dvecf + (long long)l * n, sV);
         return gemm(m.w, g);
+    }
+    if (m.kronF) {
+        // W1 = diag(1/d) expKinv G column by column, then G = W1 expK diag(d) row by row
+        if (kron(m, m.w, 0, 2, m.G, m.w.W1, vnone<double>(), vec<double>(m.dinv + (long long)l * n, sV),
+                 vnone<double>()))
+            return 1;
+        return kron(m, m.w, 1, 1, m.w.W1, m.G, vnone<double>(), vec<double>(m.dvec + (long long)l * n, sV),
+                    vnone<double>());
     }
     GemmArgs<double> g = gemm_args<double>(n, n, n, R, m.expKinv, n, 0, 0, m.G, n, nn, 0, m.w.W1, n, nn);
     g.ar = vec<double>(m.dinv + (long long)l * n, sV);
@@ -1776,6 +1833,30 @@ int dqmc_gpu_create(dqmc_gpu_ctx **out, int device, const dqmc_gpu_opts *o)
     return 0;
 }
 
+// M = F_y (x) F_x for a 16 x 16 lattice (site x + 16 y, column-major M):
+// F_x from the (0,0) block, F_y from the (0,0) entries of the blocks over
+// M[0][0]; kept when F_y (x) F_x reproduces M to 1e-12 of its largest entry
+static bool kron_split(const std::vector<double> &M, std::vector<double> &fx, std::vector<double> &fy)
+{
+    const int L = KRON_L, n = KRON_N;
+    if ((int)M.size() != n * n || !(M[0] != 0.0)) return false;
+    fx.assign(L * L, 0.0);
+    fy.assign(L * L, 0.0);
+    for (int r = 0; r < L; r++)
+        for (int c = 0; c < L; c++) {
+            fx[r * L + c] = M[r + (size_t)c * n];
+            fy[r * L + c] = M[L * r + (size_t)(L * c) * n] / M[0];
+        }
+    double mx = 0.0, err = 0.0;
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+            const double v = M[i + (size_t)j * n];
+            mx = std::max(mx, std::fabs(v));
+            err = std::max(err, std::fabs(fy[(i / L) * L + j / L] * fx[(i % L) * L + j % L] - v));
+        }
+    return std::isfinite(err) && err <= 1e-12 * mx;
+}
+
 int dqmc_gpu_add_model(dqmc_gpu_ctx *ctx, int n, int L, int stab, int delay_k, const double *expK,
                        const double *expKinv, const double *thop, const int *bipart, double U, double dtau,
                        double mu, double lambda, int Lx, int Ly, int *model_id)
@@ -1818,6 +1899,21 @@ int dqmc_gpu_add_model(dqmc_gpu_ctx *ctx, int n, int L, int stab, int delay_k, c
     m->nd[1] = std::exp(-2.0 * lambda * -1.0 * 1.0) - 1.0;
     m->h_expK.assign(expK, expK + nn);
     m->h_expKinv.assign(expKinv, expKinv + nn);
+    {
+        const char *e = getenv("DQMC_GPU_KRON");
+        std::vector<double> ex, ey, fx, fy;
+        if (!(e && e[0] == '0') && Lx == KRON_L && Ly == KRON_L && n == KRON_N && kron_split(m->h_expK, ex, ey) &&
+            kron_split(m->h_expKinv, fx, fy)) {
+            auto tr = [](const std::vector<double> &a) {
+                std::vector<double> t(a.size());
+                for (int r = 0; r < KRON_L; r++)
+                    for (int c = 0; c < KRON_L; c++) t[c * KRON_L + r] = a[r * KRON_L + c];
+                return t;
+            };
+            for (const std::vector<double> &f : {ex, ey, tr(ex), tr(ey), fx, fy, tr(fx), tr(fy)})
+                m->h_kron.insert(m->h_kron.end(), f.begin(), f.end());
+        }
+    }
     m->h_thop.assign(thop, thop + nn);
     m->h_bipart.assign(bipart, bipart + n);
     *model_id = (int)ctx->models.size();

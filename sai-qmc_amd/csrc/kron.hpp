@@ -1,0 +1,121 @@
+// B-matrix products on a 16 x 16 square lattice with separable hopping.
+//
+// With nearest-neighbour hopping along x and y, K = I_y (x) K_x + K_y (x) I_x
+// and the two terms commute, so expK = exp(-dtau K) = E_y (x) E_x exactly
+// (site i = x + 16 y): expK[(ix, iy), (jx, jy)] = E_y[iy][jy] E_x[ix][jx].
+// Applied to a vector z of n = 256 entries, viewed as the 16 x 16 grid
+// Z[ex][ey] = z[ex + 16 ey], (A_y (x) A_x) z is the grid A_x Z A_y^T: two
+// 16 x 16 x 16 products (8 FP64 MFMAs) instead of a 256-long dot product
+// per entry, 2 * 2 * 16 n flops per vector against 2 n^2.  The products
+// expK diag(d) X of the wrap and of the block products
+// (green_wrap_forward / green_wrap_backward / green_build_Bblock,
+// src/green.c:126-166, 509-546) become one pass over the columns (left
+// factor) or the rows (right factor, X M = (M^T X^T)^T) of X: HBM-bound,
+// X read and written once.
+//
+// k_kron16: one workgroup per 16 vectors (columns, or rows with `rows`) of
+// one replica.  The vectors are staged in LDS (pre-scaled), each wave
+// transforms four of them in registers:
+//   P^T = Z^T A_x^T   (A operand Z^T from LDS, B operand A_x^T),
+//   out^T = A_y P^T   (A operand A_y, B operand P^T straight from the
+//                      accumulator: FP64 16x16x4 keeps accumulator entry v
+//                      of lane (lr, lc) at row lr + 4v, which is the B
+//                      operand of k-step v with k = 4v + lr),
+// and the results go back through LDS to coalesced stores (post-scaled).
+// Scales: `pre` / `post` along the transformed index before / after the
+// transform, `oth` along the other index (it commutes with the transform).
+#pragma once
+#include "common.hpp"
+
+#define KRON_L 16
+#define KRON_N (KRON_L * KRON_L)
+#define KRON_LD 272 // LDS doubles per vector: 16 grid rows of 17 (one pad)
+
+struct KronArgs {
+    int batch, rows;     // rows = 0: transform the columns of X; 1: its rows
+    const double *in;    // n x n per replica, column-major
+    long long sIn;
+    double *out;         // may not alias `in`
+    long long sOut;
+    const double *ax, *ay; // 16 x 16 factors, row-major: grid -> ax grid ay^T
+    Vec<double> pre, post, oth;
+    const int *status;
+};
+
+__device__ __forceinline__ int kron_lds(int e) { return (e & 15) + 17 * (e >> 4); }
+
+__global__ __launch_bounds__(256) void k_kron16(KronArgs a)
+{
+    const int logical = xcd_remap(blockIdx.x, gridDim.x);
+    const int b = logical / KRON_L, v0 = (logical % KRON_L) * KRON_L;
+    if (b >= a.batch) return;
+    if (a.status && a.status[b]) return;
+    __shared__ double Z[KRON_L * KRON_LD];
+    const double *in = a.in + (long long)b * a.sIn;
+    double *out = a.out + (long long)b * a.sOut;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lc = lane & 15, lr = lane >> 4;
+
+    // 1. stage: vector t = column (row) v0 + t, entry e along it
+    double x[KRON_L];
+    if (!a.rows) {
+        // thread = entry e = tid of 16 columns: 64 consecutive doubles per wave
+#pragma unroll
+        for (int t = 0; t < KRON_L; t++) x[t] = in[tid + (long long)(v0 + t) * KRON_N];
+        const double pe = a.pre.at(b, tid);
+#pragma unroll
+        for (int t = 0; t < KRON_L; t++) Z[t * KRON_LD + kron_lds(tid)] = x[t] * pe * a.oth.at(b, v0 + t);
+    } else {
+        // thread = (row t, entries jq + 16k): 16 consecutive doubles per 16 lanes
+        const int t = tid & 15, jq = tid >> 4;
+#pragma unroll
+        for (int k = 0; k < KRON_L; k++) x[k] = in[(v0 + t) + (long long)(jq + 16 * k) * KRON_N];
+        const double oi = a.oth.at(b, v0 + t);
+#pragma unroll
+        for (int k = 0; k < KRON_L; k++) {
+            const int e = jq + 16 * k;
+            Z[t * KRON_LD + kron_lds(e)] = x[k] * a.pre.at(b, e) * oi;
+        }
+    }
+    __syncthreads();
+
+    // 2. transforms, four vectors per wave, in place in LDS (each wave reads
+    // a vector completely before it writes it back)
+    double bx[4], ay[4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        bx[s] = a.ax[lc * KRON_L + 4 * s + lr]; // B[k = 4s + lr][n = lc] = ax[lc][4s + lr]
+        ay[s] = a.ay[lc * KRON_L + 4 * s + lr]; // A[m = lc][k = 4s + lr]
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        double *Zt = Z + (4 * wave + q) * KRON_LD;
+        double za[4];
+#pragma unroll
+        for (int s = 0; s < 4; s++) za[s] = Zt[kron_lds((4 * s + lr) + 16 * lc)]; // Z[ex = 4s + lr][ey = lc]
+        f64x4 p = {0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 4; s++) p = Mfma<double>::mma(za[s], bx[s], p);
+        f64x4 o = {0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 4; s++) o = Mfma<double>::mma(ay[s], p[s], o);
+        // o[v] = out^T[ey' = lr + 4v][ex' = lc]
+#pragma unroll
+        for (int v = 0; v < 4; v++) Zt[kron_lds(lc + 16 * (lr + 4 * v))] = o[v];
+    }
+    __syncthreads();
+
+    // 3. store, post-scaled along the transformed index
+    if (!a.rows) {
+        const double po = a.post.at(b, tid);
+#pragma unroll
+        for (int t = 0; t < KRON_L; t++) out[tid + (long long)(v0 + t) * KRON_N] = Z[t * KRON_LD + kron_lds(tid)] * po;
+    } else {
+        const int t = tid & 15, jq = tid >> 4;
+#pragma unroll
+        for (int k = 0; k < KRON_L; k++) {
+            const int e = jq + 16 * k;
+            out[(v0 + t) + (long long)e * KRON_N] = Z[t * KRON_LD + kron_lds(e)] * a.post.at(b, e);
+        }
+    }
+}

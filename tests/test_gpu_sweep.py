@@ -13,6 +13,8 @@ pytestmark = pytest.mark.gpu
     ("square", 4, 4, 40, 10, 8, [7, 8], 3),
     ("square", 8, 8, 40, 10, 16, [1, 2], 2),
     ("square", 6, 6, 24, 5, 7, [3], 2),
+    # 16 x 16: the B-matrix products run as separable (Kronecker) transforms
+    ("square", 16, 16, 16, 8, 16, [1, 2], 2),
 ])
 def test_fp64_trajectory(orc, eng_mod, kind, Lx, Ly, L, stab, k, seeds, nsw):
     U, dtau = 4.0, 0.1
@@ -21,10 +23,15 @@ def test_fp64_trajectory(orc, eng_mod, kind, Lx, Ly, L, stab, k, seeds, nsw):
     eng.add_replicas(mid, seeds)
     eng.init()
     refs = [orc.Replica(kind, Lx, Ly, U, dtau, L, stab, s) for s in seeds]
+    # at n = 256 the blocked (32-column) QR of the UDV chain sits further
+    # from the oracle's unblocked one: 1.4e-10 after init and 4.2e-10 after
+    # two sweeps on seed 2, the same with the dense and the separable
+    # B-matrix products; 1e-9 there
+    tol0, tol = (1e-9, 1e-9) if Lx * Ly >= 256 else (1e-10, 5e-10)
     for r, ref in enumerate(refs):
         assert np.array_equal(eng.field(r), ref.field())
         g0 = ref.green()[0]
-        assert np.abs(eng.green(r) - g0).max() / np.abs(g0).max() < 1e-10
+        assert np.abs(eng.green(r) - g0).max() / np.abs(g0).max() < tol0
     for s in range(nsw):
         eng.sweep(1)
         st = eng.status()
@@ -38,7 +45,7 @@ def test_fp64_trajectory(orc, eng_mod, kind, Lx, Ly, L, stab, k, seeds, nsw):
             err = np.abs(eng.green(r) - go).max() / np.abs(go).max()
             # G is reproduced to the oracle's own distance from the compiled
             # reference (9.7e-11 at 8x8, beta=4 after 3 sweeps): 5e-10
-            assert err < 5e-10, err
+            assert err < tol, err
     eng.close()
 
 
@@ -186,3 +193,37 @@ def test_two_stream_schedule_bit_identical(eng_mod, monkeypatch):
         for r in range(len(seeds)):
             assert np.array_equal(a[0][r], b[0][r]), f"fields differ: sweep {s} replica {r}"
             assert np.array_equal(a[2][r], b[2][r]), f"G differs: sweep {s} replica {r}"
+
+
+def test_kron_products_match_dense(eng_mod, monkeypatch):
+    # On a 16 x 16 lattice expK = E_y (x) E_x and the wraps and block
+    # products run as 16 x 16 grid transforms; with DQMC_GPU_KRON=0 they run
+    # as dense n x n GEMMs.  Same fields and accept counts; G differs by the
+    # rounding of the factors (E_y (x) E_x is expK to 2.3e-15), carried
+    # through the stabilised chain like any other rounding: 2.5e-11 after
+    # one sweep.  Bound: 5e-10, the parity tolerance against the oracle.
+    kind, Lx, Ly, L, stab, k, nsw = "square", 16, 16, 24, 8, 16, 3
+    seeds = [5, 6, 7, 8]
+    runs = []
+    for kr in ("1", "0"):
+        monkeypatch.setenv("DQMC_GPU_KRON", kr)
+        print(f"separable products check, DQMC_GPU_KRON={kr}", flush=True)
+        eng = eng_mod.Engine(eng_mod.FP64)
+        mid = eng.add_model(kind, Lx, Ly, 4.0, 0.1, L, stab, delay_k=k)
+        eng.add_replicas(mid, seeds)
+        eng.init()
+        traj = []
+        for _ in range(nsw):
+            eng.sweep(1)
+            st = eng.status()
+            assert (st["status"] == 0).all()
+            traj.append(([eng.field(r).copy() for r in range(len(seeds))], st["accepted"].copy(),
+                         [eng.green(r).copy() for r in range(len(seeds))]))
+        eng.close()
+        runs.append(traj)
+    for s, (a, b) in enumerate(zip(*runs)):
+        assert np.array_equal(a[1], b[1]), f"accept counts differ after sweep {s}"
+        for r in range(len(seeds)):
+            assert np.array_equal(a[0][r], b[0][r]), f"fields differ: sweep {s} replica {r}"
+            err = np.abs(a[2][r] - b[2][r]).max() / np.abs(b[2][r]).max()
+            assert err < 5e-10, (s, r, err)
