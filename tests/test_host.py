@@ -144,3 +144,35 @@ def test_params_sweep_order_and_rebuild(host, tmp_path):
 def test_params_rejects(host, tmp_path, text):
     rc, _ = read(host, tmp_path, text)
     assert rc != 0
+
+
+def test_square_propagators_split(host, eng_mod):
+    # On a 16 x 16 square lattice the engine replaces the dense B-matrix
+    # products by separable ones when expK = E_y (x) E_x (site x + 16 y) to
+    # 1e-12 of its largest entry.  The propagators of both host layers (the
+    # C program's Taylor expm and the Python layer's eigh) split far inside
+    # that; open boundaries split as well, and a lattice with an extra
+    # diagonal bond does not (the engine then keeps the dense GEMMs).
+    L = 16
+    n = L * L
+
+    def split_err(M):
+        fx = M[:L, :L]
+        fy = np.array([[M[L * r, L * c] / M[0, 0] for c in range(L)] for r in range(L)])
+        return np.abs(np.kron(fy, fx) - M).max() / np.abs(M).max()
+
+    for pbc in (1, 0):
+        th, _, _ = eng_mod.lattice("square", L, L, pbc=pbc)
+        K = th - 2.0 * np.eye(n)
+        for s in (-0.1, 0.1):
+            out = np.zeros((n, n))
+            assert host.expm_sym(n, P(np.asfortranarray(K)), s, P(out)) == 0
+            assert split_err(out) < 1e-14
+        eK, eKi, _, _ = eng_mod.model(th, 4.0, 0.1)
+        assert split_err(eK) < 1e-14 and split_err(eKi) < 1e-14
+    th, _, _ = eng_mod.lattice("square", L, L)
+    th = th.copy()
+    th[0, L + 1] = th[L + 1, 0] = -1.0  # a next-nearest-neighbour bond breaks the split
+    out = np.zeros((n, n))
+    assert host.expm_sym(n, P(np.asfortranarray(th - 2.0 * np.eye(n))), -0.1, P(out)) == 0
+    assert split_err(out) > 1e-6
