@@ -1206,6 +1206,16 @@ static int stage_metro(dqmc_gpu_ctx *ctx, Model &m, int l)
             a.Cp = m.w.W3; a.Vp = m.w.W4; a.sP = nn; a.pcount = m.pcount;
             launch_metro_w<double>(a, m.w.s);
             if (launch_check("k_metro_slice")) return 1;
+            if (n % 64 == 0) {
+                // rank-P update on its own kernel (same sums as the GEMM)
+                ProfScope psf_(DQMC_GPU_PROF_FLUSH, m.w.s);
+                FlushArgs f;
+                f.n = n; f.batch = m.nrep; f.G = m.G; f.sG = nn; f.C = m.w.W3; f.V = m.w.W4; f.sP = nn;
+                f.pcount = m.pcount; f.status = m.w.status;
+                hipLaunchKernelGGL(k_flush64, dim3((n / 64) * (n / 64) * m.nrep), dim3(256), 0, m.w.s, f);
+                if (launch_check("k_flush64")) return 1;
+                continue;
+            }
             GemmArgs<double> g =
                 gemm_args<double>(n, n, n, m.nrep, m.w.W3, n, nn, 0, m.w.W4, n, nn, 1, m.G, n, nn);
             g.alpha = -1.0;

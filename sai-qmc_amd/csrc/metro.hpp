@@ -396,3 +396,72 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
         if (negflips) a.sign[b] = -a.sign[b];
     }
 }
+
+// FP64 flush G -= C^T V of the pending pairs (green_delay_flush,
+// src/green.c:462-475) for n a multiple of 64: C, V q-major (P x n per
+// replica, P = pcount[b]).  One 64 x 64 block of G per workgroup (16 per
+// replica at n = 256), 16 columns j per wave, all four 16-row i tiles:
+// D^T[j][i] = sum_q V[q][j] C[q][i] on FP64 MFMA 16x16x4 (A = V^T, B = C,
+// both read as 16 consecutive doubles per lane group, q in increasing order
+// four at a time), then G = G_old - D.  The old G values of a lane are
+// loaded before the q loop, so they arrive while it runs; no LDS, no
+// barrier.  The q order and the final G_old - D match the rank-P GEMM this
+// replaces.
+struct FlushArgs {
+    int n, batch;
+    double *G;
+    long long sG;
+    const double *C, *V;
+    long long sP;
+    const int *pcount;
+    const int *status;
+};
+
+__global__ __launch_bounds__(256) void k_flush64(FlushArgs a)
+{
+    const int n = a.n, nb = n >> 6, per = nb * nb;
+    const int logical = xcd_remap(blockIdx.x, gridDim.x);
+    const int b = logical / per, t = logical % per;
+    if (b >= a.batch) return;
+    if (a.status && a.status[b]) return;
+    const int P = a.pcount[b];
+    if (P <= 0) return;
+    const int i0 = (t % nb) * 64, j0 = (t / nb) * 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lc = lane & 15, lr = lane >> 4;
+    double *G = a.G + (long long)b * a.sG;
+    const double *C = a.C + (long long)b * a.sP;
+    const double *V = a.V + (long long)b * a.sP;
+    const int jw = j0 + 16 * wave; // this wave's 16 columns
+    // accumulator entry v of lane (lr, lc) in i tile ti: D^T[jw + lr + 4v][i0 + 16 ti + lc]
+    double old[4][4];
+#pragma unroll
+    for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) old[ti][v] = G[(i0 + 16 * ti + lc) + (long long)(jw + lr + 4 * v) * n];
+    f64x4 acc[4];
+#pragma unroll
+    for (int ti = 0; ti < 4; ti++) acc[ti] = f64x4{0, 0, 0, 0};
+    for (int q0 = 0; q0 < P; q0 += 16) {
+        double av[4], bv[4][4];
+        // clamped (always valid) rows, zeroed past P after the loads
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const long long qr = (long long)min(q0 + 4 * u + lr, P - 1) * n;
+            av[u] = V[qr + jw + lc];
+#pragma unroll
+            for (int ti = 0; ti < 4; ti++) bv[u][ti] = C[qr + i0 + 16 * ti + lc];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const double a_ = q0 + 4 * u + lr < P ? av[u] : 0.0;
+#pragma unroll
+            for (int ti = 0; ti < 4; ti++) acc[ti] = Mfma<double>::mma(a_, bv[u][ti], acc[ti]);
+        }
+    }
+#pragma unroll
+    for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            G[(i0 + 16 * ti + lc) + (long long)(jw + lr + 4 * v) * n] = old[ti][v] - acc[ti][v];
+}
