@@ -2,20 +2,22 @@
 
 Workload (the metric's configuration): 16x16 periodic square lattice, t=-1,
 U=4, mu=U/2, dtau=0.1, beta=8 (80 slices), stabilisation every 8 slices,
-delayed-update depth 32, 128 independent replicas with seeds 1..128 whose
-Ising fields come from field_init's RNG stream, tau=0 measurement with
-Szz/Sperp at all 256 momenta after every timed sweep.
+128 independent replicas with seeds 1..128 whose Ising fields come from
+field_init's RNG stream, tau=0 measurement with Szz/Sperp at all 256
+momenta after every timed sweep.
 
 A step is one forward sweep of all 128 replicas plus its measurement.  The
-FP64 engine gives `value`: it is the engine whose wrap-vs-rebuild drift
-stays at the reference's own level on this workload.  The two mixed
-FP32/FP64 engines (FP32 wrap / delayed updates / B-chains on MFMA, FP64 QR,
-rebuild and measurement) are timed on the same workload and reported beside
-it with their drift.  Inputs are resident on the GPU before the timed
-region, which is bracketed by HIP events.
+FP64 engine gives `value`; inputs are resident on the GPU before the timed
+region, which is bracketed by HIP events on the engine's stream.  Beside it:
+the FP64 drift as the reference defines it (against a from-scratch G before
+each rebuild) next to the reference's own value on this workload; the
+alternating sweep order; the two mixed FP32/FP64 engines, which need a drift
+guard and fail every replica at the metric's bound of 1e-6 (their speed is
+reported with the guard out of reach); the compiled reference on the host
+cores as the CPU baseline.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-                    [--precision all|fp64|mixed|mixed_fp32_chain] [--calibrate-copy]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--fp64-only]
+                    [--calibrate-copy] [--groups G] [--delay K]
 """
 import argparse
 import json
@@ -34,11 +36,12 @@ P32, P64 = 157.3e12, 78.6e12  # MI355X dense FP32 / FP64 matrix peaks
 HBM = 8.0e12  # MI355X HBM3E bytes/s
 
 
-def run_engine(dq, precision, steps, warmup, drift, groups=1, delay=DELAY, order="forward"):
+def run_engine(dq, precision, steps, warmup, drift=False, guard=0.0, groups=1, delay=DELAY, order="forward",
+               profile=True, allow_fail=False):
     """The 128 replicas as `groups` equal models: each model has its own
     stream and sweep graph, so the groups' sweeps run concurrently.
     order = "alternating" runs forward and backward sweeps in turn."""
-    eng = dq.Engine(precision, drift=drift)
+    eng = dq.Engine(precision, drift=drift, drift_guard=guard)
     L = int(round(BETA / DTAU))
     per = NREP // groups
     for g in range(groups):
@@ -51,17 +54,34 @@ def run_engine(dq, precision, steps, warmup, drift, groups=1, delay=DELAY, order
     eng.sweep(steps, 0)
     ms = eng.elapsed_ms()
     st = eng.status()
-    if st["status"].any():
+    nfail = int((st["status"] != 0).sum())
+    if nfail and not allow_fail:
         raise RuntimeError(f"replica failure: reasons {sorted(set(st['reason'].tolist()))}")
-    acc = float(st["accepted"].sum()) / float(st["attempts"].sum())
-    out = dict(ms=ms, value=NREP * steps / (ms / 1e3), acc=acc)
+    acc = float(st["accepted"].sum()) / max(1.0, float(st["attempts"].sum()))
+    out = dict(ms=ms, value=NREP * steps / (ms / 1e3), acc=acc, failed=nfail,
+               reasons=sorted(set(int(x) for x in st["reason"][st["status"] != 0])))
     if drift:
         mx, mn, cnt = eng.drift()
-        out["drift_max"] = float(mx.max())
-        out["drift_mean"] = float(mn.mean())
-    out["profile"] = eng.profile(1)
+        ok = st["status"] == 0
+        out["drift_max"] = float(mx[ok].max()) if ok.any() else None
+        out["drift_mean"] = float(mn[ok].mean()) if ok.any() else None
+    if profile:
+        out["profile"] = eng.profile(1)
     eng.close()
     return out
+
+
+def reference_drift():
+    """The compiled reference's own stab_drift_file on this workload
+    (tests/golden/drift_16x16_beta8.txt: 4 serial replicas, 6 sweeps)."""
+    path = os.path.join(ROOT, "tests", "golden", "drift_16x16_beta8.txt")
+    if not os.path.exists(path):
+        return None
+    rows = [ln.split() for ln in open(path) if not ln.startswith("#")]
+    mx = [float(r[9]) for r in rows]
+    mn = [float(r[10]) for r in rows]
+    return {"max": max(mx), "mean": sum(mn) / len(mn), "replicas": len(rows),
+            "source": "reference dqmc_omp stab_drift_file, 4 replicas x (2 warm-up + 4 measured) sweeps"}
 
 
 def traffic_from_profile(kernel):
@@ -114,44 +134,86 @@ def roofline_gemm(prof, mixed, nrep_launch=NREP):
 
 
 def roofline_metro(prof, acc, nrep_launch=NREP):
-    """The Metropolis stage of one slice (proposal kernels + flush GEMMs)
+    """The Metropolis stage of one slice (proposal kernels + rank-P flushes)
     against the delayed-update work the reference does for it: 2 a n^3
     flops per replica (a = acceptance)."""
     n = LX * LY
     ms, cnt = prof["metro"]
     t = ms / cnt / 1e3
     flops = 2.0 * acc * n ** 3 * nrep_launch
-    return {"bound": "mfma", "stage": "metro (k_metro_slice + flush k_gemm_big<double,0,1>)",
+    flush = "k_flush64" if n % 64 == 0 else "k_gemm_dl<1> (per-replica K)"
+    return {"bound": "mfma", "stage": f"metro (k_metro_slice + flush {flush})",
             "achieved": round(flops / t / 1e12, 3), "peak": round(P64 / 1e12, 3), "unit": "TFLOP/s",
             "frac": round(flops / t / P64, 4), "avg_ms_per_slice": round(t * 1e3, 4)}
 
 
+def roofline_qr(prof, nrep_launch=NREP):
+    """Every Householder QR of a sweep (k_qr_udv + the explicit Q of
+    k_qr_formq_reg where formed): 4/3 n^3 + 4/3 n^3 flops per replica."""
+    n = LX * LY
+    ms, cnt = prof["qr"]
+    t = ms / cnt / 1e3
+    flops = (8.0 / 3.0) * n ** 3 * nrep_launch
+    return {"bound": "mfma", "stage": "qr (k_qr_udv + k_qr_formq_reg)", "achieved": round(flops / t / 1e12, 3),
+            "peak": round(P64 / 1e12, 3), "unit": "TFLOP/s", "frac": round(flops / t / P64, 4),
+            "avg_ms_per_qr": round(t * 1e3, 4), "qr_per_sweep": cnt}
+
+
+def measure_gbs(prof, nrep_launch=NREP):
+    """The tau = 0 measurement (k_measure): bytes per sample 2 n^2 * 8 (G_up,
+    G_dn) + n nq * 8 (phase table) + 2 nq * 8 (Szz, Sperp) per replica."""
+    n = nq = LX * LY
+    ms, cnt = prof["measure"]
+    if cnt == 0:
+        return None
+    t = ms / cnt / 1e3
+    byts = (2 * n * n * 8 + n * nq * 8 + 2 * nq * 8) * nrep_launch
+    return {"bound": "hbm", "kernel": "k_measure", "achieved": round(byts / t / 1e9, 1),
+            "peak": round(HBM / 1e9, 1), "unit": "GB/s", "frac": round(byts / t / HBM, 4),
+            "avg_launch_ms": round(t * 1e3, 4), "bytes_per_launch": byts}
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, check=True).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name"):
+                return ln.split(":", 1)[1].strip()
+    except (OSError, subprocess.CalledProcessError):
+        pass
+    return None
+
+
 def cpu_baseline():
-    """The reference's own OpenMP replica driver (oracle/_ref/dqmc_omp) on
-    the host cores, two runs differing only in nmeas so setup cancels."""
+    """The reference's own OpenMP replica driver (oracle/_ref/dqmc_omp), 128
+    replicas on every core this process may use, sequential OpenBLAS, two
+    runs differing only in nmeas so set-up cancels."""
     exe = os.path.join(ROOT, "oracle", "_ref", "dqmc_omp")
-    cores = max(1, min(16, os.cpu_count() or 1))
     if not os.path.exists(exe):
         return None
-    nrep = cores
+    cores = len(os.sched_getaffinity(0))
     walls = []
-    for nmeas in (2, 10):
+    lo, hi = 2, 6
+    for nmeas in (lo, hi):
         with tempfile.TemporaryDirectory() as d:
             inp = os.path.join(d, "in.txt")
             with open(inp, "w") as f:
                 f.write(f"lattice=square\nLx={LX}\nLy={LY}\nU={U}\ndtau={DTAU}\nbeta_list={BETA}\nstab={STAB}\n"
-                        f"nwarm=0\nnmeas={nmeas}\nnbin=2\nseed=1\nparallel=omp\nnrep={nrep}\n"
+                        f"nwarm=0\nnmeas={nmeas}\nnbin=2\nseed=1\nparallel=omp\nnrep={NREP}\n"
                         "szz_q=all\nsperp_q=all\n")
             env = dict(os.environ, OMP_NUM_THREADS=str(cores), OPENBLAS_NUM_THREADS="1")
+            print(f"# cpu baseline: {NREP} replicas, nmeas={nmeas}, {cores} threads", file=sys.stderr, flush=True)
             t0 = time.perf_counter()
             subprocess.run([exe, inp], cwd=d, env=env, check=True, stdout=subprocess.DEVNULL,
                            stderr=subprocess.DEVNULL)
             walls.append(time.perf_counter() - t0)
-    rate = nrep * (10 - 2) / (walls[1] - walls[0])
+    rate = NREP * (hi - lo) / (walls[1] - walls[0])
     return {"value": round(rate, 3), "unit": "replica-sweeps/s", "cores": cores, "kind": "reference",
-            "sample": f"reference dqmc_omp, {nrep} replicas x 8 measured sweeps of the same 16x16 U=4 beta=8 "
-                      f"stab=8 workload (difference of nmeas=10 and nmeas=2 runs, nwarm=0), "
-                      f"OMP_NUM_THREADS={cores}, sequential OpenBLAS"}
+            "cpu": cpu_model(),
+            "sample": f"reference dqmc_omp, {NREP} replicas x {hi - lo} measured sweeps of the same 16x16 U=4 "
+                      f"beta=8 stab=8 workload with Szz/Sperp at all q (difference of nmeas={hi} and nmeas={lo} "
+                      f"runs, nwarm=0), OMP_NUM_THREADS={cores} (every core this process may use), "
+                      "sequential OpenBLAS"}
 
 
 def calibrate_copy():
@@ -172,12 +234,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--precision", default="all", choices=["all", "fp64", "mixed", "mixed_fp32_chain"])
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--groups", type=int, default=1, help="replica groups swept concurrently")
     ap.add_argument("--delay", type=int, default=DELAY, help="delayed-update window (sites)")
-    ap.add_argument("--no-alternating", action="store_true",
-                    help="skip the FP64 sweep_order=alternating run reported beside the main line")
+    ap.add_argument("--fp64-only", action="store_true", help="skip the mixed, drift and alternating runs")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: replicas are independent Markov chains and this engine runs on a "
@@ -187,24 +247,40 @@ def main():
 
     if a.calibrate_copy:
         calibrate_copy()
-    modes = {"fp64": dq.FP64, "mixed": dq.MIXED, "mixed_fp32_chain": dq.MIXED_FP32_CHAIN}
-    names = list(modes) if a.precision == "all" else [a.precision]
-    runs = {}
-    for name in names:
-        runs[name] = run_engine(dq, modes[name], a.steps, a.warmup, drift=True, groups=a.groups, delay=a.delay)
-        r = runs[name]
-        print(f"# {name}: {r['value']:.1f} replica-sweeps/s, {r['ms'] / a.steps:.2f} ms/sweep, "
-              f"drift max {r['drift_max']:.3e} mean {r['drift_mean']:.3e}", file=sys.stderr, flush=True)
-    if not a.no_alternating and "fp64" in names:
-        r = run_engine(dq, dq.FP64, a.steps, a.warmup, drift=True, groups=a.groups, delay=a.delay,
-                       order="alternating")
-        runs["fp64_alternating"] = r
-        names.append("fp64_alternating")
-        print(f"# fp64_alternating: {r['value']:.1f} replica-sweeps/s, {r['ms'] / a.steps:.2f} ms/sweep, "
-              f"drift max {r['drift_max']:.3e} mean {r['drift_mean']:.3e}", file=sys.stderr, flush=True)
-    main_name = names[0]
-    main_run = runs[main_name]
+    main_run = run_engine(dq, dq.FP64, a.steps, a.warmup, groups=a.groups, delay=a.delay)
+    print(f"# fp64: {main_run['value']:.1f} replica-sweeps/s, {main_run['ms'] / a.steps:.2f} ms/sweep",
+          file=sys.stderr, flush=True)
+    extra = {}
+    if not a.fp64_only:
+        # drift as the reference defines it (against a from-scratch G before
+        # every rebuild), on a short separate run: the from-scratch G costs
+        # L/stab more UDV updates per rebuild and is not part of the timed run
+        d = run_engine(dq, dq.FP64, 2, 1, drift="scratch", profile=False)
+        extra["drift"] = {"definition": "max|G_wrap - G_scratch(tau)| before each rebuild, inf-norm "
+                                        "(dqmc_record_stab_drift)", "sweeps": 3, "max": d["drift_max"],
+                          "mean": d["drift_mean"], "reference": reference_drift()}
+        print(f"# fp64 drift (from scratch): max {d['drift_max']:.3e} mean {d['drift_mean']:.3e}", file=sys.stderr,
+              flush=True)
+        r = run_engine(dq, dq.FP64, a.steps, a.warmup, groups=a.groups, delay=a.delay, order="alternating",
+                       drift="rebuild")
+        extra["fp64_alternating"] = {"value": round(r["value"], 2), "ms_per_step": round(r["ms"] / a.steps, 3),
+                                     "over_fp64": round(r["value"] / main_run["value"], 3),
+                                     "drift_max_vs_rebuilt": r["drift_max"], "drift_mean_vs_rebuilt": r["drift_mean"]}
+        print(f"# fp64_alternating: {r['value']:.1f} replica-sweeps/s", file=sys.stderr, flush=True)
+        for name, prec in (("mixed", dq.MIXED), ("mixed_fp32_chain", dq.MIXED_FP32_CHAIN)):
+            # with the metric's bound as the guard every replica fails at its
+            # first rebuild; the speed is taken with the guard out of reach
+            g = run_engine(dq, prec, 1, 0, guard=1e-6, profile=False, allow_fail=True)
+            r = run_engine(dq, prec, a.steps, a.warmup, guard=1e300, drift="rebuild", groups=a.groups,
+                           delay=a.delay, profile=False)
+            extra[name] = {"replicas_failed_at_guard_1e-6": g["failed"], "fail_reasons": g["reasons"],
+                           "value_unguarded": round(r["value"], 2), "over_fp64": round(r["value"] / main_run["value"], 3),
+                           "drift_max_vs_rebuilt": r["drift_max"], "drift_mean_vs_rebuilt": r["drift_mean"]}
+            print(f"# {name}: {g['failed']}/{NREP} replicas fail at drift_guard=1e-6; unguarded "
+                  f"{r['value']:.1f} replica-sweeps/s, drift max {r['drift_max']:.3e}", file=sys.stderr, flush=True)
     cpu = None if a.no_cpu else cpu_baseline()
+    n = LX * LY
+    acc = main_run["acc"]
     line = {
         "metric": METRIC,
         "value": round(main_run["value"], 2),
@@ -216,28 +292,25 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if main_name == "fp64" else "f32+f64",
+        "dtype": "f64",
         "data": "synthetic: Ising fields drawn from seeds 1..128 by field_init's RNG stream",
         "config": {"workload": "16x16 periodic square, t=-1, U=4, mu=U/2, dtau=0.1, beta=8 (80 slices), stab=8, "
-                               "delay depth 32, 128 replicas, Szz/Sperp at all 256 q",
-                   "replicas": NREP, "precision": main_name},
-        "drift_max": main_run["drift_max"],
-        "drift_mean": main_run["drift_mean"],
-        "acceptance": round(main_run["acc"], 4),
-        "roofline": roofline_metro_kernel(main_run["profile"], main_run["acc"], main_name != "fp64",
-                                          NREP // a.groups),
-        "roofline_gemm": roofline_gemm(main_run["profile"], main_name != "fp64", NREP // a.groups),
-        "roofline_metro": roofline_metro(main_run["profile"], main_run["acc"], NREP // a.groups),
+                               "128 replicas, Szz/Sperp at all 256 q after every sweep; delayed updates in "
+                               f"16-site windows, the pending pairs of each half-slice (about a*n/2 = "
+                               f"{round(acc * n / 2)} at acceptance a) applied as one rank-P update",
+                   "replicas": NREP, "precision": "fp64"},
+        "acceptance": round(acc, 4),
+        "roofline": roofline_metro_kernel(main_run["profile"], acc, False, NREP // a.groups),
+        "roofline_qr": roofline_qr(main_run["profile"], NREP // a.groups),
+        "roofline_gemm": roofline_gemm(main_run["profile"], False, NREP // a.groups),
+        "roofline_metro": roofline_metro(main_run["profile"], acc, NREP // a.groups),
+        "measurement": measure_gbs(main_run["profile"], NREP // a.groups),
         "profile_ms_per_sweep": {k: round(v[0], 3) for k, v in main_run["profile"].items()},
     }
-    for name in names[1:]:
-        r = runs[name]
-        line[name] = {"value": round(r["value"], 2), "ms_per_step": round(r["ms"] / a.steps, 3),
-                      "over_fp64": round(r["value"] / main_run["value"], 3),
-                      "drift_max": r["drift_max"], "drift_mean": r["drift_mean"],
-                      "roofline_metro": roofline_metro(r["profile"], r["acc"], NREP // a.groups),
-                      "profile_ms_per_sweep": {k: round(v[0], 3) for k, v in r["profile"].items()}}
+    line.update(extra)
     line["cpu_baseline"] = cpu
+    if cpu:
+        line["speedup_vs_cpu_baseline"] = round(main_run["value"] / cpu["value"], 2)
     print(json.dumps(line))
 
 

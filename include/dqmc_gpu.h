@@ -44,6 +44,7 @@ extern "C" {
 #define DQMC_GPU_FAIL_NONFINITE 16 /* non-finite matrix at a stage */
 #define DQMC_GPU_FAIL_ZERO_D 17    /* zero diagonal in a QR (D_i == 0) */
 #define DQMC_GPU_FAIL_SIGN 18      /* det sign evaluated to 0 */
+#define DQMC_GPU_FAIL_DRIFT 19     /* wrap-vs-rebuild drift above opts.drift_guard */
 
 /* Steps for dqmc_gpu_step (per-kernel golden tests). */
 #define DQMC_GPU_STEP_PROPOSE_SLICE 0 /* Metropolis over all sites of slice arg */
@@ -52,15 +53,33 @@ extern "C" {
 
 typedef struct dqmc_gpu_ctx dqmc_gpu_ctx;
 
+/* Stabilisation drift recording (dqmc_gpu_opts.drift). */
+#define DQMC_GPU_DRIFT_OFF 0
+#define DQMC_GPU_DRIFT_SCRATCH 1 /* max|G_wrap - G_scratch(tau)| against a
+                                    from-scratch G of the current fields before
+                                    each rebuild, as dqmc_record_stab_drift
+                                    (src/dqmc.c:55-98): L/stab more UDV updates
+                                    per rebuild */
+#define DQMC_GPU_DRIFT_REBUILD 2 /* max|G_wrap - G_rebuilt| against the G the
+                                    rebuild itself forms (no counterpart in the
+                                    reference; no extra work) */
+
 typedef struct {
-    int precision;   /* DQMC_GPU_FP64 or DQMC_GPU_MIXED */
+    int precision;   /* DQMC_GPU_FP64, DQMC_GPU_MIXED or DQMC_GPU_MIXED_FP32_CHAIN */
     int use_graph;   /* capture one sweep per model as a hipGraph */
-    int drift;       /* record max|G_wrap - G_rebuild| at each rebuild */
-    int autotune;    /* choose delay depth per model when delay_k <= 0 */
+    int drift;       /* DQMC_GPU_DRIFT_* */
+    int autotune;    /* time the Metropolis site segments and delay window of
+                        each model on its first sweeps and keep the fastest */
+    double drift_guard; /* > 0: a replica whose max|G_wrap - G_rebuilt| at a
+                        rebuild exceeds it fails with DQMC_GPU_FAIL_DRIFT
+                        (sticky); required (> 0) by the mixed modes */
 } dqmc_gpu_opts;
 
 /* Create a context on HIP device `device`.  `o` may be NULL (FP64, graphs
-   on, drift off). */
+   on, drift off, no guard).  The mixed modes are refused without a drift
+   guard: their FP32 wrap and delayed updates are not accurate enough for
+   every workload, and the guard turns a replica whose G has drifted into a
+   sticky failure instead of a silently wrong Green's function. */
 int dqmc_gpu_create(dqmc_gpu_ctx **out, int device, const dqmc_gpu_opts *o);
 
 /* Register one model (lattice, U, dtau, L = beta/dtau slices).  expK and
@@ -150,9 +169,9 @@ int dqmc_gpu_read_status(dqmc_gpu_ctx *ctx, int *status, int *failure_reason,
                          unsigned long long *accepted,
                          unsigned long long *attempts, double *sign);
 
-/* Stabilisation drift max|G_wrap - G_rebuild| per replica: maximum, mean
-   and sample count (dqmc_record_stab_drift, src/dqmc.c:64-98; here the
-   rebuilt G of the stack is the reference point). */
+/* Stabilisation drift per replica: maximum, mean and sample count of the
+   samples recorded under opts.drift (dqmc_record_stab_drift,
+   src/dqmc.c:64-98; DQMC_GPU_DRIFT_SCRATCH is the reference's quantity). */
 int dqmc_gpu_read_drift(dqmc_gpu_ctx *ctx, double *max_inf, double *mean_inf,
                         unsigned long long *samples);
 /* Where each replica's largest drift sample was taken: the slice tau of
@@ -220,6 +239,17 @@ int dqmc_gpu_udv_combine(int device, int n, int batch, const double *Ul,
 int dqmc_gpu_udv_inv_one_plus(int device, int n, int batch, const double *U,
                               const double *D, const double *T, double *g,
                               int *sign);
+/* udv_inv_one_plus_work on UDVs that carry a log offset each (the centered
+   rebuild's UDV.log_offset: the diagonal is D e^log_offset; src/linalg.c:
+   911-1115 with the offset-aware split of src/linalg.c:166-182, 963-986).
+   log_offset[batch]; reason[batch] (may be NULL) gets each entry's failure
+   reason (0 = ok; DQMC_GPU_FAIL_EFFECTIVE_LOG_RANGE when e^-|log|D_i| + off|
+   underflows, tests/test_udv.c:199-209).  Returns nonzero if any entry
+   failed; g and sign are then not written. */
+int dqmc_gpu_udv_inv_one_plus_off(int device, int n, int batch, const double *U,
+                                  const double *D, const double *T,
+                                  const double *log_offset, double *g,
+                                  int *sign, int *reason);
 /* measure_szz_sperp_sample + measure_sample on G_up (G_dn by the PH map),
    (src/structure_factor.c:328-400, src/measure.c:6-39).  scal5[batch*5] =
    {E, ekin, eint, ntot, doublon}; szz, sperp [batch*nq]. */

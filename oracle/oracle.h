@@ -101,6 +101,13 @@ int orc_szz_sperp(int Lx, int Ly, int nq, const double *phase_cos,
                   double *sperp);
 void orc_jackknife(const double *x, int N, double *mean, double *err);
 
+/* ---- exact G of a field (exact.c): (1 + B_{tau-1}..B_0 B_{L-1}..B_tau)^-1
+        in 80-bit long double, slices grouped by blk; s holds L*n values
+        +-1 (slice-major).  Not a reference function: the quantity the FP64
+        paths approximate.  Returns 0 ok. ---- */
+int orc_green_exact(int n, int L, int tau, int blk, const double *expK,
+                    const double expv[2], const signed char *s, double *g);
+
 /* ---- full replica engine, PH sweeps: forward (src/dqmc.c:609-813) and
         alternating forward/backward (src/dqmc.c:815-963) ---- */
 typedef struct orc_replica orc_replica;

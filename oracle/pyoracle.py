@@ -79,6 +79,7 @@ def orc():
             "orc_replica_drift": (None, [_P, _P, _P, _P]),
             "orc_replica_rebuild": (_I, [_P]),
             "orc_replica_free": (None, [_P]),
+            "orc_green_exact": (_I, [_I, _I, _I, _I, _P, _P, _P, _P]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -169,6 +170,21 @@ class Replica:
         mx, mn, c = C.c_double(), C.c_double(), C.c_ulonglong()
         orc().orc_replica_drift(self.h, C.byref(mx), C.byref(mn), C.byref(c))
         return mx.value, mn.value, c.value
+
+
+def green_exact(expK, lam, U, dtau, field, L, tau=0, blk=8):
+    """G(tau) = (1 + B_{tau-1}..B_0 B_{L-1}..B_tau)^-1 of a field (L*n values,
+    slice-major) in 80-bit extended precision (oracle/exact.c): the true G
+    that the FP64 paths approximate, for the given FP64 expK and expv."""
+    n = expK.shape[0]
+    ev = np.zeros(2)
+    orc().orc_expv(lam, U, dtau, 1.0, _ptr(ev))
+    g = np.zeros(n * n)
+    f = np.ascontiguousarray(field, dtype=np.int8)
+    rc = orc().orc_green_exact(n, L, tau, blk, _ptr(np.asfortranarray(expK)), _ptr(ev), _ptr(f), _ptr(g))
+    if rc:
+        raise RuntimeError("orc_green_exact failed")
+    return g.reshape(n, n, order="F")
 
 
 def measure(gu, thop, bipart, U, Lx, Ly, q="all"):

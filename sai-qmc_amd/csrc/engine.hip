@@ -351,13 +351,14 @@ __global__ __launch_bounds__(256) void k_udv_recenter(const double *Din, long lo
 
 // centered_update_has_margin (src/linalg.c:119-164), one workgroup per
 // replica: a bound on the largest QR input entry of the update, which
-// must stay below log(DBL_MAX) - 8; left: log|D_j| + max_i log sum_k |B_ik|;
+// must stay below log(DBL_MAX) - 8; left, M = (op(B) U) D with U
+// orthogonal: log|D_j| + max_i log sum_k |op(B)_ik|;
 // right: max_i log|D_i| + log sum_k |T_ik| max_j |B_kj|.  Sums in the log
 // domain shifted by their largest term.  A replica without the margin fails
 // (LINALG_FAILURE_CENTERED_UPDATE_MARGIN).
 __global__ __launch_bounds__(256) void k_update_margin(const double *B, long long sB, const double *T, long long sT,
-                                                       const double *D, long long sD, int n, int left, int batch,
-                                                       int *status, int *reason)
+                                                       const double *D, long long sD, int n, int left, int trans,
+                                                       int batch, int *status, int *reason)
 {
     const int b = blockIdx.x;
     if (b >= batch || status[b]) return;
@@ -367,14 +368,16 @@ __global__ __launch_bounds__(256) void k_update_margin(const double *B, long lon
     const double *Bb = B + b * sB, *Tb = T ? T + b * sT : nullptr, *Db = D + b * sD;
     double worst = -INFINITY;
     if (left) {
-        // row i of B: max_k |B_ik| then sum_k |B_ik| / max, rows over threads
+        // row i of op(B): max_k |B_ik| then sum_k |B_ik| / max, rows over
+        // threads (op(B) = B^T: column i of B)
+        const long long si = trans ? n : 1, sk = trans ? 1 : n;
         double lb = -INFINITY;
         for (int i = threadIdx.x; i < n; i += blockDim.x) {
             double mx = 0.0;
-            for (int k = 0; k < n; k++) mx = fmax(mx, fabs(Bb[i + (long long)k * n]));
+            for (int k = 0; k < n; k++) mx = fmax(mx, fabs(Bb[i * si + k * sk]));
             double sm = 0.0;
             if (mx > 0.0)
-                for (int k = 0; k < n; k++) sm += fabs(Bb[i + (long long)k * n]) / mx;
+                for (int k = 0; k < n; k++) sm += fabs(Bb[i * si + k * sk]) / mx;
             if (mx > 0.0) lb = fmax(lb, log(mx) + log(sm));
         }
         lb = wave_max(lb);
@@ -461,9 +464,10 @@ __global__ __launch_bounds__(256) void k_build_M(const double *Z, const double *
 }
 
 // det sign of 1 + U D T = sgn(prod_big D) sgn det U sgn det M, with
-// det M = det Q_M prod (D_M)_i (src/linalg.c:1039-1083).
-__global__ void k_sign_finish(const int *bigsign, const int *uq, long long suq, const int *mq,
-                              const double *DM, int n, int *det_sign, int *status, int *reason,
+// det M = det Q_M prod (D_M)_i (src/linalg.c:1039-1083); uq2, when given,
+// is the sign of a second orthogonal factor (the two-sided form).
+__global__ void k_sign_finish(const int *bigsign, const int *uq, long long suq, const int *uq2, long long suq2,
+                              const int *mq, const double *DM, int n, int *det_sign, int *status, int *reason,
                               int batch)
 {
     const int b = blockIdx.x;
@@ -478,7 +482,7 @@ __global__ void k_sign_finish(const int *bigsign, const int *uq, long long suq, 
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        int s = bigsign[b] * uq[b * suq] * mq[b] * (neg ? -1 : 1);
+        int s = bigsign[b] * uq[b * suq] * (uq2 ? uq2[b * suq2] : 1) * mq[b] * (neg ? -1 : 1);
         if (bad) s = 0;
         det_sign[b] = s;
         if (s == 0) {
@@ -496,15 +500,17 @@ __global__ void k_mul_q(const int *a, long long sa, const int *b_, long long sb,
     out[b * so] = a[b * sa] * b_[b * sb];
 }
 
-// Replace G by the rebuilt Gnew; optionally record max|G - Gnew| as the
-// stabilisation drift (dqmc_record_stab_drift, src/dqmc.c:64-98).  In mixed
-// precision the wrapped G lives in Gf and the FP32 copy is refreshed.
-// G <- Gnew after a rebuild, with the drift max|G - Gnew| and a non-finite
-// check; many blocks per replica, results gathered with global atomics
-// (the max of non-negative doubles is the max of their bit patterns) and
-// finished by k_drift_finish.
-__global__ void k_drift_copy(double *G, float *Gf, const double *Gnew, long long sG, int n, int batch, int drift,
-                             int from_f32, unsigned long long *tmax, int *tbad, const int *status)
+// G <- Gnew after a rebuild (copy = 1), with the distance max|G - Gnew|
+// (measure = 1; the wrapped G is the FP32 copy Gf when from_f32) and a
+// non-finite check of Gnew and (check_old = 1) of the wrapped G; with
+// copy = 0 only the distance is taken (Gnew then holds a from-scratch G,
+// dqmc_record_stab_drift, src/dqmc.c:64-98).
+// Many blocks per replica, results gathered with global atomics (the max
+// of non-negative doubles is the max of their bit patterns) and finished
+// by k_drift_finish.
+__global__ void k_drift_copy(double *G, float *Gf, const double *Gnew, long long sG, int n, int batch, int measure,
+                             int check_old, int from_f32, int copy, unsigned long long *tmax, int *tbad,
+                             const int *status)
 {
     const int b = blockIdx.y;
     if (b >= batch || status[b]) return;
@@ -519,20 +525,23 @@ __global__ void k_drift_copy(double *G, float *Gf, const double *Gnew, long long
             const long long idx = min(base + k * stride, nn - 1);
             v[k] = Gnew[b * sG + idx];
             o[k] = 0.0;
-            if (drift) o[k] = from_f32 ? (double)Gf[b * sG + idx] : G[b * sG + idx];
+            if (measure || check_old) o[k] = from_f32 ? (double)Gf[b * sG + idx] : G[b * sG + idx];
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const long long idx = base + k * stride;
             if (idx < nn) {
-                if (!isfinite(v[k])) bad = 1;
-                if (drift) m = fmax(m, fabs(o[k] - v[k]));
-                G[b * sG + idx] = v[k];
-                if (Gf) Gf[b * sG + idx] = (float)v[k];
+                if (!isfinite(v[k]) || (check_old && !isfinite(o[k]))) bad = 1;
+                // a NaN on either side counts as an infinite distance
+                if (measure) m = fmax(m, isnan(o[k] - v[k]) ? INFINITY : fabs(o[k] - v[k]));
+                if (copy) {
+                    G[b * sG + idx] = v[k];
+                    if (Gf) Gf[b * sG + idx] = (float)v[k];
+                }
             }
         }
     }
-    if (drift) {
+    if (measure) {
         // one atomic per workgroup: wave maxima meet in LDS first
         __shared__ double wm[16];
         m = wave_max(m);
@@ -546,25 +555,32 @@ __global__ void k_drift_copy(double *G, float *Gf, const double *Gnew, long long
     if (bad) atomicOr(&tbad[b], 1);
 }
 
-// per replica: the running sign is reset to the exact +1 at every rebuild
-// (PH path, src/dqmc.c:740-742, 799-801), failures are recorded, the drift
-// sample is accumulated, and the gather slots are cleared for the next call
-// The slice tau of the sample and the model's completed-sweep counter are
-// kept with each new maximum (max_tau, max_sweep of DqmcStabDrift,
-// src/dqmc.c:90-95).
-__global__ void k_drift_finish(int batch, int drift, unsigned long long *tmax, int *tbad, double *dmax, double *dsum,
-                               unsigned long long *dcnt, int *status, int *reason, double *sign, int tau,
-                               const unsigned long long *sweeps, int *dtau, unsigned long long *dsweep)
+// Per replica, after k_drift_copy: failures are recorded (a non-finite
+// Gnew; with guard > 0 a distance above guard, DQMC_GPU_FAIL_DRIFT), the
+// running sign is reset to the exact +1 when sign is given (a rebuild, PH
+// path, src/dqmc.c:740-742, 799-801), the distance is accumulated as a
+// drift sample when record is set, and the gather slots are cleared for the
+// next call.  The slice tau of the sample and the model's completed-sweep
+// counter are kept with each new maximum (max_tau, max_sweep of
+// DqmcStabDrift, src/dqmc.c:90-95).
+__global__ void k_drift_finish(int batch, int record, double guard, unsigned long long *tmax, int *tbad,
+                               double *dmax, double *dsum, unsigned long long *dcnt, int *status, int *reason,
+                               double *sign, int tau, const unsigned long long *sweeps, int *dtau,
+                               unsigned long long *dsweep)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= batch) return;
     if (!status[b]) {
+        const double mm = __longlong_as_double((long long)tmax[b]);
         if (sign) sign[b] = 1.0;
         if (tbad[b]) {
             status[b] = 1;
             reason[b] = DQMC_GPU_FAIL_NONFINITE;
-        } else if (drift) {
-            const double mm = __longlong_as_double((long long)tmax[b]);
+        } else if (guard > 0.0 && !(mm <= guard)) {
+            status[b] = 1;
+            reason[b] = DQMC_GPU_FAIL_DRIFT;
+        }
+        if (record && !tbad[b]) {
             if (dcnt[b] == 0 || mm > dmax[b]) {
                 dmax[b] = mm;
                 dtau[b] = tau;
@@ -582,6 +598,87 @@ __global__ void k_drift_finish(int batch, int drift, unsigned long long *tmax, i
 __global__ void k_sweep_tick(unsigned long long *sweeps)
 {
     if (threadIdx.x == 0) *sweeps += 1;
+}
+
+// Column pre-pivoting of a UDV update's QR input M (one workgroup per
+// replica): perm = the columns of M in order of decreasing 2-norm (ties by
+// index), so that the QR of M P = Q R factors the graded columns largest
+// first.  Norms are compared as log(max|x|) + log ||x / max|x| || so that
+// entries near the double range neither overflow nor underflow; each wave
+// takes a column at a time (lanes over rows, coalesced), then a bitonic
+// sort of the (key, index) pairs in LDS.
+#define CP_MAX 512
+__global__ __launch_bounds__(256) void k_col_order(const double *M, long long sM, int n, int *perm, int batch,
+                                                   const int *status)
+{
+    const int b = blockIdx.x;
+    if (b >= batch || status[b]) return;
+    __shared__ double key[CP_MAX];
+    __shared__ int idx[CP_MAX];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const double *Mb = M + b * sM;
+    for (int j = wave; j < n; j += nw) {
+        const double *col = Mb + (long long)j * n;
+        double mx = 0.0;
+        for (int i = lane; i < n; i += 64) mx = fmax(mx, fabs(col[i]));
+        mx = wave_max(mx);
+        double ss = 0.0;
+        if (mx > 0.0)
+            for (int i = lane; i < n; i += 64) {
+                const double t = col[i] / mx;
+                ss += t * t;
+            }
+        ss = wave_sum(ss);
+        if (lane == 0) {
+            const double k = mx > 0.0 ? log(mx) + 0.5 * log(ss) : -INFINITY;
+            key[j] = isnan(k) ? -INFINITY : k;
+            idx[j] = j;
+        }
+    }
+    int n2 = 1;
+    while (n2 < n) n2 <<= 1;
+    for (int j = n + (int)threadIdx.x; j < n2; j += blockDim.x) {
+        key[j] = -INFINITY;
+        idx[j] = j;
+    }
+    __syncthreads();
+    // bitonic sort, descending key, ascending index among equal keys
+    for (int k = 2; k <= n2; k <<= 1)
+        for (int jj = k >> 1; jj > 0; jj >>= 1) {
+            for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+                const int l = i ^ jj;
+                if (l > i) {
+                    const bool before = key[i] > key[l] || (key[i] == key[l] && idx[i] < idx[l]);
+                    const bool up = (i & k) == 0; // this half ends up descending
+                    if (up != before) {
+                        const double tk = key[i];
+                        key[i] = key[l];
+                        key[l] = tk;
+                        const int ti = idx[i];
+                        idx[i] = idx[l];
+                        idx[l] = ti;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int k = threadIdx.x; k < n; k += blockDim.x) perm[(long long)b * n + k] = idx[k];
+}
+
+// out[:, k] = in[:, perm[k]] (rows = 0) or out[k, :] = in[perm[k], :]
+// (rows = 1), n x n per batch entry
+__global__ void k_gather(const double *in, long long sIn, double *out, long long sOut, const int *perm, int n,
+                         int rows, int batch, const int *status)
+{
+    const int b = blockIdx.y;
+    if (b >= batch || status[b]) return;
+    const long long nn = (long long)n * n;
+    const int *pb = perm + (long long)b * n;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < nn; e += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(e % n), k = (int)(e / n);
+        const long long src = rows ? (long long)pb[i] + (long long)k * n : (long long)i + (long long)pb[k] * n;
+        out[b * sOut + e] = in[b * sIn + src];
+    }
 }
 
 // ---------------------------------------------------------------- buffers
@@ -608,6 +705,7 @@ struct Work {
     double *W1 = nullptr, *W2 = nullptr, *W3 = nullptr, *W4 = nullptr;
     double *V1 = nullptr, *V2 = nullptr, *V3 = nullptr, *V4 = nullptr, *V5 = nullptr, *Tws = nullptr;
     int *I1 = nullptr, *I2 = nullptr, *status = nullptr, *reason = nullptr, *det_sign = nullptr;
+    int *perm = nullptr; // column order of a pre-pivoted QR, n per entry
     float *F1 = nullptr, *F2 = nullptr;
     std::vector<void *> owned;
 
@@ -623,10 +721,10 @@ struct Work {
         const int npan = (n + QR_NB - 1) / QR_NB;
         if (dalloc(&Tws, (size_t)npan * QR_NB * QR_NB * nb)) return 1;
         if (dalloc(&I1, nb) || dalloc(&I2, nb) || dalloc(&status, nb) || dalloc(&reason, nb) ||
-            dalloc(&det_sign, nb))
+            dalloc(&det_sign, nb) || dalloc(&perm, (size_t)n * nb))
             return 1;
         if (mixed && (dalloc(&F1, nn) || dalloc(&F2, nn))) return 1;
-        void *ps[] = {W1, W2, W3, W4, V1, V2, V3, V4, V5, Tws, I1, I2, status, reason, det_sign, F1, F2};
+        void *ps[] = {W1, W2, W3, W4, V1, V2, V3, V4, V5, Tws, I1, I2, status, reason, det_sign, perm, F1, F2};
         for (void *p : ps)
             if (p) owned.push_back(p);
         HIPCHK(hipMemset(status, 0, sizeof(int) * nb));
@@ -756,21 +854,79 @@ static int copy_nn(Work &w, const double *src, long long sS, double *dst, long l
     return launch_check("k_copy_d");
 }
 
+// Column pre-pivoting of the UDV updates (DQMC_GPU_PIVOT=0 turns it off):
+// the QR input M of an update is factored as M P = Q R with P ordering its
+// columns by decreasing norm, so U' = Q, D' = diag R and T' = (D'^-1 R) P^T T.
+// The normalised R then has entries of order one and the accumulated T
+// stays bounded (at 16x16, U=4, beta=8 max|T| ~ 3e2 against 3e5 unpivoted),
+// which is what keeps the boundary rebuild's T_l T_r^T product accurate.
+static bool pivot_on()
+{
+    static int v = -1;
+    if (v < 0) {
+        const char *e = getenv("DQMC_GPU_PIVOT");
+        v = (e && e[0] == '0') ? 0 : 1;
+    }
+    return v == 1;
+}
+
+// QR of an update's input M (n x n per entry, left unchanged when pivoted)
+// into U' (Q), Rn (W3), D' and sgn det Q; with pivoting the columns are
+// first ordered into W4 (w.perm holds the order) and factored from there.
+// T' = Rn P^T T is left to update_T.
+static int qr_update(Work &w, double *M, const Udv &out)
+{
+    const long long nn = (long long)w.n * w.n;
+    double *qin = M;
+    if (pivot_on()) {
+        hipLaunchKernelGGL(k_col_order, dim3(w.nb), dim3(256), 0, w.s, M, nn, w.n, w.perm, w.nb, w.status);
+        if (launch_check("k_col_order")) return 1;
+        hipLaunchKernelGGL(k_gather, grid2(nn, w.nb), dim3(256), 0, w.s, M, nn, w.W4, nn, w.perm, w.n, 0, w.nb,
+                           w.status);
+        if (launch_check("k_gather")) return 1;
+        qin = w.W4;
+    }
+    return qr(w, qin, out.U, out.sU, w.W3, nn, out.D, out.sD, out.q, out.sq);
+}
+
+// out.T = Rn P^T in.T after qr_update (Rn in W3); W2 and W4 are scratch
+static int update_T(Work &w, const Udv &in, const Udv &out)
+{
+    const int n = w.n;
+    const long long nn = (long long)n * n;
+    if (pivot_on()) {
+        hipLaunchKernelGGL(k_gather, grid2(nn, w.nb), dim3(256), 0, w.s, in.T, in.sU, w.W2, nn, w.perm, n, 1, w.nb,
+                           w.status);
+        if (launch_check("k_gather")) return 1;
+        return gemm(w, gemm_args<double>(n, n, n, w.nb, w.W3, n, nn, 0, w.W2, n, nn, 0, out.T, n, out.sU));
+    }
+    const bool inplace = in.T == out.T;
+    GemmArgs<double> g = gemm_args<double>(n, n, n, w.nb, w.W3, n, nn, 0, in.T, n, in.sU, 0,
+                                           inplace ? w.W4 : out.T, n, inplace ? nn : out.sU);
+    if (gemm(w, g)) return 1;
+    return inplace ? copy_nn(w, w.W4, nn, out.T, out.sU, nn) : 0;
+}
+
 // udv_combine (src/linalg.c:796-909): C = Dl (Tl Ur) Dr; QR(C) = Q R;
-// out = (Ul Q, diag R, (D^-1 R) Tr).
-static int udv_combine(Work &w, const Udv &l, const Udv &r, const Udv &o)
+// out = (Ul Q, diag R, (D^-1 R) Tr).  rT: the right factor is stored
+// transposed (U D T = R^T, the suffix stack), so R = Tr^T Dr Ur^T,
+// C = Dl (Tl Tr^T) Dr and out T = (D^-1 R) Ur^T, which is no longer
+// triangular; udv_inv_one_plus does not need it to be.
+static int udv_combine(Work &w, const Udv &l, const Udv &r, const Udv &o, int rT = 0)
 {
     ProfScope ps_(DQMC_GPU_PROF_COMBINE, w.s);
     const int n = w.n;
     const long long nn = (long long)n * n;
-    GemmArgs<double> g = gemm_args<double>(n, n, n, w.nb, l.T, n, l.sU, 0, r.U, n, r.sU, 0, w.W1, n, nn);
+    GemmArgs<double> g = rT ? gemm_args<double>(n, n, n, w.nb, l.T, n, l.sU, 0, r.T, n, r.sU, 1, w.W1, n, nn)
+                            : gemm_args<double>(n, n, n, w.nb, l.T, n, l.sU, 0, r.U, n, r.sU, 0, w.W1, n, nn);
     g.ar = vec(l.D, l.sD);
     g.bc = vec(r.D, r.sD);
     if (gemm(w, g)) return 1;
     if (qr(w, w.W1, w.W2, nn, w.W3, nn, o.D, o.sD, w.I1, 1)) return 1;
     g = gemm_args<double>(n, n, n, w.nb, l.U, n, l.sU, 0, w.W2, n, nn, 0, o.U, n, o.sU);
     if (gemm(w, g)) return 1;
-    g = gemm_args<double>(n, n, n, w.nb, w.W3, n, nn, 0, r.T, n, r.sU, 0, o.T, n, o.sU);
+    g = rT ? gemm_args<double>(n, n, n, w.nb, w.W3, n, nn, 0, r.U, n, r.sU, 1, o.T, n, o.sU)
+           : gemm_args<double>(n, n, n, w.nb, w.W3, n, nn, 0, r.T, n, r.sU, 0, o.T, n, o.sU);
     if (gemm(w, g)) return 1;
     hipLaunchKernelGGL(k_mul_q, dim3((w.nb + 255) / 256), dim3(256), 0, w.s, l.q, l.sq, w.I1, 1LL, o.q,
                        o.sq, w.nb, w.status);
@@ -798,20 +954,23 @@ static int udv_inv_one_plus(Work &w, const Udv &u, double *Gout, long long sG)
     g.cr = vec<double>(w.V3, n, 1);
     if (gemm(w, g)) return 1;                           // W1 = D_N^-1 Q_N^T X
     if (trsm(w, w.W3, nn, w.W1, nn, Gout, sG)) return 1; // G = Rn_N^-1 W1
-    hipLaunchKernelGGL(k_sign_finish, dim3(w.nb), dim3(256), 0, w.s, w.I1, u.q, u.sq, w.I2, w.V3, n,
-                       w.det_sign, w.status, w.reason, w.nb);
+    hipLaunchKernelGGL(k_sign_finish, dim3(w.nb), dim3(256), 0, w.s, w.I1, u.q, u.sq, (const int *)nullptr, 0LL,
+                       w.I2, w.V3, n, w.det_sign, w.status, w.reason, w.nb);
     return launch_check("k_sign_finish");
 }
 
-// G = (1 + Ul Dl Tl Ur Dr Tr)^-1 at a block boundary with one QR, the
-// fused form of udv_combine + udv_inv_one_plus (src/linalg.c:796-1115):
-// 1 + L R = Ul Dlb M Drb Tr with
-//   M = Dlb^-1 (Ul^T Tr^-1) Drb^-1 + Dls (Tl Ur) Drs
-// (both terms bounded; T of every UDV here is unit upper triangular), so
-//   G = Tr^-1 Drb^-1 Rn_M^-1 D_M^-1 Q_M^T Dlb^-1 Ul^T,
-// triangular solves for Tr^-T, Rn_M^-1 and Tr^-1, one GEMM for Tl Ur and
-// one for Q_M^T Ul^T.  sgn det(1 + L R) = sgn det Ul * prod_{|D|>1} sgn D
-// (both sides) * sgn det Q_M * prod sgn (D_M)_i.
+// G = (1 + L R)^-1 at a block boundary with one QR, the fused form of
+// udv_combine + udv_inv_one_plus (src/linalg.c:796-1115).  L = Ul Dl Tl
+// (prefix) and R stored transposed (suffix stack: Ur Dr Tr = R^T, so
+// R = Tr^T Dr Ur^T).  With the Db/Ds split of both diagonals
+//   1 + L R = Ul Dlb M Drb Ur^T,
+//   M = Dlb^-1 (Ul^T Ur) Drb^-1 + Dls (Tl Tr^T) Drs
+// (both terms bounded: the big parts are divided out), so with
+// M = Q_M D_M Rn_M
+//   G = Ur Drb^-1 Rn_M^-1 D_M^-1 Q_M^T Dlb^-1 Ul^T:
+// two GEMMs for the terms of M, one for Q_M^T Ul^T, one triangular solve
+// and one GEMM for Ur.  sgn det(1 + L R) = sgn det Ul * sgn det Ur *
+// prod_{|D|>1} sgn D (both sides) * sgn det Q_M * prod sgn (D_M)_i.
 static int udv_inv_two_sided(Work &w, const Udv &l, const Udv &r, double *Gout, long long sG)
 {
     ProfScope ps_(DQMC_GPU_PROF_INV, w.s);
@@ -820,9 +979,11 @@ static int udv_inv_two_sided(Work &w, const Udv &l, const Udv &r, double *Gout, 
     hipLaunchKernelGGL(k_split2, dim3(w.nb), dim3(256), 0, w.s, l.D, l.sD, l.off, l.so, r.D, r.sD, r.off, r.so, n,
                        w.V1, w.V2, w.V4, w.V5, w.I1, w.nb, w.status, w.reason);
     if (launch_check("k_split2")) return 1;
-    if (trsm(w, r.T, r.sU, l.U, l.sU, w.W1, nn, 1)) return 1; // W1 = Tr^-T Ul = (Ul^T Tr^-1)^T
-    GemmArgs<double> g = gemm_args<double>(n, n, n, w.nb, l.T, n, l.sU, 0, r.U, n, r.sU, 0, w.W2, n, nn);
-    if (gemm(w, g)) return 1; // W2 = Tl Ur
+    // W1 = Ur^T Ul (k_build_M reads it transposed), W2 = Tl Tr^T
+    GemmArgs<double> g = gemm_args<double>(n, n, n, w.nb, r.U, n, r.sU, 1, l.U, n, l.sU, 0, w.W1, n, nn);
+    if (gemm(w, g)) return 1;
+    g = gemm_args<double>(n, n, n, w.nb, l.T, n, l.sU, 0, r.T, n, r.sU, 1, w.W2, n, nn);
+    if (gemm(w, g)) return 1;
     const int bnt = (n + BN_T - 1) / BN_T;
     hipLaunchKernelGGL(k_build_M, dim3(bnt * bnt, w.nb), dim3(256), 0, w.s, w.W1, w.W2, w.V1, w.V2, w.V4, w.V5, w.W4,
                        n, w.nb, w.status);
@@ -831,10 +992,12 @@ static int udv_inv_two_sided(Work &w, const Udv &l, const Udv &r, double *Gout, 
     g = gemm_args<double>(n, n, n, w.nb, w.W1, n, nn, 1, l.U, n, l.sU, 1, w.W3, n, nn);
     g.br = vec<double>(w.V1, n);    // Dlb^-1 on the rows of Ul^T
     g.cr = vec<double>(w.V3, n, 1); // D_M^-1
-    if (gemm(w, g)) return 1;                                            // W3 = D_M^-1 Q_M^T Dlb^-1 Ul^T
-    if (trsm(w, w.W2, nn, w.W3, nn, w.W4, nn)) return 1;                 // W4 = Rn_M^-1 W3
-    if (trsm(w, r.T, r.sU, w.W4, nn, Gout, sG, 0, w.V4, n)) return 1;    // G = Tr^-1 Drb^-1 W4
-    hipLaunchKernelGGL(k_sign_finish, dim3(w.nb), dim3(256), 0, w.s, w.I1, l.q, l.sq, w.I2, w.V3, n,
+    if (gemm(w, g)) return 1;                            // W3 = D_M^-1 Q_M^T Dlb^-1 Ul^T
+    if (trsm(w, w.W2, nn, w.W3, nn, w.W4, nn)) return 1; // W4 = Rn_M^-1 W3
+    g = gemm_args<double>(n, n, n, w.nb, r.U, n, r.sU, 0, w.W4, n, nn, 0, Gout, n, sG);
+    g.ac = vec<double>(w.V4, n); // Drb^-1
+    if (gemm(w, g)) return 1;    // G = Ur Drb^-1 W4
+    hipLaunchKernelGGL(k_sign_finish, dim3(w.nb), dim3(256), 0, w.s, w.I1, l.q, l.sq, r.q, r.sq, w.I2, w.V3, n,
                        w.det_sign, w.status, w.reason, w.nb);
     return launch_check("k_sign_finish");
 }
@@ -878,6 +1041,9 @@ struct Model {
     int *Lq = nullptr;
     double *CU = nullptr, *CD = nullptr, *CT = nullptr;
     int *Cq = nullptr;
+    // from-scratch UDV of the drift samples (DQMC_GPU_DRIFT_SCRATCH only)
+    double *XU = nullptr, *XD = nullptr, *XT = nullptr, *XO = nullptr;
+    int *Xq = nullptr;
     unsigned long long *acc = nullptr, *att = nullptr, *acc_prev = nullptr, *att_prev = nullptr;
     int *pcount = nullptr; // pending delayed-update pairs per replica after a slice
     unsigned long long *dtmax = nullptr; // drift gather slots (k_drift_copy -> k_drift_finish)
@@ -918,6 +1084,10 @@ struct Model {
     Udv left() const
     {
         return Udv{LU_, LD, LT, Lq, (long long)n * n, (long long)n, 1, LO, 1};
+    }
+    Udv xdrift() const
+    {
+        return Udv{XU, XD, XT, Xq, (long long)n * n, (long long)n, 1, XO, 1};
     }
     Udv comb() const
     {
@@ -1046,6 +1216,13 @@ static int model_alloc(dqmc_gpu_ctx *ctx, Model &m)
         dalloc(&m.CT, nn * R) || dalloc(&m.Cq, R))
         return 1;
     own(m.LU_); own(m.LD); own(m.LT); own(m.Lq); own(m.CU); own(m.CD); own(m.CT); own(m.Cq);
+    if (ctx->opts.drift == DQMC_GPU_DRIFT_SCRATCH) {
+        if (dalloc(&m.XU, nn * R) || dalloc(&m.XD, (size_t)n * R) || dalloc(&m.XT, nn * R) || dalloc(&m.Xq, R) ||
+            dalloc(&m.XO, R))
+            return 1;
+        own(m.XU); own(m.XD); own(m.XT); own(m.Xq); own(m.XO);
+        HIPCHK(hipMemset(m.XO, 0, sizeof(double) * R));
+    }
     if (dalloc(&m.pcount, R) || dalloc(&m.dtmax, R) || dalloc(&m.dtbad, R)) return 1;
     own(m.pcount);
     own(m.dtmax);
@@ -1351,16 +1528,18 @@ static int udv_recenter(Model &m, const Udv &from, const Udv &to)
 
 // the update's margin check of the centered mode on B (and T for rmul)
 static int update_margin(Model &m, const double *B, long long sB, const double *T, long long sT, const double *D,
-                         long long sD, int left)
+                         long long sD, int left, int trans = 0)
 {
-    hipLaunchKernelGGL(k_update_margin, dim3(m.nrep), dim3(256), 0, m.w.s, B, sB, T, sT, D, sD, m.n, left, m.nrep,
-                       m.w.status, m.w.reason);
+    hipLaunchKernelGGL(k_update_margin, dim3(m.nrep), dim3(256), 0, m.w.s, B, sB, T, sT, D, sD, m.n, left, trans,
+                       m.nrep, m.w.status, m.w.reason);
     return launch_check("k_update_margin");
 }
 
 // out <- B_{lb+len-1} ... B_{lb} in   (udv_lmul_work, src/linalg.c:515-642);
-// `out` may be `in` (in place) or another UDV (the prefix stack's next entry)
-static int stage_lmul(dqmc_gpu_ctx *ctx, Model &m, const Udv &in, const Udv &out, int lb, int len)
+// `out` may be `in` (in place) or another UDV (the prefix stack's next entry);
+// Bpre, when given, is that block product already formed (a cached one)
+static int stage_lmul(dqmc_gpu_ctx *ctx, Model &m, const Udv &in, const Udv &out, int lb, int len,
+                      const double *Bpre = nullptr)
 {
     ProfScope ps_(DQMC_GPU_PROF_LMUL, m.w.s);
     const int n = m.n, R = m.nrep;
@@ -1382,12 +1561,16 @@ static int stage_lmul(dqmc_gpu_ctx *ctx, Model &m, const Udv &in, const Udv &out
         g.bc = vec(src.D, src.sD);
         if (gemm(m.w, g)) return 1;
         cur = m.w.W1;
-    } else if (m.BB && (aligned || m.centered)) {
+    } else if (Bpre || (m.BB && (aligned || m.centered))) {
         // Bblock_j = B_{lb+len-1} ... B_{lb}, left-accumulated (src/green.c:151-166),
         // kept for the next sweep's suffix stack (a block not on the stack
         // grid, centered only, goes to W3); then (Bblock U) diag(D)
-        double *Bj = aligned ? m.BB + (long long)(lb / m.stab) * R * nn : m.w.W3;
-        if (bblock_build(m, lb, len, Bj)) return 1;
+        const double *Bj = Bpre;
+        if (!Bj) {
+            double *Bw = aligned ? m.BB + (long long)(lb / m.stab) * R * nn : m.w.W3;
+            if (bblock_build(m, lb, len, Bw)) return 1;
+            Bj = Bw;
+        }
         if (m.centered && update_margin(m, Bj, nn, nullptr, 0, src.D, src.sD, 1)) return 1;
         GemmArgs<double> g = gemm_args<double>(n, n, n, R, Bj, n, nn, 0, in.U, n, in.sU, 0, m.w.W1, n, nn);
         g.bc = vec(src.D, src.sD);
@@ -1408,78 +1591,60 @@ static int stage_lmul(dqmc_gpu_ctx *ctx, Model &m, const Udv &in, const Udv &out
             std::swap(cur, oth);
         }
     }
-    // QR: U' = Q, D' = diag R, T' = (D'^-1 R) T
-    if (qr(m.w, cur, out.U, out.sU, m.w.W3, nn, out.D, out.sD, out.q, out.sq)) return 1;
-    const bool inplace = in.T == out.T;
-    GemmArgs<double> g =
-        gemm_args<double>(n, n, n, R, m.w.W3, n, nn, 0, in.T, n, in.sU, 0, inplace ? m.w.W4 : out.T, n,
-                          inplace ? nn : out.sU);
-    if (gemm(m.w, g)) return 1;
-    if (inplace && copy_nn(m.w, m.w.W4, nn, out.T, out.sU, nn)) return 1;
+    // QR: U' = Q, D' = diag R, T' = (D'^-1 R) P^T T
+    if (qr_update(m.w, cur, out) || update_T(m.w, in, out)) return 1;
     return m.centered ? udv_recenter(m, out, out) : 0;
 }
 
 // S[j] = S[j+1] B_{b_{j+1}-1} ... B_{b_j}   (green_stack_build_suffix +
-// udv_rmul, src/green.c:203-226, src/linalg.c:662-784)
+// udv_rmul, src/green.c:203-226, src/linalg.c:662-784), kept transposed:
+// stack entry j holds U D T = S[j]^T, so the step is the left update
+// S[j]^T = Bblock_j^T S[j+1]^T: M = (Bblock_j^T U) diag(D), QR(M) = Q R,
+// U' = Q, D' = diag R, T' = (D'^-1 R) T.  Its QR input is graded by
+// columns, as in the left chain, where the blocked Householder QR keeps
+// the small entries to working accuracy; the reference's D (T B) is graded
+// by rows, where a blocked (compact WY) trailing update loses digits in the
+// small rows.  S[M] is the identity either way.
 static int stage_rmul(dqmc_gpu_ctx *ctx, Model &m, int j, Work &w)
 {
     ProfScope ps_(DQMC_GPU_PROF_STACK, w.s);
     const int n = m.n, R = m.nrep;
-    const long long nn = (long long)n * n, sV = (long long)m.L * n;
+    const long long nn = (long long)n * n;
     const int bj = std::min(j * m.stab, m.L), bj1 = std::min((j + 1) * m.stab, m.L);
     const Udv s1 = m.stack(j + 1), s0 = m.stack(j);
+    const bool cached = m.BB && m.bb_valid && !chain_f32(ctx);
+    if (!cached && &w != &m.w) {
+        set_err("suffix update without cached block products runs on the main stream only");
+        return 1;
+    }
+    // the block product Bblock_j: cached, or formed into W3 (main stream)
+    const double *Bj;
+    if (cached) {
+        Bj = m.BB + (long long)j * R * nn;
+    } else if (chain_f32(ctx)) {
+        float *Bf = bblock_f32(m, bj, bj1 - bj);
+        hipLaunchKernelGGL(k_f2d, grid2(nn, R), dim3(256), 0, w.s, Bf, nn, w.W3, nn, nn, R);
+        if (launch_check("k_f2d")) return 1;
+        Bj = w.W3;
+    } else {
+        if (bblock_build(m, bj, bj1 - bj, w.W3)) return 1;
+        Bj = w.W3;
+    }
     // centered mode (udv_rmul_centered, src/linalg.c:687-704): S[j+1]'s D
     // recentred into S[j], from where the update reads it
     const double *Dsrc = s1.D;
     long long sDsrc = s1.sD;
     if (m.centered) {
-        if (!(m.BB && m.bb_valid)) {
-            set_err("centered rebuild needs the block-product cache");
-            return 1;
-        }
         if (udv_recenter(m, s1, s0)) return 1;
         Dsrc = s0.D;
         sDsrc = s0.sD;
-        if (update_margin(m, m.BB + (long long)j * R * nn, nn, s1.T, s1.sU, Dsrc, sDsrc, 0)) return 1;
+        if (update_margin(m, Bj, nn, nullptr, 0, Dsrc, sDsrc, 1, 1)) return 1;
     }
-    double *cur;
-    if (chain_f32(ctx)) {
-        float *Bf = bblock_f32(m, bj, bj1 - bj);
-        hipLaunchKernelGGL(k_f2d, grid2(nn, R), dim3(256), 0, w.s, Bf, nn, w.W3, nn, nn, R);
-        GemmArgs<double> g = gemm_args<double>(n, n, n, R, s1.T, n, s1.sU, 0, w.W3, n, nn, 0, w.W1, n, nn);
-        g.ar = vec(Dsrc, sDsrc);
-        if (gemm(w, g)) return 1;
-        cur = w.W1;
-    } else if (m.BB && m.bb_valid) {
-        // diag(D) T Bblock_j with the block product the left chain formed
-        GemmArgs<double> g = gemm_args<double>(n, n, n, R, s1.T, n, s1.sU, 0, m.BB + (long long)j * R * nn, n, nn, 0,
-                                               w.W1, n, nn);
-        g.ar = vec(Dsrc, sDsrc);
-        if (gemm(w, g)) return 1;
-        cur = w.W1;
-    } else {
-        // diag(D) T B_top ... B_bj, B applied one slice at a time from the right
-        const int top = bj1 - 1;
-        GemmArgs<double> g = gemm_args<double>(n, n, n, R, s1.T, n, s1.sU, 0, m.expK, n, 0, 0, w.W1, n, nn);
-        g.ar = vec(Dsrc, sDsrc);
-        g.cc = vec<double>(m.dvec + (long long)top * n, sV);
-        if (gemm(w, g)) return 1;
-        cur = w.W1;
-        double *oth = w.W2;
-        for (int l = top - 1; l >= bj; l--) {
-            g = gemm_args<double>(n, n, n, R, cur, n, nn, 0, m.expK, n, 0, 0, oth, n, nn);
-            g.cc = vec<double>(m.dvec + (long long)l * n, sV);
-            if (gemm(w, g)) return 1;
-            std::swap(cur, oth);
-        }
-    }
-    double *Qb = (cur == w.W1) ? w.W2 : w.W1;
-    if (qr(w, cur, Qb, nn, s0.T, s0.sU, s0.D, s0.sD, w.I1, 1)) return 1;
-    GemmArgs<double> g = gemm_args<double>(n, n, n, R, s1.U, n, s1.sU, 0, Qb, n, nn, 0, s0.U, n, s0.sU);
-    if (gemm(w, g)) return 1;
-    hipLaunchKernelGGL(k_mul_q, dim3((R + 255) / 256), dim3(256), 0, w.s, s1.q, s1.sq, w.I1, 1LL, s0.q,
-                       s0.sq, R, w.status);
-    if (launch_check("k_mul_q")) return 1;
+    GemmArgs<double> g = gemm_args<double>(n, n, n, R, Bj, n, nn, 1, s1.U, n, s1.sU, 0, w.W1, n, nn);
+    g.bc = vec(Dsrc, sDsrc);
+    if (gemm(w, g)) return 1; // W1 = Bblock_j^T U diag(D)
+    // U' = Q, D' = diag R, T' = (D'^-1 R) P^T T
+    if (qr_update(w, w.W1, s0) || update_T(w, s1, s0)) return 1;
     return m.centered ? udv_recenter(m, s0, s0) : 0;
 }
 
@@ -1491,18 +1656,58 @@ static int udv_identity(Model &m, const Udv &u)
     return launch_check("k_udv_identity");
 }
 
-// G <- Gnew with drift bookkeeping and the FP32 copy
-static int stage_accept_rebuild(dqmc_gpu_ctx *ctx, Model &m, int record_drift, int tau)
+// G <- Gnew after a rebuild with the FP32 copy, the sign reset, the drift
+// guard and, when record is set, the distance max|G_wrap - Gnew| as a
+// drift sample (DQMC_GPU_DRIFT_REBUILD, and DQMC_GPU_DRIFT_SCRATCH where
+// the rebuild is itself the from-scratch G of tau).  A non-finite wrapped
+// G (check_old: every rebuild of a sweep) fails the replica
+// (DQMC_GPU_FAIL_NONFINITE), as does a non-finite rebuilt one.
+static int stage_accept_rebuild(dqmc_gpu_ctx *ctx, Model &m, int record, int tau, int check_old = 1)
 {
     ProfScope ps_(DQMC_GPU_PROF_ACCEPT, m.w.s);
     const long long nn = (long long)m.n * m.n;
-    const int drift = record_drift && ctx->opts.drift ? 1 : 0;
+    const double guard = ctx->opts.drift_guard > 0.0 ? ctx->opts.drift_guard : 0.0;
+    const int rec = record && ctx->opts.drift != DQMC_GPU_DRIFT_OFF ? 1 : 0;
+    const int measure = rec || guard > 0.0 ? 1 : 0;
     hipLaunchKernelGGL(k_drift_copy, grid2(nn / 4, m.nrep), dim3(256), 0, m.w.s, m.G, m.Gf, m.Gnew, nn, m.n, m.nrep,
-                       drift, is_mixed(ctx) ? 1 : 0, m.dtmax, m.dtbad, m.w.status);
+                       measure && check_old, check_old, is_mixed(ctx) ? 1 : 0, 1, m.dtmax, m.dtbad, m.w.status);
     if (launch_check("k_drift_copy")) return 1;
-    hipLaunchKernelGGL(k_drift_finish, dim3((m.nrep + 255) / 256), dim3(256), 0, m.w.s, m.nrep, drift, m.dtmax,
+    hipLaunchKernelGGL(k_drift_finish, dim3((m.nrep + 255) / 256), dim3(256), 0, m.w.s, m.nrep, rec && check_old,
+                       check_old ? guard : 0.0, m.dtmax,
                        m.dtbad, m.dmax, m.dsum, m.dcnt, m.w.status, m.w.reason, m.sign, tau, m.sweeps, m.dtaumax,
                        m.dsweepmax);
+    return launch_check("k_drift_finish");
+}
+
+// dqmc_record_stab_drift (src/dqmc.c:64-98) with DQMC_GPU_DRIFT_SCRATCH:
+// G(tau) from scratch, B_{tau-1} ... B_0 B_{L-1} ... B_tau accumulated in
+// blocks of stab slices from tau as green_from_scratch does
+// (src/green.c:262-306), into Gnew from the drift UDV, and the distance to
+// the wrapped G recorded as a drift sample; G itself is left alone.  tau is
+// a block boundary and every cached block product is current (the fields of
+// each block are final when its product is formed).
+static int stage_drift_scratch(dqmc_gpu_ctx *ctx, Model &m, int tau)
+{
+    if (ctx->opts.drift != DQMC_GPU_DRIFT_SCRATCH) return 0;
+    if (!m.XU || !m.BB || !m.bb_valid) {
+        set_err("from-scratch drift needs the FP64 block products");
+        return 1;
+    }
+    const long long nn = (long long)m.n * m.n;
+    const Udv x = m.xdrift();
+    if (udv_identity(m, x)) return 1;
+    const int j0 = tau / m.stab;
+    for (int k = 0; k < m.M; k++) {
+        const int j = (j0 + k) % m.M, lb = j * m.stab;
+        if (stage_lmul(ctx, m, x, x, lb, std::min(m.stab, m.L - lb), m.BB + (long long)j * m.nrep * nn)) return 1;
+    }
+    if (udv_inv_one_plus(m.w, x, m.Gnew, nn)) return 1;
+    hipLaunchKernelGGL(k_drift_copy, grid2(nn / 4, m.nrep), dim3(256), 0, m.w.s, m.G, m.Gf, m.Gnew, nn, m.n, m.nrep,
+                       1, 0, is_mixed(ctx) ? 1 : 0, 0, m.dtmax, m.dtbad, m.w.status);
+    if (launch_check("k_drift_copy")) return 1;
+    hipLaunchKernelGGL(k_drift_finish, dim3((m.nrep + 255) / 256), dim3(256), 0, m.w.s, m.nrep, 1, 0.0, m.dtmax,
+                       m.dtbad, m.dmax, m.dsum, m.dcnt, m.w.status, m.w.reason, (double *)nullptr, tau, m.sweeps,
+                       m.dtaumax, m.dsweepmax);
     return launch_check("k_drift_finish");
 }
 
@@ -1517,7 +1722,8 @@ static int stage_from_scratch(dqmc_gpu_ctx *ctx, Model &m, int l0)
         for (int lb = sg[0]; lb < sg[1]; lb += m.stab)
             if (stage_lmul(ctx, m, lf, lf, lb, std::min(m.stab, sg[1] - lb))) return 1;
     if (udv_inv_one_plus(m.w, lf, m.Gnew, (long long)m.n * m.n)) return 1;
-    return stage_accept_rebuild(ctx, m, 0, l0);
+    // G before this rebuild may be anything (unset at init)
+    return stage_accept_rebuild(ctx, m, 0, l0, 0);
 }
 
 // G <- B_l^-1 G B_l = diag(1/d_l) expKinv G expK diag(d_l)
@@ -1551,18 +1757,21 @@ static int stage_wrap_back(dqmc_gpu_ctx *ctx, Model &m, int l)
     return gemm(m.w, g);
 }
 
-// G at a block boundary from a prefix and a suffix UDV
-// (green_from_boundary_factors, src/green.c:308-345), then drift + copy
+// G at a block boundary from a prefix and a (transposed) suffix UDV
+// (green_from_boundary_factors, src/green.c:308-345), after the drift
+// sample against the from-scratch G (DQMC_GPU_DRIFT_SCRATCH), then the
+// copy with the guard
 static int stage_boundary_rebuild(dqmc_gpu_ctx *ctx, Model &m, const Udv &prefix, const Udv &suffix, int tau)
 {
     const long long nn = (long long)m.n * m.n;
+    if (stage_drift_scratch(ctx, m, tau)) return 1;
     if (m.two_sided) {
         if (udv_inv_two_sided(m.w, prefix, suffix, m.Gnew, nn)) return 1;
     } else {
-        if (udv_combine(m.w, prefix, suffix, m.comb())) return 1;
+        if (udv_combine(m.w, prefix, suffix, m.comb(), 1)) return 1;
         if (udv_inv_one_plus(m.w, m.comb(), m.Gnew, nn)) return 1;
     }
-    return stage_accept_rebuild(ctx, m, 1, tau);
+    return stage_accept_rebuild(ctx, m, ctx->opts.drift == DQMC_GPU_DRIFT_REBUILD, tau);
 }
 
 // Sweep variants: the plain forward sweep, and the two legs of the
@@ -1637,6 +1846,9 @@ static int sweep_forward(dqmc_gpu_ctx *ctx, Model &m, bool build_suffix, bool ca
     if (m.L - tau_left > 0) {
         if (stage_lmul(ctx, m, carry ? m.alt(tau_left / m.stab) : lf, fin, tau_left, m.L - tau_left)) return 1;
     }
+    // the end rebuild reads the whole chain B_{L-1} ... B_0 accumulated from
+    // the identity in blocks of stab slices: it is the from-scratch G(0) of
+    // the drift samples as well
     if (udv_inv_one_plus(m.w, fin, m.Gnew, nn)) return 1;
     return stage_accept_rebuild(ctx, m, 1, 0);
 }
@@ -1683,8 +1895,11 @@ static int sweep_backward(dqmc_gpu_ctx *ctx, Model &m, bool build_prefix)
         if (stage_wrap_back(ctx, m, l - 1)) return 1;
     }
     if (stage_suffix_block(ctx, m, 0)) return 1;
-    if (udv_inv_one_plus(m.w, m.stack(0), m.Gnew, nn)) return 1;
-    return stage_accept_rebuild(ctx, m, 1, 0);
+    // G(0) = (1 + S[0])^-1 with S[0] stored transposed: the two-sided form
+    // with the identity prefix A[0]
+    if (stage_drift_scratch(ctx, m, 0)) return 1;
+    if (udv_inv_two_sided(m.w, m.alt(0), m.stack(0), m.Gnew, nn)) return 1;
+    return stage_accept_rebuild(ctx, m, ctx->opts.drift == DQMC_GPU_DRIFT_REBUILD, 0);
 }
 
 static int sweep_body(dqmc_gpu_ctx *ctx, Model &m, int v)
@@ -1828,9 +2043,27 @@ int dqmc_gpu_create(dqmc_gpu_ctx **out, int device, const dqmc_gpu_opts *o)
     c->opts.use_graph = 1;
     c->opts.drift = 0;
     c->opts.autotune = 0;
+    c->opts.drift_guard = 0.0;
     if (o) c->opts = *o;
     if (c->opts.precision < DQMC_GPU_FP64 || c->opts.precision > DQMC_GPU_MIXED_FP32_CHAIN) {
         set_err("unknown precision %d", c->opts.precision);
+        delete c;
+        return 1;
+    }
+    if (c->opts.drift < DQMC_GPU_DRIFT_OFF || c->opts.drift > DQMC_GPU_DRIFT_REBUILD) {
+        set_err("unknown drift mode %d", c->opts.drift);
+        delete c;
+        return 1;
+    }
+    if (!std::isfinite(c->opts.drift_guard) || c->opts.drift_guard < 0.0) {
+        set_err("drift_guard must be a finite non-negative bound");
+        delete c;
+        return 1;
+    }
+    // the FP32 wrap and delayed updates can leave G far from the rebuilt one
+    // (at 16x16, U=4, beta=8 by O(10)): a mixed context needs the guard
+    if (c->opts.precision != DQMC_GPU_FP64 && !(c->opts.drift_guard > 0.0)) {
+        set_err("mixed precision needs a drift guard (opts.drift_guard > 0)");
         delete c;
         return 1;
     }
@@ -2497,33 +2730,38 @@ int dqmc_gpu_udv_rmul(int device, int n, int batch, double *U, double *D, double
     return hb_lr_mul(device, n, batch, U, D, T, B, 0);
 }
 
-// Orthogonal U from outside the engine: its determinant sign is not tracked,
-// so it is taken from an LU of U on the host (la_logdet, src/linalg.c:255-286).
-static int host_det_sign(int n, const double *A)
+// sgn det A = sgn det Q * prod sgn R_ii from a QR of A, one workgroup per
+// batch entry; 0 when a diagonal entry of R is zero or not finite
+__global__ void k_qr_det_sign(const int *qsign, const double *D, int n, int *out, int batch, const int *status)
 {
-    std::vector<double> M(A, A + (size_t)n * n);
-    int s = 1;
-    for (int j = 0; j < n; j++) {
-        int p = j;
-        double best = std::fabs(M[j + (size_t)j * n]);
-        for (int i = j + 1; i < n; i++)
-            if (std::fabs(M[i + (size_t)j * n]) > best) {
-                best = std::fabs(M[i + (size_t)j * n]);
-                p = i;
-            }
-        if (M[p + (size_t)j * n] == 0.0) return 0;
-        if (p != j) {
-            s = -s;
-            for (int k = 0; k < n; k++) std::swap(M[j + (size_t)k * n], M[p + (size_t)k * n]);
-        }
-        const double piv = M[j + (size_t)j * n];
-        if (piv < 0.0) s = -s;
-        for (int i = j + 1; i < n; i++) {
-            const double f = M[i + (size_t)j * n] / piv;
-            for (int k = j + 1; k < n; k++) M[i + (size_t)k * n] -= f * M[j + (size_t)k * n];
-        }
+    const int b = blockIdx.x;
+    if (b >= batch || status[b]) return;
+    __shared__ int neg, bad;
+    if (threadIdx.x == 0) {
+        neg = 0;
+        bad = 0;
     }
-    return s;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double d = D[(long long)b * n + i];
+        if (d < 0.0) atomicXor(&neg, 1);
+        if (!(d != 0.0) || !isfinite(d)) bad = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) out[b] = bad ? 0 : qsign[b] * (neg ? -1 : 1);
+}
+
+// Orthogonal U from outside the engine carries no tracked determinant sign:
+// it is taken on the device from a QR of U (the reference takes it from an
+// LU, la_logdet, src/linalg.c:255-286); U is copied first, the QR destroys
+// its input.
+static int device_det_sign(Work &w, const double *U, int *out)
+{
+    const long long nn = (long long)w.n * w.n;
+    if (copy_nn(w, U, nn, w.W4, nn, nn)) return 1;
+    if (qr(w, w.W4, nullptr, 0, nullptr, 0, w.V5, w.n, w.I2, 1)) return 1;
+    hipLaunchKernelGGL(k_qr_det_sign, dim3(w.nb), dim3(256), 0, w.s, w.I2, w.V5, w.n, out, w.nb, w.status);
+    return launch_check("k_qr_det_sign");
 }
 
 int dqmc_gpu_udv_combine(int device, int n, int batch, const double *Ul, const double *Dl, const double *Tl,
@@ -2545,9 +2783,7 @@ int dqmc_gpu_udv_combine(int device, int n, int batch, const double *Ul, const d
         if (hb_upload(h, src[k], cnt[k] * batch, &a[k])) return 1;
     for (int k = 0; k < 3; k++)
         if (hb_int(h, batch, &q[k])) return 1;
-    std::vector<int> qs(batch);
-    for (int b = 0; b < batch; b++) qs[b] = host_det_sign(n, Ul + nn * b);
-    HIPCHK(hipMemcpy(q[0], qs.data(), batch * sizeof(int), hipMemcpyHostToDevice));
+    if (device_det_sign(h.w, a[0], q[0])) return 1;
     const Udv l{a[0], a[1], a[2], q[0], (long long)nn, n, 1};
     const Udv r{a[3], a[4], a[5], q[1], (long long)nn, n, 1};
     const Udv o{a[6], a[7], a[8], q[2], (long long)nn, n, 1};
@@ -2558,8 +2794,8 @@ int dqmc_gpu_udv_combine(int device, int n, int batch, const double *Ul, const d
     return 0;
 }
 
-int dqmc_gpu_udv_inv_one_plus(int device, int n, int batch, const double *U, const double *D, const double *T,
-                              double *g, int *sign)
+static int inv_one_plus_batch(int device, int n, int batch, const double *U, const double *D, const double *T,
+                              const double *off, double *g, int *sign, int *reason)
 {
     if (!U || !D || !T || !g) {
         set_err("null argument");
@@ -2568,19 +2804,39 @@ int dqmc_gpu_udv_inv_one_plus(int device, int n, int batch, const double *U, con
     HostBatch h;
     if (hb_init(h, device, n, batch)) return 1;
     const size_t nn = (size_t)n * n;
-    double *dU, *dD, *dT, *dG;
+    double *dU, *dD, *dT, *dG, *dO = nullptr;
     int *q;
     if (hb_upload(h, U, nn * batch, &dU) || hb_upload(h, D, (size_t)n * batch, &dD) ||
         hb_upload(h, T, nn * batch, &dT) || hb_upload(h, nullptr, nn * batch, &dG) || hb_int(h, batch, &q))
         return 1;
-    std::vector<int> qs(batch);
-    for (int b = 0; b < batch; b++) qs[b] = host_det_sign(n, U + nn * b);
-    HIPCHK(hipMemcpy(q, qs.data(), batch * sizeof(int), hipMemcpyHostToDevice));
-    const Udv u{dU, dD, dT, q, (long long)nn, n, 1};
-    if (udv_inv_one_plus(h.w, u, dG, (long long)nn) || hb_status(h)) return 1;
+    if (off && hb_upload(h, off, (size_t)batch, &dO)) return 1;
+    if (device_det_sign(h.w, dU, q)) return 1;
+    const Udv u{dU, dD, dT, q, (long long)nn, n, 1, dO, 1};
+    if (udv_inv_one_plus(h.w, u, dG, (long long)nn)) return 1;
+    if (reason) {
+        HIPCHK(hipStreamSynchronize(h.w.s));
+        HIPCHK(hipMemcpy(reason, h.w.reason, batch * sizeof(int), hipMemcpyDeviceToHost));
+    }
+    if (hb_status(h)) return 1;
     HIPCHK(hipMemcpy(g, dG, nn * batch * 8, hipMemcpyDeviceToHost));
     if (sign) HIPCHK(hipMemcpy(sign, h.w.det_sign, batch * sizeof(int), hipMemcpyDeviceToHost));
     return 0;
+}
+
+int dqmc_gpu_udv_inv_one_plus(int device, int n, int batch, const double *U, const double *D, const double *T,
+                              double *g, int *sign)
+{
+    return inv_one_plus_batch(device, n, batch, U, D, T, nullptr, g, sign, nullptr);
+}
+
+int dqmc_gpu_udv_inv_one_plus_off(int device, int n, int batch, const double *U, const double *D, const double *T,
+                                  const double *log_offset, double *g, int *sign, int *reason)
+{
+    if (!log_offset) {
+        set_err("null argument");
+        return 1;
+    }
+    return inv_one_plus_batch(device, n, batch, U, D, T, log_offset, g, sign, reason);
 }
 
 int dqmc_gpu_measure(int device, int Lx, int Ly, int batch, const double *gu, const double *thop,

@@ -64,13 +64,21 @@ SIGNATURES = {
     "dqmc_gpu_udv_rmul": (_I, [_I, _I, _I, _P, _P, _P, _P]),
     "dqmc_gpu_udv_combine": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "dqmc_gpu_udv_inv_one_plus": (_I, [_I, _I, _I, _P, _P, _P, _P, _P]),
+    "dqmc_gpu_udv_inv_one_plus_off": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "dqmc_gpu_measure": (_I, [_I, _I, _I, _I, _P, _P, _P, _D, _I, _P, _P, _P, _P]),
     "dqmc_gpu_last_error": (C.c_char_p, []),
 }
 
 
+DRIFT = {False: 0, None: 0, "off": 0, True: 1, "scratch": 1, "rebuild": 2}
+FAIL_NONFINITE = 16
+FAIL_SIGN = 18
+FAIL_DRIFT = 19
+
+
 class _Opts(C.Structure):
-    _fields_ = [("precision", C.c_int), ("use_graph", C.c_int), ("drift", C.c_int), ("autotune", C.c_int)]
+    _fields_ = [("precision", C.c_int), ("use_graph", C.c_int), ("drift", C.c_int), ("autotune", C.c_int),
+                ("drift_guard", C.c_double)]
 
 
 def lib():
@@ -219,9 +227,14 @@ class ModelInfo:
 class Engine:
     """One engine context on one device (dqmc_gpu_create ... destroy)."""
 
-    def __init__(self, precision: int = FP64, device: int = 0, use_graph: bool = True, drift: bool = False):
+    def __init__(self, precision: int = FP64, device: int = 0, use_graph: bool = True, drift=False,
+                 drift_guard: float = 0.0, autotune: bool = False):
+        """drift: False / "off", True / "scratch" (the reference's from-scratch
+        G, dqmc_record_stab_drift) or "rebuild" (the rebuilt G itself);
+        drift_guard > 0 fails a replica whose G drifts further (required by
+        the mixed modes)."""
         L = lib()
-        o = _Opts(precision, 1 if use_graph else 0, 1 if drift else 0, 0)
+        o = _Opts(precision, 1 if use_graph else 0, DRIFT[drift], 1 if autotune else 0, float(drift_guard))
         h = C.c_void_p()
         _check(L.dqmc_gpu_create(C.byref(h), device, C.byref(o)), "dqmc_gpu_create")
         self.h = h
@@ -418,6 +431,21 @@ def udv_inv_one_plus(U, D, T, device=0):
     _check(lib().dqmc_gpu_udv_inv_one_plus(device, n, b, _ptr(U), _ptr(D), _ptr(T), _ptr(g), _ptr(sg)),
            "dqmc_gpu_udv_inv_one_plus")
     return g, sg
+
+
+def udv_inv_one_plus_off(U, D, T, off, device=0):
+    """Batched udv_inv_one_plus_work on UDVs with log offsets.  Returns
+    (g, sign, reason, rc): rc != 0 when an entry failed (reason[b] != 0),
+    g and sign then unset."""
+    U, D, T = map(_f, (U, D, T))
+    off = _f(off)
+    b, n = D.shape
+    g = np.zeros_like(U)
+    sg = np.zeros(b, dtype=np.int32)
+    rs = np.zeros(b, dtype=np.int32)
+    rc = lib().dqmc_gpu_udv_inv_one_plus_off(device, n, b, _ptr(U), _ptr(D), _ptr(T), _ptr(off), _ptr(g), _ptr(sg),
+                                             _ptr(rs))
+    return g, sg, rs, rc
 
 
 def measure(Lx, Ly, gu, thop, bipart, U, phase=None, nq=0, device=0):

@@ -35,6 +35,10 @@ typedef struct {
     char precision[24]; /* fp64 | mixed | mixed_fp32_chain */
     int delay;          /* delayed-update depth, 0 = engine default */
     int device;
+    double drift_guard; /* fail a replica whose G drifts further from the
+                           rebuilt one; < 0 = engine default (1e-6 in the
+                           mixed modes, off in fp64) */
+    int autotune;       /* 1 = time the Metropolis launch shapes per model */
 } Params;
 
 /* key=value parser with the reference's defaults and checks

@@ -350,7 +350,11 @@ int main(int argc, char **argv)
             return 1;
         }
         const double beta_eff = Ltr * p.dtau, T = 1.0 / beta_eff;
-        dqmc_gpu_opts o = {precision, 1, p.stab_drift_file[0] ? 1 : 0, 0};
+        /* stab_drift_file: the reference's drift (against a from-scratch G
+           before each rebuild, dqmc_record_stab_drift, src/dqmc.c:64-98) */
+        const double guard = p.drift_guard >= 0.0 ? p.drift_guard : (precision != DQMC_GPU_FP64 ? 1e-6 : 0.0);
+        dqmc_gpu_opts o = {precision, 1, p.stab_drift_file[0] ? DQMC_GPU_DRIFT_SCRATCH : DQMC_GPU_DRIFT_OFF,
+                           p.autotune, guard};
         dqmc_gpu_ctx *ctx = NULL;
         if (dqmc_gpu_create(&ctx, p.device, &o)) return fail("dqmc_gpu_create", NULL);
         int mid, first;
@@ -423,8 +427,7 @@ int main(int argc, char **argv)
         }
         if (p.stab_drift_file[0]) {
             /* append_stab_drift_row (src/replica_run.c:58-88), one row per
-               replica; the reference point of the drift is the stack's
-               rebuilt G at the boundary */
+               replica */
             double *dmx = malloc(sizeof(double) * nrep), *dmn = malloc(sizeof(double) * nrep);
             unsigned long long *dct = malloc(sizeof(unsigned long long) * nrep),
                                *dsw = malloc(sizeof(unsigned long long) * nrep);

@@ -7,7 +7,10 @@
  * Keys that select reference code paths the GPU engine does not implement
  * (lattice=file, profile=1, the UDV diagnostics files) are accepted only
  * at their default values.
- * Additional keys: precision=fp64|mixed|mixed_fp32_chain, delay=<k>, device=<id>.
+ * Additional keys: precision=fp64|mixed|mixed_fp32_chain, delay=<k>, device=<id>,
+ * drift_guard=<bound> (a replica whose G drifts further from the rebuilt G
+ * fails; default 1e-6 in the mixed modes, which require it, off in fp64),
+ * autotune=0|1 (time the Metropolis launch shapes of each model).
  */
 #include "host.h"
 
@@ -46,6 +49,8 @@ static void defaults(Params *p)
     strcpy(p->green_rebuild, "combine");
     p->delay = 0;
     p->device = 0;
+    p->drift_guard = -1.0;
+    p->autotune = 0;
 }
 
 static char *trim(char *s)
@@ -239,6 +244,10 @@ int params_read(Params *p, const char *path)
             if (to_int(val, &p->delay)) BAD("ERROR: delay must be integer (got %s)\n", val);
         } else if (!strcmp(key, "device")) {
             if (to_int(val, &p->device)) BAD("ERROR: device must be integer (got %s)\n", val);
+        } else if (!strcmp(key, "drift_guard")) {
+            if (to_double(val, &p->drift_guard)) BAD("ERROR: drift_guard must be numeric (got %s)\n", val);
+        } else if (!strcmp(key, "autotune")) {
+            if (to_int(val, &p->autotune)) BAD("ERROR: autotune must be integer (got %s)\n", val);
         } else {
             BAD("ERROR: unknown key %s\n", key);
         }
@@ -280,6 +289,8 @@ int params_read(Params *p, const char *path)
     else if (strcmp(p->precision, "fp64") && strcmp(p->precision, "mixed") && strcmp(p->precision, "mixed_fp32_chain"))
         err = "precision must be fp64, mixed or mixed_fp32_chain";
     else if (p->delay < 0) err = "delay must be >= 0";
+    else if (!(p->drift_guard >= 0.0) && p->drift_guard != -1.0) err = "drift_guard must be >= 0";
+    else if (p->autotune != 0 && p->autotune != 1) err = "autotune must be 0 or 1";
     if (err) {
         fprintf(stderr, "ERROR: %s\n", err);
         return 1;

@@ -107,12 +107,25 @@ def test_driver_optional_outputs(tmp_path):
         within(s1[:, 10], s1[:, 11], s0[:, 10], s0[:, 11], f)
     assert (tmp_path / "reps.dat").read_text() == open(gold("reps")).read()
     assert (tmp_path / "hopping_used.txt").read_text() == open(gold("hopping_used")).read()
-    # drift rows: identical identification columns and sample counts; the
-    # drift itself is measured against the stack's rebuilt G here and
-    # against a from-scratch G in the reference, both tiny at this size
+    # drift rows: identical identification columns and sample counts.  Both
+    # sides measure max|G_wrap - G_scratch(tau)| against a from-scratch G
+    # before every rebuild (dqmc_record_stab_drift, src/dqmc.c:64-98); the
+    # values are rounding (1e-14 .. 1e-10 here), so they agree in size, not
+    # digit for digit: the mean over 4400 / 6600 samples within a factor 20
+    # of the reference's.  The maximum is a single sample, the one accepted
+    # flip with the smallest ratio amplifying that slice's rounding: two FP64
+    # implementations of the same chain differ there by orders of magnitude
+    # (the oracle's maxima on these four replicas are 4e-12 .. 4e-11, the
+    # reference's 8e-12 .. 4e-11, the engine's 2e-11 .. 6e-9), so it is held
+    # to the absolute bound of the reference's drift test
+    # (tests/test_dqmc_stab_drift.c:55-57, < 1e-8)
     g0, d0 = split(gold("drift"))
     g1, d1 = split(tmp_path / "drift.dat")
+    print("reference drift max/mean:", d0[:, 9], d0[:, 10])
+    print("engine drift max/mean:   ", d1[:, 9], d1[:, 10], flush=True)
     assert g1 == g0 and d1.shape == d0.shape == (4, 14)
     assert np.array_equal(d1[:, :9], d0[:, :9]) and np.array_equal(d1[:, 13], d0[:, 13])
-    assert (d1[:, 9] < 1e-8).all() and (d1[:, 10] <= d1[:, 9]).all()
+    assert (d1[:, 10] <= d1[:, 9]).all()
+    assert ((d1[:, 10] > 0.05 * d0[:, 10]) & (d1[:, 10] < 20.0 * d0[:, 10])).all()
+    assert (d1[:, 9] < 1e-8).all()
     assert ((d1[:, 11] >= 0) & (d1[:, 11] < d1[:, 1])).all() and (d1[:, 12] < 1100).all()

@@ -23,7 +23,8 @@ class Params(C.Structure):
                 ("sperp_file", C.c_char * 256), ("stab_drift_file", C.c_char * 256),
                 ("seed", C.c_ulonglong), ("replica_log", C.c_char * 256), ("spin_consistency_file", C.c_char * 256),
                 ("sweep_order", C.c_char * 16), ("green_rebuild", C.c_char * 16),
-                ("precision", C.c_char * 24), ("delay", C.c_int), ("device", C.c_int)]
+                ("precision", C.c_char * 24), ("delay", C.c_int), ("device", C.c_int),
+                ("drift_guard", C.c_double), ("autotune", C.c_int)]
 
 
 class Lattice(C.Structure):

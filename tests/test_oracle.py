@@ -403,3 +403,54 @@ def test_oracle_vs_live_reference_alternating(orc):
             assert np.array_equal(a.field(), b.field())
             ga, gb = a.green()[0], b.green()[0]
             assert np.abs(ga - gb).max() / np.abs(gb).max() < 1e-11
+
+
+# ------------------------------------------------ extended-precision G
+
+def test_exact_green_matches_mpmath(orc):
+    # oracle/exact.c against the direct formula (1 + B..B)^-1 in mpmath at
+    # 50 digits (tests/golden/make_exact.py): chain 4 (beta=4), 4x4 at tau=0
+    # and tau=10, 6x6, 8x8 (beta=4); the long double's own rounding, grown to
+    # ~2e-15 over 40 single-slice QR steps at 8x8, and the FP64 rounding of
+    # the stored values are the only differences
+    d = gold("exact_g.npz")
+    for k in range(int(d["ncases"])):
+        U, dtau, L, tau, lam = d[f"case{k}_params"]
+        G = d[f"case{k}_G"]
+        for blk in (1, 5, 8):
+            g = orc.green_exact(d[f"case{k}_expK"], lam, U, dtau, d[f"case{k}_field"], int(L), int(tau), blk)
+            assert np.abs(g - G).max() / np.abs(G).max() < 1e-14, (k, blk)
+
+
+def test_exact_green_u0_and_shifted_tau(orc):
+    # U = 0: every B_l = expK, G(tau) = (1 + exp(-beta K))^-1 for all tau
+    # (tests/test_green_init.c:29-46), here on 8x8 at beta = 4, where the
+    # closed form by eigh is good to ~1e-15
+    L, dtau = 40, 0.1
+    r = orc.Replica("square", 8, 8, 0.0, dtau, L, 10, 3)
+    eK, _, lam = r.model()
+    K = r.thop - 0.0 * np.eye(64)
+    w, V = np.linalg.eigh(K)
+    Gx = (V * (1.0 / (1.0 + np.exp(-L * dtau * w)))) @ V.T
+    for tau in (0, 17):
+        g = orc.green_exact(eK, lam, 0.0, dtau, r.field(), L, tau, 10)
+        assert np.abs(g - Gx).max() < 1e-13, tau
+
+
+def test_exact_green_wraps_like_g(orc):
+    # G(tau + 1) = B_tau G(tau) B_tau^-1 exactly (src/green.c:509-527): the
+    # extended-precision G at two neighbouring slices related by the FP64
+    # wrap of the oracle, on an 8x8 field at beta = 2
+    n, L = 64, 20
+    r = orc.Replica("square", 8, 8, 4.0, 0.1, L, 5, 11)
+    r.sweep()
+    eK, eKi, lam = r.model()
+    f = r.field()
+    g3 = orc.green_exact(eK, lam, 4.0, 0.1, f, L, 3, 5)
+    g4 = orc.green_exact(eK, lam, 4.0, 0.1, f, L, 4, 5)
+    ev = np.zeros(2)
+    orc.orc().orc_expv(lam, 4.0, 0.1, 1.0, P(ev))
+    g = np.asfortranarray(g3.copy())
+    orc.orc().orc_wrap(n, P(np.asfortranarray(eK)), P(np.asfortranarray(eKi)), P(ev),
+                       P(np.ascontiguousarray(f[3 * n:4 * n])), P(g))
+    assert np.abs(g - g4).max() / np.abs(g4).max() < 1e-12
