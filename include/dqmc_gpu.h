@@ -258,6 +258,24 @@ int dqmc_gpu_measure(int device, int Lx, int Ly, int batch, const double *gu,
                      const double *phase_cos, double *scal5, double *szz,
                      double *sperp);
 
+/* No counterpart in the reference: dqmc_gpu_udv_inv_one_plus for callers
+   that cannot allocate device memory (code under HIP graph capture, tight
+   serving loops, harnesses that forbid hipMalloc).  U, D, T (inputs), g
+   (output, batch * n * n), sign (output, batch ints: the det sign, 0 for an
+   entry that failed) and reason (output, batch ints or NULL: the failure
+   reason, 0 = ok) are DEVICE pointers owned by the caller; the work is
+   enqueued on `stream` (a hipStream_t, NULL for the null stream) and the
+   call returns without waiting for it.  Nothing is allocated: the workspace
+   is a static device array of 64 MiB per device, held until the process
+   ends, which bounds the problem (about 4 n^2 + 6 n + 8192 doubles per
+   batch entry: batch <= 29 at n = 256, <= 460 at n = 64); larger calls are
+   refused with an error.  All calls on one device share that workspace:
+   calls that overlap on one device race. */
+int dqmc_gpu_udv_inv_one_plus_noalloc(int n, int batch, const double *U,
+                                      const double *D, const double *T,
+                                      double *g, int *sign, int *reason,
+                                      void *stream);
+
 const char *dqmc_gpu_last_error(void);
 
 #ifdef __cplusplus

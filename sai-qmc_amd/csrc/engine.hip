@@ -1969,8 +1969,15 @@ static int stage_measure(Model &m, int bin)
 
 static int set_lds_limits()
 {
-    static bool done = false;
-    if (done) return 0;
+    // function attributes are set once per device (the current one)
+    static bool done_dev[64] = {};
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) {
+        set_err("device %d beyond the engine's table of 64", dev);
+        return 1;
+    }
+    if (done_dev[dev]) return 0;
     // 160 KiB of LDS per CU on gfx950, less each kernel's static LDS
     const void *kernels[] = {(const void *)k_metro_slice<double, 16>,
                              (const void *)k_metro_slice<double, 8>,  (const void *)k_metro_slice<float, 32>,
@@ -1990,7 +1997,7 @@ static int set_lds_limits()
         const int dyn = 160 * 1024 - (int)at.sharedSizeBytes;
         HIPCHK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
     }
-    done = true;
+    done_dev[dev] = true;
     return 0;
 }
 
@@ -2873,6 +2880,123 @@ int dqmc_gpu_measure(int device, int Lx, int Ly, int batch, const double *gu, co
         HIPCHK(hipMemcpy(sperp, dp, (size_t)nq * batch * 8, hipMemcpyDeviceToHost));
     }
     return 0;
+}
+
+// ---------------------------------------- inverse without allocations
+//
+// dqmc_gpu_udv_inv_one_plus_noalloc: the batched (1 + U D T)^-1 on device
+// buffers the caller owns, for callers that cannot allocate (code under
+// graph capture, serving loops, harnesses that forbid hipMalloc).  Its
+// workspace is the file-scope device array `scratch`, NOALLOC_BYTES long;
+// each device holds its own copy, whose address hipGetSymbolAddress gives
+// once per device (noalloc_scratch) and which host code then uses in
+// place of the array's name.  All calls on one device share that copy:
+// calls that overlap on one device race.
+
+#define NOALLOC_BYTES (64u << 20)
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+static void *g_noalloc_ptr[64];
+static bool g_noalloc_ready[64];
+
+static int noalloc_scratch(void **ptr)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) {
+        set_err("device %d beyond the engine's table of 64", dev);
+        return 1;
+    }
+    if (!g_noalloc_ready[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_err("hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+            return 1;
+        }
+        if (set_lds_limits()) return 1;
+        g_noalloc_ptr[dev] = p;
+        g_noalloc_ready[dev] = true;
+    }
+    *ptr = g_noalloc_ptr[dev];
+    return 0;
+}
+
+// bytes of scratch for n, batch, each piece 256-byte aligned
+static size_t noalloc_layout(int n, int batch, Work *w, int **q, unsigned char *base)
+{
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> unsigned char * {
+        unsigned char *p = base ? base + off : nullptr;
+        off += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    const size_t nn = (size_t)n * n * batch, nv = (size_t)n * batch;
+    const int npan = (n + QR_NB - 1) / QR_NB;
+    double **mats[] = {&w->W1, &w->W2, &w->W3, &w->W4};
+    for (double **m : mats) *m = (double *)take(nn * 8);
+    double **vecs[] = {&w->V1, &w->V2, &w->V3, &w->V4, &w->V5};
+    for (double **v : vecs) *v = (double *)take(nv * 8);
+    w->Tws = (double *)take((size_t)npan * QR_NB * QR_NB * batch * 8);
+    int **ints[] = {&w->I1, &w->I2, &w->status, &w->reason, &w->det_sign, q};
+    for (int **v : ints) *v = (int *)take((size_t)batch * 4);
+    w->perm = (int *)take(nv * 4);
+    return off;
+}
+
+// status and reason cleared by a kernel (no memset node under capture)
+__global__ void k_clear_status(int *status, int *reason, int batch)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < batch) {
+        status[b] = 0;
+        reason[b] = 0;
+    }
+}
+
+// sign 0 for a failed entry; the reason copied out when asked for
+__global__ void k_noalloc_out(const int *det_sign, const int *status, const int *reason, int *sign, int *reason_out,
+                              int batch)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch) return;
+    sign[b] = status[b] ? 0 : det_sign[b];
+    if (reason_out) reason_out[b] = status[b] ? reason[b] : 0;
+}
+
+int dqmc_gpu_udv_inv_one_plus_noalloc(int n, int batch, const double *U, const double *D, const double *T, double *g,
+                                      int *sign, int *reason, void *stream)
+{
+    if (!U || !D || !T || !g || !sign) {
+        set_err("null argument");
+        return 1;
+    }
+    if (n <= 0 || n > METRO_MAX_N || batch <= 0) {
+        set_err("invalid n=%d batch=%d", n, batch);
+        return 1;
+    }
+    Work w;
+    int *q = nullptr;
+    const size_t need = noalloc_layout(n, batch, &w, &q, nullptr);
+    if (need > NOALLOC_BYTES) {
+        set_err("n=%d batch=%d needs %zu bytes of workspace, above the %u of dqmc_gpu_udv_inv_one_plus_noalloc", n,
+                batch, need, NOALLOC_BYTES);
+        return 1;
+    }
+    void *base = nullptr;
+    if (noalloc_scratch(&base)) return 1;
+    noalloc_layout(n, batch, &w, &q, (unsigned char *)base);
+    w.n = n;
+    w.nb = batch;
+    w.s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_clear_status, dim3((batch + 255) / 256), dim3(256), 0, w.s, w.status, w.reason, batch);
+    if (launch_check("k_clear_status")) return 1;
+    if (device_det_sign(w, U, q)) return 1;
+    const Udv u{const_cast<double *>(U), const_cast<double *>(D), const_cast<double *>(T), q, (long long)n * n, n, 1};
+    if (udv_inv_one_plus(w, u, g, (long long)n * n)) return 1;
+    hipLaunchKernelGGL(k_noalloc_out, dim3((batch + 255) / 256), dim3(256), 0, w.s, w.det_sign, w.status, w.reason,
+                       sign, reason, batch);
+    return launch_check("k_noalloc_out");
 }
 
 } // extern "C"

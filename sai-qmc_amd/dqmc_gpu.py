@@ -66,6 +66,7 @@ SIGNATURES = {
     "dqmc_gpu_udv_inv_one_plus": (_I, [_I, _I, _I, _P, _P, _P, _P, _P]),
     "dqmc_gpu_udv_inv_one_plus_off": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "dqmc_gpu_measure": (_I, [_I, _I, _I, _I, _P, _P, _P, _D, _I, _P, _P, _P, _P]),
+    "dqmc_gpu_udv_inv_one_plus_noalloc": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "dqmc_gpu_last_error": (C.c_char_p, []),
 }
 
