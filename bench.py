@@ -37,11 +37,11 @@ HBM = 8.0e12  # MI355X HBM3E bytes/s
 
 
 def run_engine(dq, precision, steps, warmup, drift=False, guard=0.0, groups=1, delay=DELAY, order="forward",
-               profile=True, allow_fail=False):
+               profile=True, allow_fail=False, autotune=True):
     """The 128 replicas as `groups` equal models: each model has its own
     stream and sweep graph, so the groups' sweeps run concurrently.
     order = "alternating" runs forward and backward sweeps in turn."""
-    eng = dq.Engine(precision, drift=drift, drift_guard=guard)
+    eng = dq.Engine(precision, drift=drift, drift_guard=guard, autotune=autotune)
     L = int(round(BETA / DTAU))
     per = NREP // groups
     for g in range(groups):
@@ -50,6 +50,7 @@ def run_engine(dq, precision, steps, warmup, drift=False, guard=0.0, groups=1, d
         eng.add_replicas(mid, list(range(1 + g * per, 1 + (g + 1) * per)))
         eng.set_measure(mid, nbin=1, q="all")
     eng.init()
+    tuning = eng.tuning(0)
     eng.sweep(warmup)
     eng.sweep(steps, 0)
     ms = eng.elapsed_ms()
@@ -58,7 +59,7 @@ def run_engine(dq, precision, steps, warmup, drift=False, guard=0.0, groups=1, d
     if nfail and not allow_fail:
         raise RuntimeError(f"replica failure: reasons {sorted(set(st['reason'].tolist()))}")
     acc = float(st["accepted"].sum()) / max(1.0, float(st["attempts"].sum()))
-    out = dict(ms=ms, value=NREP * steps / (ms / 1e3), acc=acc, failed=nfail,
+    out = dict(ms=ms, value=NREP * steps / (ms / 1e3), acc=acc, failed=nfail, tuning=tuning,
                reasons=sorted(set(int(x) for x in st["reason"][st["status"] != 0])))
     if drift:
         mx, mn, cnt = eng.drift()
@@ -84,37 +85,41 @@ def reference_drift():
             "source": "reference dqmc_omp stab_drift_file, 4 replicas x (2 warm-up + 4 measured) sweeps"}
 
 
-def traffic_from_profile(kernel):
+def traffic_from_profile(kernel, segments=None):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter reads
-    for a device copy of known size)."""
+    for a device copy of known size); None when the summary was taken with
+    another number of Metropolis site segments than this run's."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_16x16_beta8_128rep_pmc.txt")
     if not os.path.exists(path):
         return None
+    val, seg = None, None
     for line in open(path):
         if line.startswith("hbm_bytes_per_launch"):
-            return float(line.split()[-1])
-    return None
+            val = float(line.split()[-1])
+        if line.startswith("metro_segments"):
+            seg = int(line.split()[-1])
+    if segments is not None and seg != segments:
+        return None
+    return val
 
 
-def roofline_metro_kernel(prof, acc, mixed, nrep_launch=NREP):
+def roofline_metro_kernel(prof, acc, segments, window, nrep_launch=NREP):
     """Roofline of the dominant kernel by total time, the Metropolis kernel
-    (one launch per half-slice site segment).  Algorithmic bytes per replica
-    and launch: the window columns and rows of G it corrects, n^2 values for
-    the n/2 sites of a segment, plus the pending pairs it writes, a n/2
-    pairs of 2n values (a = acceptance).  Launch duration from HIP events of
-    a profiled sweep: the Metropolis stage minus its flush GEMMs, over the
-    number of flushes (one per launch)."""
+    (one launch per site segment of n/S sites, S segments per slice).
+    Algorithmic bytes per replica and launch: the window columns and rows of
+    G it corrects, 2 n^2 / S values, plus the pending pairs it writes,
+    a n / S pairs of 2n values (a = acceptance), 8 bytes each.  Launch
+    duration from HIP events around every proposal launch of a profiled
+    sweep."""
     n = LX * LY
-    ms, _ = prof["metro"]
-    fms, fcnt = prof["flush"]
-    t = (ms - fms) / fcnt / 1e3
-    w = 4 if mixed else 8
-    byts = (n * n + acc * (n // 2) * 2 * n) * w * nrep_launch
-    return {"bound": "hbm", "kernel": "k_metro_slice<float, 32>" if mixed else "k_metro_slice<double, 16>",
+    pms, fcnt = prof["propose"]
+    t = pms / fcnt / 1e3
+    byts = (2 * n * n / segments + acc * (n / segments) * 2 * n) * 8 * nrep_launch
+    return {"bound": "hbm", "kernel": f"k_metro_slice<double, {window}>",
             "achieved": round(byts / t / 1e9, 1), "peak": round(HBM / 1e9, 1), "unit": "GB/s",
             "frac": round(byts / t / HBM, 4),
-            "traffic": None if mixed else traffic_from_profile("k_metro_slice"),
+            "traffic": traffic_from_profile("k_metro_slice", segments),
             "avg_launch_ms": round(t * 1e3, 4), "launches_per_sweep": fcnt, "bytes_per_launch": round(byts)}
 
 
@@ -217,15 +222,25 @@ def cpu_baseline():
 
 
 def calibrate_copy():
-    """A plain device copy of known size (256 MiB read, 256 MiB written) for
-    scaling the PMC byte counters."""
-    import torch
-    a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
+    """A plain device copy of known size (256 MiB read, 256 MiB written,
+    hipMemcpy device to device) for scaling the PMC byte counters."""
+    import ctypes as C
+    h = C.CDLL("libamdhip64.so")
+    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    h.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    h.hipFree.argtypes = [C.c_void_p]
+    nbytes = 256 << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if h.hipMalloc(C.byref(a), nbytes) or h.hipMalloc(C.byref(b), nbytes) or h.hipMemset(a, 1, nbytes):
+        raise RuntimeError("calibration buffers")
     for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(f"# calibration copy: {a.numel() * 4} bytes read and written per dispatch", file=sys.stderr, flush=True)
+        if h.hipMemcpy(b, a, nbytes, 3):
+            raise RuntimeError("calibration copy")
+    h.hipDeviceSynchronize()
+    h.hipFree(a)
+    h.hipFree(b)
+    print(f"# calibration copy: {nbytes} bytes read and written per dispatch", file=sys.stderr, flush=True)
 
 
 def main():
@@ -281,6 +296,7 @@ def main():
     cpu = None if a.no_cpu else cpu_baseline()
     n = LX * LY
     acc = main_run["acc"]
+    tw, ts, tsep = main_run["tuning"]
     line = {
         "metric": METRIC,
         "value": round(main_run["value"], 2),
@@ -296,11 +312,13 @@ def main():
         "data": "synthetic: Ising fields drawn from seeds 1..128 by field_init's RNG stream",
         "config": {"workload": "16x16 periodic square, t=-1, U=4, mu=U/2, dtau=0.1, beta=8 (80 slices), stab=8, "
                                "128 replicas, Szz/Sperp at all 256 q after every sweep; delayed updates in "
-                               f"16-site windows, the pending pairs of each half-slice (about a*n/2 = "
-                               f"{round(acc * n / 2)} at acceptance a) applied as one rank-P update",
-                   "replicas": NREP, "precision": "fp64"},
+                               f"{tw}-site windows, the pending pairs of each of the {ts} site segments of a slice "
+                               f"(about a*n/{ts} = {round(acc * n / ts)} at acceptance a) applied as one rank-P "
+                               "update (window and segments chosen by the autotuner), column pre-pivoted UDV "
+                               f"chain, {'separable' if tsep else 'dense'} B-matrix products",
+                   "replicas": NREP, "precision": "fp64", "metro_window": tw, "metro_segments": ts},
         "acceptance": round(acc, 4),
-        "roofline": roofline_metro_kernel(main_run["profile"], acc, False, NREP // a.groups),
+        "roofline": roofline_metro_kernel(main_run["profile"], acc, ts, tw, NREP // a.groups),
         "roofline_qr": roofline_qr(main_run["profile"], NREP // a.groups),
         "roofline_gemm": roofline_gemm(main_run["profile"], False, NREP // a.groups),
         "roofline_metro": roofline_metro(main_run["profile"], acc, NREP // a.groups),

@@ -181,6 +181,15 @@ int dqmc_gpu_read_drift(dqmc_gpu_ctx *ctx, double *max_inf, double *mean_inf,
 int dqmc_gpu_read_drift_at(dqmc_gpu_ctx *ctx, int *max_tau,
                            unsigned long long *max_sweep);
 
+/* Launch shape of a model: Metropolis delay window (sites), site segments
+   per slice (each ending in a rank-P flush) and whether the B-matrix
+   products run as separable 16 x 16 transforms (1) or dense GEMMs (0): the
+   engine's defaults, or the autotuner's choice after dqmc_gpu_init with
+   opts.autotune (no counterpart in the reference, whose delay capacity is
+   the fixed GREEN_DELAY_CAPACITY, src/green.c:10). */
+int dqmc_gpu_read_tuning(dqmc_gpu_ctx *ctx, int model_id, int *window,
+                         int *segments, int *separable);
+
 /* Parity and debug hooks. */
 int dqmc_gpu_read_green(dqmc_gpu_ctx *ctx, int replica, double *g);
 int dqmc_gpu_write_green(dqmc_gpu_ctx *ctx, int replica, const double *g);
@@ -206,8 +215,9 @@ int dqmc_gpu_last_elapsed_ms(dqmc_gpu_ctx *ctx, double *ms);
 #define DQMC_GPU_PROF_QR 8      /* every QR (also counted in its stage) */
 #define DQMC_GPU_PROF_TRSM 9    /* every triangular solve (also in INV) */
 #define DQMC_GPU_PROF_GEMM 10   /* every n x n x n GEMM launch (also in its stage) */
-#define DQMC_GPU_PROF_FLUSH 11  /* every delayed-update flush GEMM (also in METRO) */
-#define DQMC_GPU_PROF_N 12
+#define DQMC_GPU_PROF_FLUSH 11  /* every delayed-update flush launch (also in METRO) */
+#define DQMC_GPU_PROF_PROPOSE 12 /* every Metropolis proposal launch (also in METRO) */
+#define DQMC_GPU_PROF_N 13
 
 /* Profiling aid without a counterpart in the reference (which uses its
    PROF_BEGIN/PROF_END region timers, src/profiler.h:93-94): runs nsweep

@@ -600,45 +600,57 @@ __global__ void k_sweep_tick(unsigned long long *sweeps)
     if (threadIdx.x == 0) *sweeps += 1;
 }
 
-// Column pre-pivoting of a UDV update's QR input M (one workgroup per
-// replica): perm = the columns of M in order of decreasing 2-norm (ties by
-// index), so that the QR of M P = Q R factors the graded columns largest
-// first.  Norms are compared as log(max|x|) + log ||x / max|x| || so that
-// entries near the double range neither overflow nor underflow; each wave
-// takes a column at a time (lanes over rows, coalesced), then a bitonic
+// Column pre-pivoting of a UDV update's QR input M: perm = the columns of
+// M in order of decreasing 2-norm (ties by index), so that the QR of
+// M P = Q R factors the graded columns largest first.  Norms are compared
+// as log(max|x|) + log ||x / max|x|||, so entries near the double range
+// neither overflow nor underflow.  k_col_keys: one wave per column (lanes
+// over rows, the column held in registers for both passes), many
+// workgroups per replica; k_col_sort: one workgroup per replica, a bitonic
 // sort of the (key, index) pairs in LDS.
 #define CP_MAX 512
-__global__ __launch_bounds__(256) void k_col_order(const double *M, long long sM, int n, int *perm, int batch,
-                                                   const int *status)
+__global__ __launch_bounds__(256) void k_col_keys(const double *M, long long sM, int n, double *keys, int batch,
+                                                  const int *status)
+{
+    const int b = blockIdx.y;
+    if (b >= batch || status[b]) return;
+    const int lane = threadIdx.x & 63;
+    const int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (j >= n) return; // wave-uniform
+    const double *col = M + b * sM + (long long)j * n;
+    double v[CP_MAX / 64];
+#pragma unroll
+    for (int k = 0; k < CP_MAX / 64; k++) {
+        const int i = lane + 64 * k;
+        v[k] = i < n ? col[i] : 0.0;
+    }
+    double mx = 0.0;
+#pragma unroll
+    for (int k = 0; k < CP_MAX / 64; k++) mx = fmax(mx, fabs(v[k]));
+    mx = wave_max(mx);
+    double ss = 0.0;
+    if (mx > 0.0) {
+        const double r = 1.0 / mx;
+#pragma unroll
+        for (int k = 0; k < CP_MAX / 64; k++) ss += (v[k] * r) * (v[k] * r);
+    }
+    ss = wave_sum(ss);
+    if (lane == 0) {
+        const double key = mx > 0.0 ? log(mx) + 0.5 * log(ss) : -INFINITY;
+        keys[(long long)b * n + j] = isnan(key) ? -INFINITY : key;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_col_sort(const double *keys, int n, int *perm, int batch, const int *status)
 {
     const int b = blockIdx.x;
     if (b >= batch || status[b]) return;
     __shared__ double key[CP_MAX];
     __shared__ int idx[CP_MAX];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const double *Mb = M + b * sM;
-    for (int j = wave; j < n; j += nw) {
-        const double *col = Mb + (long long)j * n;
-        double mx = 0.0;
-        for (int i = lane; i < n; i += 64) mx = fmax(mx, fabs(col[i]));
-        mx = wave_max(mx);
-        double ss = 0.0;
-        if (mx > 0.0)
-            for (int i = lane; i < n; i += 64) {
-                const double t = col[i] / mx;
-                ss += t * t;
-            }
-        ss = wave_sum(ss);
-        if (lane == 0) {
-            const double k = mx > 0.0 ? log(mx) + 0.5 * log(ss) : -INFINITY;
-            key[j] = isnan(k) ? -INFINITY : k;
-            idx[j] = j;
-        }
-    }
     int n2 = 1;
     while (n2 < n) n2 <<= 1;
-    for (int j = n + (int)threadIdx.x; j < n2; j += blockDim.x) {
-        key[j] = -INFINITY;
+    for (int j = threadIdx.x; j < n2; j += blockDim.x) {
+        key[j] = j < n ? keys[(long long)b * n + j] : -INFINITY;
         idx[j] = j;
     }
     __syncthreads();
@@ -706,6 +718,7 @@ struct Work {
     double *V1 = nullptr, *V2 = nullptr, *V3 = nullptr, *V4 = nullptr, *V5 = nullptr, *Tws = nullptr;
     int *I1 = nullptr, *I2 = nullptr, *status = nullptr, *reason = nullptr, *det_sign = nullptr;
     int *perm = nullptr; // column order of a pre-pivoted QR, n per entry
+    bool pivot = false;  // UDV updates with column pre-pivoting (qr_update)
     float *F1 = nullptr, *F2 = nullptr;
     std::vector<void *> owned;
 
@@ -854,21 +867,12 @@ static int copy_nn(Work &w, const double *src, long long sS, double *dst, long l
     return launch_check("k_copy_d");
 }
 
-// Column pre-pivoting of the UDV updates (DQMC_GPU_PIVOT=0 turns it off):
+// Column pre-pivoting of the UDV updates (a model's DQMC_GPU_PIVOT=0 turns it off):
 // the QR input M of an update is factored as M P = Q R with P ordering its
 // columns by decreasing norm, so U' = Q, D' = diag R and T' = (D'^-1 R) P^T T.
 // The normalised R then has entries of order one and the accumulated T
 // stays bounded (at 16x16, U=4, beta=8 max|T| ~ 3e2 against 3e5 unpivoted),
 // which is what keeps the boundary rebuild's T_l T_r^T product accurate.
-static bool pivot_on()
-{
-    static int v = -1;
-    if (v < 0) {
-        const char *e = getenv("DQMC_GPU_PIVOT");
-        v = (e && e[0] == '0') ? 0 : 1;
-    }
-    return v == 1;
-}
 
 // QR of an update's input M (n x n per entry, left unchanged when pivoted)
 // into U' (Q), Rn (W3), D' and sgn det Q; with pivoting the columns are
@@ -878,9 +882,13 @@ static int qr_update(Work &w, double *M, const Udv &out)
 {
     const long long nn = (long long)w.n * w.n;
     double *qin = M;
-    if (pivot_on()) {
-        hipLaunchKernelGGL(k_col_order, dim3(w.nb), dim3(256), 0, w.s, M, nn, w.n, w.perm, w.nb, w.status);
-        if (launch_check("k_col_order")) return 1;
+    if (w.pivot) {
+        // the keys go to V5 (n per entry), free during an update
+        hipLaunchKernelGGL(k_col_keys, dim3((w.n + 3) / 4, w.nb), dim3(256), 0, w.s, M, nn, w.n, w.V5, w.nb,
+                           w.status);
+        if (launch_check("k_col_keys")) return 1;
+        hipLaunchKernelGGL(k_col_sort, dim3(w.nb), dim3(256), 0, w.s, w.V5, w.n, w.perm, w.nb, w.status);
+        if (launch_check("k_col_sort")) return 1;
         hipLaunchKernelGGL(k_gather, grid2(nn, w.nb), dim3(256), 0, w.s, M, nn, w.W4, nn, w.perm, w.n, 0, w.nb,
                            w.status);
         if (launch_check("k_gather")) return 1;
@@ -894,7 +902,7 @@ static int update_T(Work &w, const Udv &in, const Udv &out)
 {
     const int n = w.n;
     const long long nn = (long long)n * n;
-    if (pivot_on()) {
+    if (w.pivot) {
         hipLaunchKernelGGL(k_gather, grid2(nn, w.nb), dim3(256), 0, w.s, in.T, in.sU, w.W2, nn, w.perm, n, 1, w.nb,
                            w.status);
         if (launch_check("k_gather")) return 1;
@@ -1007,6 +1015,9 @@ static int udv_inv_two_sided(Work &w, const Udv &l, const Udv &r, double *Gout, 
 struct Model {
     int n = 0, L = 0, stab = 0, M = 0, W = 0, Wf = 0, nrep = 0, first = 0, cap = 0;
     int msplit = 1; // Metropolis launches (site segments, each with its flush) per slice
+    bool tuned_separable = false; // B-matrix products on the separable path after tuning
+    bool pivot = true; // column pre-pivoted UDV updates (DQMC_GPU_PIVOT=0: off)
+    bool fuse = true;  // last segment's flush inside the wrap (DQMC_GPU_FUSE=0: off)
     bool two_sided = true; // boundary rebuilds by udv_inv_two_sided (one QR) instead of combine + inv_one_plus
     int Lx = 1, Ly = 1, nq = 0, nbin = 0;
     double U = 0, dtau = 0, mu = 0, lambda = 0;
@@ -1173,10 +1184,12 @@ static int model_alloc(dqmc_gpu_ctx *ctx, Model &m)
     const int mixed = ctx->opts.precision != DQMC_GPU_FP64;
     HIPCHK(hipStreamCreateWithFlags(&m.w.s, hipStreamNonBlocking));
     if (m.w.alloc(n, R, mixed)) return 1;
+    m.w.pivot = m.pivot;
     const char *ov = getenv("DQMC_GPU_OVERLAP");
     if (!(ov && ov[0] == '0')) {
         HIPCHK(hipStreamCreateWithFlags(&m.w2.s, hipStreamNonBlocking));
         if (m.w2.alloc(n, R, 0)) return 1;
+        m.w2.pivot = m.pivot;
         // a replica that fails on either stream is skipped on both
         m.w2.status = m.w.status;
         m.w2.reason = m.w.reason;
@@ -1337,7 +1350,7 @@ template <typename T> static void launch_metro_w(const MetroArgs<T> &a, hipStrea
 // the slice's accepted pairs in C, V (workspace), then one batched GEMM
 // with per-replica K applies G -= C V^T (green_delay_flush,
 // src/green.c:462-475).
-static int stage_metro(dqmc_gpu_ctx *ctx, Model &m, int l)
+static int stage_metro(dqmc_gpu_ctx *ctx, Model &m, int l, bool defer = false)
 {
     ProfScope ps_(DQMC_GPU_PROF_METRO, m.w.s);
     const int n = m.n;
@@ -1361,7 +1374,10 @@ static int stage_metro(dqmc_gpu_ctx *ctx, Model &m, int l)
             a.nu[0] = m.nu[0]; a.nu[1] = m.nu[1]; a.nd[0] = m.nd[0]; a.nd[1] = m.nd[1];
             a.acc = m.acc; a.att = m.att; a.sign = m.sign; a.status = m.w.status;
             a.Cp = m.w.F1; a.Vp = m.w.F2; a.sP = nn; a.pcount = m.pcount;
-            launch_metro_w<float>(a, m.w.s);
+            {
+                ProfScope psp_(DQMC_GPU_PROF_PROPOSE, m.w.s);
+                launch_metro_w<float>(a, m.w.s);
+            }
             if (launch_check("k_metro_slice")) return 1;
             GemmArgs<float> g =
                 gemm_args<float>(n, n, n, m.nrep, m.w.F1, n, nn, 0, m.w.F2, n, nn, 1, m.Gf, n, nn);
@@ -1381,8 +1397,13 @@ static int stage_metro(dqmc_gpu_ctx *ctx, Model &m, int l)
             a.nu[0] = m.nu[0]; a.nu[1] = m.nu[1]; a.nd[0] = m.nd[0]; a.nd[1] = m.nd[1];
             a.acc = m.acc; a.att = m.att; a.sign = m.sign; a.status = m.w.status;
             a.Cp = m.w.W3; a.Vp = m.w.W4; a.sP = nn; a.pcount = m.pcount;
-            launch_metro_w<double>(a, m.w.s);
+            {
+                ProfScope psp_(DQMC_GPU_PROF_PROPOSE, m.w.s);
+                launch_metro_w<double>(a, m.w.s);
+            }
             if (launch_check("k_metro_slice")) return 1;
+            // defer: the last segment's pairs stay pending for the wrap
+            if (defer && s1 == n) continue;
             if (n % 64 == 0) {
                 // rank-P update on its own kernel (same sums as the GEMM)
                 ProfScope psf_(DQMC_GPU_PROF_FLUSH, m.w.s);
@@ -1409,9 +1430,15 @@ static int stage_metro(dqmc_gpu_ctx *ctx, Model &m, int l)
 // 0 expK from the left, 1 expK from the right, 2 expKinv from the left,
 // 3 expKinv from the right
 static int kron(Model &m, Work &w, int rows, int f, const double *in, double *out, Vec<double> pre,
-                Vec<double> post, Vec<double> oth)
+                Vec<double> post, Vec<double> oth, bool pending = false)
 {
     KronArgs a;
+    // the last Metropolis segment's rank-P update, left pending in W3 / W4
+    // (stage_metro with defer), applied by this column pass
+    a.fc = pending ? m.w.W3 : nullptr;
+    a.fv = pending ? m.w.W4 : nullptr;
+    a.sP = (long long)m.n * m.n;
+    a.pcount = m.pcount;
     a.batch = m.nrep;
     a.rows = rows;
     a.in = in;
@@ -1428,8 +1455,16 @@ static int kron(Model &m, Work &w, int rows, int f, const double *in, double *ou
     return launch_check("k_kron16");
 }
 
-// G <- B_l G B_l^-1 = expK diag(d_l) G diag(1/d_l) expKinv (src/green.c:509-527)
-static int stage_wrap(dqmc_gpu_ctx *ctx, Model &m, int l)
+// whether the last Metropolis segment of a slice leaves its rank-P update to
+// the wrap that follows it (separable FP64 products; DQMC_GPU_FUSE=0 off)
+static bool fuse_flush(dqmc_gpu_ctx *ctx, const Model &m)
+{
+    return m.fuse && m.kronF && !is_mixed(ctx);
+}
+
+// G <- B_l G B_l^-1 = expK diag(d_l) G diag(1/d_l) expKinv (src/green.c:509-527);
+// pending: first apply the rank-P update the Metropolis stage left pending
+static int stage_wrap(dqmc_gpu_ctx *ctx, Model &m, int l, bool pending = false)
 {
     ProfScope ps_(DQMC_GPU_PROF_WRAP, m.w.s);
     const int n = m.n, R = m.nrep;
@@ -1445,7 +1480,7 @@ static int stage_wrap(dqmc_gpu_ctx *ctx, Model &m, int l)
     if (m.kronF) {
         // W1 = expK diag(d) G diag(1/d) column by column, then G = W1 expKinv row by row
         if (kron(m, m.w, 0, 0, m.G, m.w.W1, vec<double>(m.dvec + (long long)l * n, sV), vnone<double>(),
-                 vec<double>(m.dinv + (long long)l * n, sV)))
+                 vec<double>(m.dinv + (long long)l * n, sV), pending))
             return 1;
         return kron(m, m.w, 1, 3, m.w.W1, m.G, vnone<double>(), vnone<double>(), vnone<double>());
     }
@@ -1728,7 +1763,7 @@ static int stage_from_scratch(dqmc_gpu_ctx *ctx, Model &m, int l0)
 
 // G <- B_l^-1 G B_l = diag(1/d_l) expKinv G expK diag(d_l)
 // (green_wrap_backward, src/green.c:529-546)
-static int stage_wrap_back(dqmc_gpu_ctx *ctx, Model &m, int l)
+static int stage_wrap_back(dqmc_gpu_ctx *ctx, Model &m, int l, bool pending = false)
 {
     ProfScope ps_(DQMC_GPU_PROF_WRAP, m.w.s);
     const int n = m.n, R = m.nrep;
@@ -1744,7 +1779,7 @@ static int stage_wrap_back(dqmc_gpu_ctx *ctx, Model &m, int l)
     if (m.kronF) {
         // W1 = diag(1/d) expKinv G column by column, then G = W1 expK diag(d) row by row
         if (kron(m, m.w, 0, 2, m.G, m.w.W1, vnone<double>(), vec<double>(m.dinv + (long long)l * n, sV),
-                 vnone<double>()))
+                 vnone<double>(), pending))
             return 1;
         return kron(m, m.w, 1, 1, m.w.W1, m.G, vnone<double>(), vec<double>(m.dvec + (long long)l * n, sV),
                     vnone<double>());
@@ -1824,9 +1859,10 @@ static int sweep_forward(dqmc_gpu_ctx *ctx, Model &m, bool build_suffix, bool ca
     const Udv lf = m.left();
     if (!carry && udv_identity(m, lf)) return 1;
     int tau_left = 0;
+    const bool fuse = fuse_flush(ctx, m);
     for (int l = 0; l < m.L; l++) {
-        if (stage_metro(ctx, m, l)) return 1;
-        if (stage_wrap(ctx, m, l)) return 1;
+        if (stage_metro(ctx, m, l, fuse)) return 1;
+        if (stage_wrap(ctx, m, l, fuse)) return 1;
         if (((l + 1) % m.stab) == 0 && (l + 1) < m.L) {
             const int tau = l + 1, j = tau / m.stab;
             const Udv cur = carry ? m.alt(j) : lf;
@@ -1884,15 +1920,19 @@ static int sweep_backward(dqmc_gpu_ctx *ctx, Model &m, bool build_prefix)
         }
     }
     if (stage_wrap_back(ctx, m, m.L - 1)) return 1;
+    const bool fuse = fuse_flush(ctx, m);
     for (int l = m.L - 1; l >= 0; l--) {
-        if (stage_metro(ctx, m, l)) return 1;
+        // the flush rides on the next backward wrap unless a rebuild (which
+        // reads the wrapped G for its drift) or the sweep's end comes first
+        const bool f = fuse && l > 0 && l % m.stab != 0;
+        if (stage_metro(ctx, m, l, f)) return 1;
         if (l == 0) break;
         if (l % m.stab == 0) {
             const int j = l / m.stab;
             if (stage_suffix_block(ctx, m, j)) return 1;
             if (stage_boundary_rebuild(ctx, m, m.alt(j), m.stack(j), l)) return 1;
         }
-        if (stage_wrap_back(ctx, m, l - 1)) return 1;
+        if (stage_wrap_back(ctx, m, l - 1, f)) return 1;
     }
     if (stage_suffix_block(ctx, m, 0)) return 1;
     // G(0) = (1 + S[0])^-1 with S[0] stored transposed: the two-sided form
@@ -1998,6 +2038,139 @@ static int set_lds_limits()
         HIPCHK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
     }
     done_dev[dev] = true;
+    return 0;
+}
+
+// ------------------------------------------------------------- autotuner
+//
+// opts.autotune: after a model's initial rebuild, its Metropolis launch shape
+// (delay window W and the number of site segments per slice, each segment
+// ending in a rank-P flush) and, on a 16 x 16 lattice, the B-matrix product
+// path (separable 16 x 16 transforms or dense GEMMs) are timed on the
+// model's own replicas and state, and the fastest is kept.  Every candidate
+// starts from the same saved state (G, fields and their diagonals, RNG
+// streams, counters, sign), which is restored afterwards, so tuning leaves
+// the Markov chains exactly where the initial rebuild put them.  n_wrap
+// (stab) is not tuned: it is an input of the reference (the rebuild
+// interval), and the drift it allows is part of the result.
+
+struct TuneSave {
+    std::vector<void *> bufs;
+    struct Item {
+        void *dst, *src;
+        size_t bytes;
+    };
+    std::vector<Item> items;
+    ~TuneSave()
+    {
+        for (void *p : bufs) (void)hipFree(p);
+    }
+    int add(void *live, size_t bytes)
+    {
+        if (!live || !bytes) return 0;
+        void *p = nullptr;
+        HIPCHK(hipMalloc(&p, bytes));
+        bufs.push_back(p);
+        items.push_back({live, p, bytes});
+        HIPCHK(hipMemcpy(p, live, bytes, hipMemcpyDeviceToDevice));
+        return 0;
+    }
+    int restore(hipStream_t s)
+    {
+        for (const Item &it : items) HIPCHK(hipMemcpyAsync(it.dst, it.src, it.bytes, hipMemcpyDeviceToDevice, s));
+        return 0;
+    }
+};
+
+// milliseconds of `reps` slices of f() on the model's stream, from the saved state
+template <typename F> static int tune_time(Model &m, TuneSave &sv, int reps, F f, float *ms)
+{
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    int rc = sv.restore(m.w.s);
+    for (int l = 0; l < reps && !rc; l++) rc = f(l); // warm-up pass
+    if (!rc) rc = sv.restore(m.w.s);
+    if (!rc && hipEventRecord(a, m.w.s) != hipSuccess) rc = 1;
+    for (int l = 0; l < reps && !rc; l++) rc = f(l);
+    if (!rc && (hipEventRecord(b, m.w.s) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                hipEventElapsedTime(ms, a, b) != hipSuccess))
+        rc = 1;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc && g_err.empty()) set_err("autotune timing failed");
+    return rc;
+}
+
+static int tune_model(dqmc_gpu_ctx *ctx, Model &m, int model_id)
+{
+    const size_t nn = (size_t)m.n * m.n, R = (size_t)m.nrep, Ln = (size_t)m.L * m.n;
+    // the saved state must be the initial rebuild's, which runs on the
+    // model's (non-blocking) stream
+    HIPCHK(hipStreamSynchronize(m.w.s));
+    TuneSave sv;
+    if (sv.add(m.G, nn * R * 8) || sv.add(m.Gf, m.Gf ? nn * R * 4 : 0) || sv.add(m.field, Ln * R) ||
+        sv.add(m.dvec, Ln * R * 8) || sv.add(m.dinv, Ln * R * 8) || sv.add(m.dvecf, m.dvecf ? Ln * R * 4 : 0) ||
+        sv.add(m.dinvf, m.dinvf ? Ln * R * 4 : 0) || sv.add(m.rng, 32 * R) || sv.add(m.acc, 8 * R) ||
+        sv.add(m.att, 8 * R) || sv.add(m.sign, 8 * R))
+        return 1;
+    const int reps = std::min(m.L, 4);
+    const bool mixed = is_mixed(ctx);
+    const size_t elem = mixed ? sizeof(float) : sizeof(double);
+    const int Wmax = metro_window(m.n, mixed ? 32 : 16, elem);
+    int bestW = mixed ? m.Wf : m.W, bestS = m.msplit;
+    double *const kf = m.kronF;
+    bool bestK = kf != nullptr;
+    float best = 1e30f;
+    std::string tried;
+    // a slice as the sweep runs it: Metropolis (the last segment's flush
+    // left to the wrap where that applies), then the wrap
+    auto slice = [&](int l) {
+        const bool f = fuse_flush(ctx, m);
+        return stage_metro(ctx, m, l, f) || stage_wrap(ctx, m, l, f);
+    };
+    // product paths to try: separable (when expK splits) and dense; the
+    // mixed modes' wrap is a dense FP32 GEMM either way, so they keep theirs
+    std::vector<int> paths;
+    if (kf) paths.push_back(1);
+    if (!kf || !mixed) paths.push_back(0);
+    for (int useK : paths) {
+        m.kronF = useK ? kf : nullptr;
+        for (int W : {8, 16, 32}) {
+            if (W > Wmax) continue;
+            const int nwin = (m.n + W - 1) / W;
+            for (int sp = 1; sp <= 4 && sp <= nwin; sp++) {
+                (mixed ? m.Wf : m.W) = W;
+                m.msplit = sp;
+                float ms = 0.0f;
+                if (tune_time(m, sv, reps, slice, &ms)) {
+                    m.kronF = kf;
+                    return 1;
+                }
+                char buf[64];
+                snprintf(buf, sizeof buf, " %sW%d/S%d %.1f", kf ? (useK ? "sep:" : "dense:") : "", W, sp,
+                         1e3 * ms / reps);
+                tried += buf;
+                if (ms < best) {
+                    best = ms;
+                    bestW = W;
+                    bestS = sp;
+                    bestK = useK == 1;
+                }
+            }
+        }
+    }
+    (mixed ? m.Wf : m.W) = bestW;
+    m.msplit = bestS;
+    m.kronF = bestK ? kf : nullptr; // an unused factor buffer stays owned by the model
+    const char *prod = bestK ? "separable" : "dense";
+    m.tuned_separable = m.kronF != nullptr;
+    if (sv.restore(m.w.s)) return 1;
+    HIPCHK(hipStreamSynchronize(m.w.s));
+    fprintf(stderr,
+            "dqmc_gpu autotune: model %d (n=%d, L=%d, %d replicas): window %d, %d site segment(s) per slice, %s "
+            "B-matrix products; us per slice (Metropolis and wrap):%s\n",
+            model_id, m.n, m.L, m.nrep, bestW, bestS, prod, tried.c_str());
     return 0;
 }
 
@@ -2136,6 +2309,8 @@ int dqmc_gpu_add_model(dqmc_gpu_ctx *ctx, int n, int L, int stab, int delay_k, c
     m->msplit = n >= 128 ? 2 : 1;
     if (const char *e = getenv("DQMC_GPU_METRO_SPLIT")) m->msplit = std::max(1, std::min(8, atoi(e)));
     if (const char *e = getenv("DQMC_GPU_REBUILD")) m->two_sided = strcmp(e, "combine") != 0;
+    if (const char *e = getenv("DQMC_GPU_PIVOT")) m->pivot = e[0] != '0';
+    if (const char *e = getenv("DQMC_GPU_FUSE")) m->fuse = e[0] != '0';
     m->Wf = metro_window(n, delay_k > 0 ? delay_k : 32, sizeof(float));
     // expv for sigma = +1 (src/green.c:63-67), N = exp(-2 lambda sigma s) - 1 (src/field.c:10-17)
     const double c = -dtau * U / 2.0;
@@ -2298,6 +2473,11 @@ int dqmc_gpu_init(dqmc_gpu_ctx *ctx)
         if (stage_from_scratch(ctx, *m, 0)) return 1; // also forms every block product
         m->bb_valid = m->BB != nullptr;
     }
+    if (ctx->opts.autotune)
+        for (size_t k = 0; k < ctx->models.size(); k++) {
+            Model *m = ctx->models[k];
+            if (m->nrep > 0 && tune_model(ctx, *m, (int)k)) return 1;
+        }
     if (sync_all(ctx)) return 1;
     ctx->initialised = true;
     return 0;
@@ -2447,6 +2627,19 @@ int dqmc_gpu_read_drift_at(dqmc_gpu_ctx *ctx, int *max_tau, unsigned long long *
             if (max_sweep) max_sweep[f + r] = ss[r];
         }
     }
+    return 0;
+}
+
+int dqmc_gpu_read_tuning(dqmc_gpu_ctx *ctx, int model_id, int *window, int *segments, int *separable)
+{
+    if (!ctx || model_id < 0 || model_id >= (int)ctx->models.size()) {
+        set_err("invalid read_tuning arguments");
+        return 1;
+    }
+    const Model &m = *ctx->models[model_id];
+    if (window) *window = is_mixed(ctx) ? m.Wf : m.W;
+    if (segments) *segments = m.msplit;
+    if (separable) *separable = m.kronF != nullptr ? 1 : 0;
     return 0;
 }
 

@@ -22,6 +22,10 @@
 //                      of lane (lr, lc) at row lr + 4v, which is the B
 //                      operand of k-step v with k = 4v + lr),
 // and the results go back through LDS to coalesced stores (post-scaled).
+// A column pass can first apply a pending rank-P delayed update to its 16
+// columns (the last Metropolis segment's flush, green_delay_flush,
+// src/green.c:462-475, when the wrap follows it directly), so that G is
+// read and written once for both.
 // Scales: `pre` / `post` along the transformed index before / after the
 // transform, `oth` along the other index (it commutes with the transform).
 #pragma once
@@ -40,6 +44,12 @@ struct KronArgs {
     const double *ax, *ay; // 16 x 16 factors, row-major: grid -> ax grid ay^T
     Vec<double> pre, post, oth;
     const int *status;
+    // optional pending delayed update applied to the columns before the
+    // transform (rows = 0 only): in <- in - C^T V with C, V q-major
+    // (P x n per replica, stride sP), P = pcount[b] (fc == null: none)
+    const double *fc, *fv;
+    long long sP;
+    const int *pcount;
 };
 
 __device__ __forceinline__ int kron_lds(int e) { return (e & 15) + 17 * (e >> 4); }
@@ -62,6 +72,43 @@ __global__ __launch_bounds__(256) void k_kron16(KronArgs a)
         // thread = entry e = tid of 16 columns: 64 consecutive doubles per wave
 #pragma unroll
         for (int t = 0; t < KRON_L; t++) x[t] = in[tid + (long long)(v0 + t) * KRON_N];
+        // the pending rank-P update of these 16 columns, D[e][t] =
+        // sum_q C[q][e] V[q][v0 + t], on FP64 MFMA: wave w takes rows
+        // 64w .. 64w + 63 (four 16-row tiles), q four at a time in
+        // increasing order; D goes to Z in the staging layout
+        const int P = a.fc ? a.pcount[b] : 0;
+        if (P > 0) {
+            const double *C = a.fc + (long long)b * a.sP, *V = a.fv + (long long)b * a.sP;
+            const int e0 = 64 * wave;
+            f64x4 acc[4];
+#pragma unroll
+            for (int ti = 0; ti < 4; ti++) acc[ti] = f64x4{0, 0, 0, 0};
+            for (int q0 = 0; q0 < P; q0 += 16) {
+                double av[4][4], bv[4];
+                // clamped (always valid) rows, zeroed past P after the loads
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const long long qr = (long long)min(q0 + 4 * u + lr, P - 1) * KRON_N;
+                    bv[u] = V[qr + v0 + lc];
+#pragma unroll
+                    for (int ti = 0; ti < 4; ti++) av[u][ti] = C[qr + e0 + 16 * ti + lc];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const double b_ = q0 + 4 * u + lr < P ? bv[u] : 0.0;
+#pragma unroll
+                    for (int ti = 0; ti < 4; ti++) acc[ti] = Mfma<double>::mma(av[u][ti], b_, acc[ti]);
+                }
+            }
+#pragma unroll
+            for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    Z[lc * KRON_LD + kron_lds(e0 + 16 * ti + Mfma<double>::row(lane, v))] = acc[ti][v];
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < KRON_L; t++) x[t] -= Z[t * KRON_LD + kron_lds(tid)];
+        }
         const double pe = a.pre.at(b, tid);
 #pragma unroll
         for (int t = 0; t < KRON_L; t++) Z[t * KRON_LD + kron_lds(tid)] = x[t] * pe * a.oth.at(b, v0 + t);

@@ -51,6 +51,7 @@ SIGNATURES = {
     "dqmc_gpu_read_status": (_I, [_P, _P, _P, _P, _P, _P]),
     "dqmc_gpu_read_drift": (_I, [_P, _P, _P, _P]),
     "dqmc_gpu_read_drift_at": (_I, [_P, _P, _P]),
+    "dqmc_gpu_read_tuning": (_I, [_P, _I, _P, _P, _P]),
     "dqmc_gpu_read_green": (_I, [_P, _I, _P]),
     "dqmc_gpu_write_green": (_I, [_P, _I, _P]),
     "dqmc_gpu_read_field": (_I, [_P, _I, _P]),
@@ -315,7 +316,7 @@ class Engine:
         return v.value
 
     PROF_NAMES = ["metro", "wrap", "stack", "lmul", "combine", "inv_one_plus", "accept", "measure", "qr",
-                  "trsm", "gemm", "flush"]
+                  "trsm", "gemm", "flush", "propose"]
 
     def profile(self, nsweep=1):
         """Per-stage milliseconds and stage counts of nsweep sweeps
@@ -330,6 +331,13 @@ class Engine:
             if m.first <= r < m.first + m.nrep:
                 return m
         raise IndexError(r)
+
+    def tuning(self, model_id):
+        """(window, segments, separable) of a model (dqmc_gpu_read_tuning)."""
+        w, sg, sp = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().dqmc_gpu_read_tuning(self.h, model_id, C.byref(w), C.byref(sg), C.byref(sp)),
+               "dqmc_gpu_read_tuning")
+        return w.value, sg.value, sp.value
 
     def green(self, r):
         m = self._model_of(r)

@@ -319,3 +319,38 @@ def test_pivoted_chain_accuracy_16x16_beta8(eng_mod, monkeypatch, exact):
         e0 = np.abs(g0 - exact(kind, Lx, Ly, U, dtau, L, f0)).max() / np.abs(gx).max()
         print(f"replica {r}: pivoted G vs exact {e:.2e}, unpivoted {e0:.2e}", flush=True)
         assert e < 1e-10, (r, e)
+
+
+@pytest.mark.parametrize("order,nsw", [("forward", 3), ("alternating", 4)])
+def test_fused_flush_matches_separate_flush(eng_mod, monkeypatch, order, nsw):
+    # On the separable 16 x 16 path the last Metropolis segment of a slice
+    # leaves its rank-P update to the wrap's first pass, which applies it
+    # while it reads G (DQMC_GPU_FUSE=0: a separate flush launch).  Same
+    # fields and accept counts; G differs only by the rounding order of the
+    # update, within 1e-12 of max|G|.
+    kind, Lx, Ly, L, stab, k = "square", 16, 16, 24, 8, 16
+    seeds = [5, 6, 7, 8]
+    runs = []
+    for fu in ("1", "0"):
+        monkeypatch.setenv("DQMC_GPU_FUSE", fu)
+        print(f"fused flush check, {order}, DQMC_GPU_FUSE={fu}", flush=True)
+        eng = eng_mod.Engine(eng_mod.FP64)
+        mid = eng.add_model(kind, Lx, Ly, 4.0, 0.1, L, stab, delay_k=k)
+        eng.set_sweep_order(mid, order)
+        eng.add_replicas(mid, seeds)
+        eng.init()
+        traj = []
+        for _ in range(nsw):
+            eng.sweep(1)
+            st = eng.status()
+            assert (st["status"] == 0).all()
+            traj.append(([eng.field(r).copy() for r in range(len(seeds))], st["accepted"].copy(),
+                         [eng.green(r).copy() for r in range(len(seeds))]))
+        eng.close()
+        runs.append(traj)
+    for s, (a, b) in enumerate(zip(*runs)):
+        assert np.array_equal(a[1], b[1]), f"accept counts differ after sweep {s}"
+        for r in range(len(seeds)):
+            assert np.array_equal(a[0][r], b[0][r]), f"fields differ: sweep {s} replica {r}"
+            err = np.abs(a[2][r] - b[2][r]).max() / np.abs(b[2][r]).max()
+            assert err < 1e-12, (s, r, err)
