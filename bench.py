@@ -86,17 +86,18 @@ def reference_drift():
 
 
 def traffic_from_profile(kernel, segments=None):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter reads
-    for a device copy of known size); None when the summary was taken with
-    another number of Metropolis site segments than this run's."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_16x16_beta8_128rep_pmc.txt")
+    """HBM bytes per launch of `kernel` (its name's prefix) from the committed
+    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, each scaled by what the
+    counter reads for a device copy of known size); None when the summary
+    was taken with another number of Metropolis site segments than this
+    run's."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_fp64_16x16_beta8_128rep_pmc.txt")
     if not os.path.exists(path):
         return None
     val, seg = None, None
     for line in open(path):
-        if line.startswith("hbm_bytes_per_launch"):
-            val = float(line.split()[-1])
+        if line.startswith(kernel) and val is None:
+            val = float(line.split()[-1]) * 1e6
         if line.startswith("metro_segments"):
             seg = int(line.split()[-1])
     if segments is not None and seg != segments:
@@ -119,7 +120,7 @@ def roofline_metro_kernel(prof, acc, segments, window, nrep_launch=NREP):
     return {"bound": "hbm", "kernel": f"k_metro_slice<double, {window}>",
             "achieved": round(byts / t / 1e9, 1), "peak": round(HBM / 1e9, 1), "unit": "GB/s",
             "frac": round(byts / t / HBM, 4),
-            "traffic": traffic_from_profile("k_metro_slice", segments),
+            "traffic": traffic_from_profile(f"void k_metro_slice<double, {window}>", segments),
             "avg_launch_ms": round(t * 1e3, 4), "launches_per_sweep": fcnt, "bytes_per_launch": round(byts)}
 
 
@@ -134,7 +135,7 @@ def roofline_gemm(prof, mixed, nrep_launch=NREP):
     peak = P64
     return {"bound": "mfma", "kernel": "k_gemm_dl", "achieved": round(flops / t / 1e12, 3),
             "peak": round(peak / 1e12, 3), "unit": "TFLOP/s", "frac": round(flops / t / peak, 4),
-            "traffic": None if mixed else traffic_from_profile("k_gemm_dl"),
+            "traffic": None if mixed else traffic_from_profile("void k_gemm_dl<0>"),
             "avg_launch_ms": round(t * 1e3, 4), "launches_per_sweep": cnt, "flops_per_launch": flops}
 
 
@@ -253,6 +254,8 @@ def main():
     ap.add_argument("--groups", type=int, default=1, help="replica groups swept concurrently")
     ap.add_argument("--delay", type=int, default=DELAY, help="delayed-update window (sites)")
     ap.add_argument("--fp64-only", action="store_true", help="skip the mixed, drift and alternating runs")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the engine's default launch shape (DQMC_GPU_METRO_SPLIT sets the segments)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: replicas are independent Markov chains and this engine runs on a "
@@ -262,7 +265,8 @@ def main():
 
     if a.calibrate_copy:
         calibrate_copy()
-    main_run = run_engine(dq, dq.FP64, a.steps, a.warmup, groups=a.groups, delay=a.delay)
+    main_run = run_engine(dq, dq.FP64, a.steps, a.warmup, groups=a.groups, delay=a.delay,
+                          autotune=not a.no_autotune)
     print(f"# fp64: {main_run['value']:.1f} replica-sweeps/s, {main_run['ms'] / a.steps:.2f} ms/sweep",
           file=sys.stderr, flush=True)
     extra = {}
