@@ -181,12 +181,13 @@ int dqmc_gpu_read_drift(dqmc_gpu_ctx *ctx, double *max_inf, double *mean_inf,
 int dqmc_gpu_read_drift_at(dqmc_gpu_ctx *ctx, int *max_tau,
                            unsigned long long *max_sweep);
 
-/* Launch shape of a model: Metropolis delay window (sites), site segments
-   per slice (each ending in a rank-P flush) and whether the B-matrix
-   products run as separable 16 x 16 transforms (1) or dense GEMMs (0): the
-   engine's defaults, or the autotuner's choice after dqmc_gpu_init with
-   opts.autotune (no counterpart in the reference, whose delay capacity is
-   the fixed GREEN_DELAY_CAPACITY, src/green.c:10). */
+/* Launch shape of a model: Metropolis delay window (sites; 0 for the FP64
+   submatrix path, which decides each segment on its submatrix of G), site
+   segments per slice (each ending in a rank-P flush) and whether the
+   B-matrix products run as separable 16 x 16 transforms (1) or dense GEMMs
+   (0): the engine's defaults, or the autotuner's choice after
+   dqmc_gpu_init with opts.autotune (no counterpart in the reference, whose
+   delay capacity is the fixed GREEN_DELAY_CAPACITY, src/green.c:10). */
 int dqmc_gpu_read_tuning(dqmc_gpu_ctx *ctx, int model_id, int *window,
                          int *segments, int *separable);
 

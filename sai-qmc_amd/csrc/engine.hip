@@ -23,6 +23,7 @@
 #include "kron.hpp"
 #include "measure.hpp"
 #include "metro.hpp"
+#include "metro_sub.hpp"
 #include "qr.hpp"
 
 // ------------------------------------------------------------------ errors
@@ -1018,6 +1019,7 @@ struct Model {
     bool tuned_separable = false; // B-matrix products on the separable path after tuning
     bool pivot = true; // column pre-pivoted UDV updates (DQMC_GPU_PIVOT=0: off)
     bool fuse = true;  // last segment's flush inside the wrap (DQMC_GPU_FUSE=0: off)
+    int sub = 0;       // FP64 Metropolis segments on the submatrix path (0: windowed path)
     bool two_sided = true; // boundary rebuilds by udv_inv_two_sided (one QR) instead of combine + inv_one_plus
     int Lx = 1, Ly = 1, nq = 0, nbin = 0;
     double U = 0, dtau = 0, mu = 0, lambda = 0;
@@ -1350,9 +1352,80 @@ template <typename T> static void launch_metro_w(const MetroArgs<T> &a, hipStrea
 // the slice's accepted pairs in C, V (workspace), then one batched GEMM
 // with per-replica K applies G -= C V^T (green_delay_flush,
 // src/green.c:462-475).
+// the rank-P update G -= C^T V of the pending pairs in W3 / W4
+// (green_delay_flush, src/green.c:462-475)
+static int flush_pending(Model &m)
+{
+    const int n = m.n;
+    const long long nn = (long long)n * n;
+    if (n % 64 == 0) {
+        // rank-P update on its own kernel (same sums as the GEMM)
+        ProfScope psf_(DQMC_GPU_PROF_FLUSH, m.w.s);
+        FlushArgs f;
+        f.n = n; f.batch = m.nrep; f.G = m.G; f.sG = nn; f.C = m.w.W3; f.V = m.w.W4; f.sP = nn;
+        f.pcount = m.pcount; f.status = m.w.status;
+        hipLaunchKernelGGL(k_flush64, dim3((n / 64) * (n / 64) * m.nrep), dim3(256), 0, m.w.s, f);
+        return launch_check("k_flush64");
+    }
+    GemmArgs<double> g = gemm_args<double>(n, n, n, m.nrep, m.w.W3, n, nn, 0, m.w.W4, n, nn, 1, m.G, n, nn);
+    g.alpha = -1.0;
+    g.beta = 1.0;
+    g.Kb = m.pcount;
+    return gemm(m.w, g);
+}
+
+// FP64 Metropolis of one slice on the submatrix path: m.sub segments of at
+// most MSUB_MAX sites, each decided by k_metro_sub, its pending pairs
+// written by k_metro_pairs (K in W2, the accepted sites in perm, the pairs
+// in W3 / W4) and flushed; with defer the last segment's pairs stay pending
+// for the wrap.
+static int stage_metro_sub(Model &m, int l, bool defer)
+{
+    const int n = m.n;
+    const long long nn = (long long)n * n, sV = (long long)m.L * n;
+    const int per = (n + m.sub - 1) / m.sub;
+    if (per > MSUB_MAX) {
+        set_err("submatrix Metropolis segment of %d sites exceeds %d", per, MSUB_MAX);
+        return 1;
+    }
+    for (int s0 = 0; s0 < n; s0 += per) {
+        const int s1 = std::min(n, s0 + per), sm = s1 - s0;
+        MetroSubArgs a;
+        a.n = n; a.L = m.L; a.l = l; a.batch = m.nrep; a.s0 = s0; a.s1 = s1;
+        a.G = m.G; a.sG = nn; a.field = m.field; a.sF = sV;
+        a.dvec = m.dvec; a.dinv = m.dinv; a.sV = sV;
+        a.e[0] = m.expv[0]; a.e[1] = m.expv[1];
+        a.einv[0] = m.expvinv[0]; a.einv[1] = m.expvinv[1];
+        a.rng = m.rng;
+        a.nu[0] = m.nu[0]; a.nu[1] = m.nu[1]; a.nd[0] = m.nd[0]; a.nd[1] = m.nd[1];
+        a.acc = m.acc; a.att = m.att; a.sign = m.sign; a.status = m.w.status;
+        a.K = m.w.W2; a.sK = nn; a.aq = m.w.perm; a.sA = n; a.pcount = m.pcount;
+        {
+            ProfScope psp_(DQMC_GPU_PROF_PROPOSE, m.w.s);
+            if (sm <= 64)
+                hipLaunchKernelGGL((k_metro_sub<1, 8>), dim3(m.nrep), dim3(MSUB_THREADS), metro_sub_lds_bytes(sm),
+                                   m.w.s, a);
+            else
+                hipLaunchKernelGGL((k_metro_sub<2, 12>), dim3(m.nrep), dim3(MSUB_THREADS), metro_sub_lds_bytes(sm),
+                                   m.w.s, a);
+            if (launch_check("k_metro_sub")) return 1;
+            MetroPairsArgs pa;
+            pa.n = n; pa.batch = m.nrep; pa.s0 = s0; pa.m = sm;
+            pa.G = m.G; pa.sG = nn; pa.K = m.w.W2; pa.sK = nn; pa.aq = m.w.perm; pa.sA = n;
+            pa.pcount = m.pcount; pa.status = m.w.status; pa.C = m.w.W3; pa.V = m.w.W4; pa.sP = nn;
+            hipLaunchKernelGGL(k_metro_pairs, dim3(((n + MP_COLS - 1) / MP_COLS) * m.nrep), dim3(256), 0, m.w.s, pa);
+            if (launch_check("k_metro_pairs")) return 1;
+        }
+        if (defer && s1 == n) continue;
+        if (flush_pending(m)) return 1;
+    }
+    return 0;
+}
+
 static int stage_metro(dqmc_gpu_ctx *ctx, Model &m, int l, bool defer = false)
 {
     ProfScope ps_(DQMC_GPU_PROF_METRO, m.w.s);
+    if (!is_mixed(ctx) && m.sub > 0) return stage_metro_sub(m, l, defer);
     const int n = m.n;
     const long long nn = (long long)n * n, sV = (long long)m.L * n;
     const int W = is_mixed(ctx) ? m.Wf : m.W;
@@ -1404,22 +1477,7 @@ static int stage_metro(dqmc_gpu_ctx *ctx, Model &m, int l, bool defer = false)
             if (launch_check("k_metro_slice")) return 1;
             // defer: the last segment's pairs stay pending for the wrap
             if (defer && s1 == n) continue;
-            if (n % 64 == 0) {
-                // rank-P update on its own kernel (same sums as the GEMM)
-                ProfScope psf_(DQMC_GPU_PROF_FLUSH, m.w.s);
-                FlushArgs f;
-                f.n = n; f.batch = m.nrep; f.G = m.G; f.sG = nn; f.C = m.w.W3; f.V = m.w.W4; f.sP = nn;
-                f.pcount = m.pcount; f.status = m.w.status;
-                hipLaunchKernelGGL(k_flush64, dim3((n / 64) * (n / 64) * m.nrep), dim3(256), 0, m.w.s, f);
-                if (launch_check("k_flush64")) return 1;
-                continue;
-            }
-            GemmArgs<double> g =
-                gemm_args<double>(n, n, n, m.nrep, m.w.W3, n, nn, 0, m.w.W4, n, nn, 1, m.G, n, nn);
-            g.alpha = -1.0;
-            g.beta = 1.0;
-            g.Kb = m.pcount;
-            if (gemm(m.w, g)) return 1;
+            if (flush_pending(m)) return 1;
         }
     }
     return 0;
@@ -2019,7 +2077,8 @@ static int set_lds_limits()
     }
     if (done_dev[dev]) return 0;
     // 160 KiB of LDS per CU on gfx950, less each kernel's static LDS
-    const void *kernels[] = {(const void *)k_metro_slice<double, 16>,
+    const void *kernels[] = {(const void *)k_metro_sub<1, 8>, (const void *)k_metro_sub<2, 12>,
+                             (const void *)k_metro_slice<double, 16>,
                              (const void *)k_metro_slice<double, 8>,  (const void *)k_metro_slice<float, 32>,
                              (const void *)k_metro_slice<float, 16>,  (const void *)k_metro_slice<float, 8>,
                              (const void *)k_qr_udv<1>, (const void *)k_qr_udv<2>, (const void *)k_qr_udv<4>, (const void *)k_qr_udv<8>,
@@ -2119,6 +2178,10 @@ static int tune_model(dqmc_gpu_ctx *ctx, Model &m, int model_id)
     const size_t elem = mixed ? sizeof(float) : sizeof(double);
     const int Wmax = metro_window(m.n, mixed ? 32 : 16, elem);
     int bestW = mixed ? m.Wf : m.W, bestS = m.msplit;
+    // the candidates are windowed shapes: the submatrix path
+    // (DQMC_GPU_METRO_SUB) measured slower at every size, and tuning
+    // replaces it
+    m.sub = 0;
     double *const kf = m.kronF;
     bool bestK = kf != nullptr;
     float best = 1e30f;
@@ -2311,6 +2374,15 @@ int dqmc_gpu_add_model(dqmc_gpu_ctx *ctx, int n, int L, int stab, int delay_k, c
     if (const char *e = getenv("DQMC_GPU_REBUILD")) m->two_sided = strcmp(e, "combine") != 0;
     if (const char *e = getenv("DQMC_GPU_PIVOT")) m->pivot = e[0] != '0';
     if (const char *e = getenv("DQMC_GPU_FUSE")) m->fuse = e[0] != '0';
+    // FP64: DQMC_GPU_METRO_SUB=k runs the Metropolis on the submatrix path
+    // with k site segments (at least enough for segments of MSUB_MAX sites);
+    // the autotuner also times it
+    if (ctx->opts.precision == DQMC_GPU_FP64) {
+        if (const char *e = getenv("DQMC_GPU_METRO_SUB")) {
+            const int v = atoi(e);
+            m->sub = v <= 0 ? 0 : std::min(n, std::max((n + MSUB_MAX - 1) / MSUB_MAX, v));
+        }
+    }
     m->Wf = metro_window(n, delay_k > 0 ? delay_k : 32, sizeof(float));
     // expv for sigma = +1 (src/green.c:63-67), N = exp(-2 lambda sigma s) - 1 (src/field.c:10-17)
     const double c = -dtau * U / 2.0;
@@ -2637,8 +2709,9 @@ int dqmc_gpu_read_tuning(dqmc_gpu_ctx *ctx, int model_id, int *window, int *segm
         return 1;
     }
     const Model &m = *ctx->models[model_id];
-    if (window) *window = is_mixed(ctx) ? m.Wf : m.W;
-    if (segments) *segments = m.msplit;
+    const bool sub = !is_mixed(ctx) && m.sub > 0;
+    if (window) *window = sub ? 0 : is_mixed(ctx) ? m.Wf : m.W;
+    if (segments) *segments = sub ? m.sub : m.msplit;
     if (separable) *separable = m.kronF != nullptr ? 1 : 0;
     return 0;
 }

@@ -354,3 +354,46 @@ def test_fused_flush_matches_separate_flush(eng_mod, monkeypatch, order, nsw):
             assert np.array_equal(a[0][r], b[0][r]), f"fields differ: sweep {s} replica {r}"
             err = np.abs(a[2][r] - b[2][r]).max() / np.abs(b[2][r]).max()
             assert err < 1e-12, (s, r, err)
+
+
+@pytest.mark.parametrize("Lx,L,stab,sub,order", [(4, 20, 5, 1, "forward"), (8, 40, 10, 1, "alternating"),
+                                                  (12, 24, 8, 2, "forward"), (16, 24, 8, 3, "forward"),
+                                                  (16, 24, 8, 4, "alternating")])
+def test_submatrix_metropolis_matches_windowed(eng_mod, exact, monkeypatch, Lx, L, stab, sub, order):
+    # DQMC_GPU_METRO_SUB=k: each of k site segments decided on its submatrix
+    # of G, the change to G applied as one Woodbury update per segment.
+    # Against the windowed delayed updates (the default path): the same
+    # fields and accept counts after every sweep, and G of both at the exact
+    # G of the field.
+    seeds = [3, 4, 5, 6]
+    runs = []
+    for env in (str(sub), "0"):
+        monkeypatch.setenv("DQMC_GPU_METRO_SUB", env)
+        print(f"{Lx}x{Lx} L={L} {order}: DQMC_GPU_METRO_SUB={env}", flush=True)
+        eng = eng_mod.Engine(eng_mod.FP64)
+        mid = eng.add_model("square", Lx, Lx, 4.0, 0.1, L, stab, delay_k=16)
+        eng.set_sweep_order(mid, order)
+        eng.add_replicas(mid, seeds)
+        eng.init()
+        w, sg, _ = eng.tuning(mid)
+        assert (w == 0 and sg == sub) if env != "0" else w > 0
+        traj = []
+        for _ in range(3):
+            eng.sweep(1)
+            st = eng.status()
+            assert (st["status"] == 0).all() and (st["sign"] == 1.0).all()
+            traj.append(([eng.field(r).copy() for r in range(len(seeds))], st["accepted"].copy(),
+                         [eng.green(r).copy() for r in range(len(seeds))]))
+        eng.close()
+        runs.append(traj)
+    for s, (a, b) in enumerate(zip(*runs)):
+        assert np.array_equal(a[1], b[1]), f"accept counts differ after sweep {s}"
+        for r in range(len(seeds)):
+            assert np.array_equal(a[0][r], b[0][r]), f"fields differ: sweep {s} replica {r}"
+    for r in range(len(seeds)):
+        f, g = runs[0][-1][0][r], runs[0][-1][2][r]
+        gx = exact("square", Lx, Lx, 4.0, 0.1, L, f)
+        e = np.abs(g - gx).max() / np.abs(gx).max()
+        e0 = np.abs(runs[1][-1][2][r] - gx).max() / np.abs(gx).max()
+        print(f"replica {r}: submatrix G vs exact {e:.2e}, windowed {e0:.2e}", flush=True)
+        assert e < 1e-10, (r, e)
