@@ -191,6 +191,14 @@ int dqmc_gpu_read_drift_at(dqmc_gpu_ctx *ctx, int *max_tau,
 int dqmc_gpu_read_tuning(dqmc_gpu_ctx *ctx, int model_id, int *window,
                          int *segments, int *separable);
 
+/* Ratio R_up R_dn = (1 + (1 - G_ii) N_up)(1 + G_ii N_dn) of every site at
+   the latest Metropolis slice of a replica (n values in site order), as
+   the proposal loop of src/dqmc.c:656-679 evaluates it
+   (green_delay_ratio, src/green.c:401-416; the reference keeps no log of
+   it).  A parity hook: the model must have been added with
+   DQMC_GPU_RATIO_LOG=1 in the environment, otherwise it returns an error. */
+int dqmc_gpu_read_ratios(dqmc_gpu_ctx *ctx, int replica, double *ratios);
+
 /* Parity and debug hooks. */
 int dqmc_gpu_read_green(dqmc_gpu_ctx *ctx, int replica, double *g);
 int dqmc_gpu_write_green(dqmc_gpu_ctx *ctx, int replica, const double *g);

@@ -908,6 +908,7 @@ struct orc_replica {
     int alternating, dir, prefix_ok, suffix_ok;
     int rebuild;        /* 0 combine, 1 two_sided, 2 centered (src/green.h:10-14) */
     int reason;         /* LINALG_FAILURE_* of a failed replica */
+    double *rlog;       /* ratio R_up R_dn of every proposal, [l * n + i] (NULL: off) */
 };
 
 static void udv_alloc(Udv *u, int n)
@@ -1118,6 +1119,7 @@ static void metro_slice(orc_replica *R, int l)
         const double Ru = 1.0 + (1.0 - gii) * Nu;
         const double Rd = 1.0 + gii * Nd;
         const double Rt = Ru * Rd;
+        if (R->rlog) R->rlog[(size_t)l * n + i] = Rt;
         R->att++;
         if (orc_rng_double(R->rng) < fabs(Rt)) {
             R->acc++;
@@ -1371,9 +1373,23 @@ void orc_replica_drift(const orc_replica *R, double *max_inf,
     *mean_inf = R->drift_samples ? R->drift_sum / (double)R->drift_samples : 0.0;
 }
 
+/* Record the ratio of every proposal from now on (the last sweep's value
+   per slice and site); out receives L * n values, slice-major. */
+int orc_replica_log_ratios(orc_replica *R)
+{
+    if (!R->rlog) R->rlog = calloc((size_t)R->L * R->n, sizeof(double));
+    return R->rlog ? 0 : 1;
+}
+
+void orc_replica_ratios(const orc_replica *R, double *out)
+{
+    if (R->rlog) memcpy(out, R->rlog, sizeof(double) * (size_t)R->L * R->n);
+}
+
 void orc_replica_free(orc_replica *R)
 {
     if (!R) return;
+    free(R->rlog);
     free(R->expK); free(R->expKinv); free(R->g); free(R->gd); free(R->C); free(R->V);
     free(R->bipart); free(R->b); free(R->s);
     for (int j = 0; j <= R->M; j++) udv_release(&R->S[j]);

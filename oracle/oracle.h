@@ -131,6 +131,11 @@ void orc_replica_drift(const orc_replica *R, double *max_inf,
                        double *mean_inf, unsigned long long *samples);
 /* Re-run the initial rebuild from the current field (src/dqmc.c:466). */
 int orc_replica_rebuild(orc_replica *R);
+/* Ratio log: after orc_replica_log_ratios, every proposal's R_up R_dn
+   (src/green.c:401-416) is kept per slice and site; orc_replica_ratios
+   copies the L * n values of the latest visit of each slice. */
+int orc_replica_log_ratios(orc_replica *R);
+void orc_replica_ratios(const orc_replica *R, double *out);
 void orc_replica_free(orc_replica *R);
 
 #ifdef __cplusplus

@@ -78,6 +78,8 @@ def orc():
             "orc_replica_model": (None, [_P, _P, _P, _P]),
             "orc_replica_drift": (None, [_P, _P, _P, _P]),
             "orc_replica_rebuild": (_I, [_P]),
+            "orc_replica_log_ratios": (_I, [_P]),
+            "orc_replica_ratios": (None, [_P, _P]),
             "orc_replica_free": (None, [_P]),
             "orc_green_exact": (_I, [_I, _I, _I, _I, _P, _P, _P, _P]),
         }
@@ -111,7 +113,7 @@ class Replica:
     forward/backward sweeps with carried stacks)."""
 
     def __init__(self, kind, Lx, Ly, U, dtau, L, stab, seed, delay_cap=16, drift=False, t=-1.0,
-                 alternating=False, rebuild="combine"):
+                 alternating=False, rebuild="combine", ratio_log=False):
         th, bp, _ = lattice(kind, Lx, Ly, t)
         self.n = th.shape[0]
         self.L = L
@@ -123,6 +125,8 @@ class Replica:
             orc().orc_replica_set_rebuild(self.h, REBUILD[rebuild])
         if alternating:
             orc().orc_replica_set_alternating(self.h, 1)
+        if ratio_log and orc().orc_replica_log_ratios(self.h):
+            raise MemoryError("oracle ratio log")
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -165,6 +169,13 @@ class Replica:
         lam = C.c_double()
         orc().orc_replica_model(self.h, _ptr(eK), _ptr(eKi), C.byref(lam))
         return eK.reshape(self.n, self.n, order="F"), eKi.reshape(self.n, self.n, order="F"), lam.value
+
+    def ratios(self):
+        """R_up R_dn of the latest proposal at each (slice, site), L x n
+        (needs ratio_log=True)."""
+        r = np.zeros(self.L * self.n)
+        orc().orc_replica_ratios(self.h, _ptr(r))
+        return r.reshape(self.L, self.n)
 
     def drift(self):
         mx, mn, c = C.c_double(), C.c_double(), C.c_ulonglong()

@@ -1020,6 +1020,7 @@ struct Model {
     bool pivot = true; // column pre-pivoted UDV updates (DQMC_GPU_PIVOT=0: off)
     bool fuse = true;  // last segment's flush inside the wrap (DQMC_GPU_FUSE=0: off)
     int sub = 0;       // FP64 Metropolis segments on the submatrix path (0: windowed path)
+    bool flush64 = true; // rank-P flush on k_flush64 where n % 64 == 0 (DQMC_GPU_FLUSH64=0: the GEMM)
     bool two_sided = true; // boundary rebuilds by udv_inv_two_sided (one QR) instead of combine + inv_one_plus
     int Lx = 1, Ly = 1, nq = 0, nbin = 0;
     double U = 0, dtau = 0, mu = 0, lambda = 0;
@@ -1059,6 +1060,8 @@ struct Model {
     int *Xq = nullptr;
     unsigned long long *acc = nullptr, *att = nullptr, *acc_prev = nullptr, *att_prev = nullptr;
     int *pcount = nullptr; // pending delayed-update pairs per replica after a slice
+    bool rlog_on = false;  // DQMC_GPU_RATIO_LOG=1: keep each proposal's ratio
+    double *rlog = nullptr; // ratio R_up R_dn of every site of the latest slice, n per replica
     unsigned long long *dtmax = nullptr; // drift gather slots (k_drift_copy -> k_drift_finish)
     int *dtbad = nullptr;
     // block products Bblock_j = B_{(j+1)s-1} ... B_{js} (green_build_Bblock,
@@ -1240,6 +1243,11 @@ static int model_alloc(dqmc_gpu_ctx *ctx, Model &m)
     }
     if (dalloc(&m.pcount, R) || dalloc(&m.dtmax, R) || dalloc(&m.dtbad, R)) return 1;
     own(m.pcount);
+    if (m.rlog_on) {
+        if (dalloc(&m.rlog, (size_t)n * R)) return 1;
+        own(m.rlog);
+        HIPCHK(hipMemset(m.rlog, 0, sizeof(double) * n * R));
+    }
     own(m.dtmax);
     own(m.dtbad);
     HIPCHK(hipMemset(m.dtmax, 0, sizeof(unsigned long long) * R));
@@ -1358,7 +1366,7 @@ static int flush_pending(Model &m)
 {
     const int n = m.n;
     const long long nn = (long long)n * n;
-    if (n % 64 == 0) {
+    if (n % 64 == 0 && m.flush64) {
         // rank-P update on its own kernel (same sums as the GEMM)
         ProfScope psf_(DQMC_GPU_PROF_FLUSH, m.w.s);
         FlushArgs f;
@@ -1399,7 +1407,7 @@ static int stage_metro_sub(Model &m, int l, bool defer)
         a.rng = m.rng;
         a.nu[0] = m.nu[0]; a.nu[1] = m.nu[1]; a.nd[0] = m.nd[0]; a.nd[1] = m.nd[1];
         a.acc = m.acc; a.att = m.att; a.sign = m.sign; a.status = m.w.status;
-        a.K = m.w.W2; a.sK = nn; a.aq = m.w.perm; a.sA = n; a.pcount = m.pcount;
+        a.K = m.w.W2; a.sK = nn; a.aq = m.w.perm; a.sA = n; a.pcount = m.pcount; a.rlog = m.rlog;
         {
             ProfScope psp_(DQMC_GPU_PROF_PROPOSE, m.w.s);
             if (sm <= 64)
@@ -1446,7 +1454,7 @@ static int stage_metro(dqmc_gpu_ctx *ctx, Model &m, int l, bool defer = false)
             a.rng = m.rng;
             a.nu[0] = m.nu[0]; a.nu[1] = m.nu[1]; a.nd[0] = m.nd[0]; a.nd[1] = m.nd[1];
             a.acc = m.acc; a.att = m.att; a.sign = m.sign; a.status = m.w.status;
-            a.Cp = m.w.F1; a.Vp = m.w.F2; a.sP = nn; a.pcount = m.pcount;
+            a.Cp = m.w.F1; a.Vp = m.w.F2; a.sP = nn; a.pcount = m.pcount; a.rlog = m.rlog;
             {
                 ProfScope psp_(DQMC_GPU_PROF_PROPOSE, m.w.s);
                 launch_metro_w<float>(a, m.w.s);
@@ -1469,7 +1477,7 @@ static int stage_metro(dqmc_gpu_ctx *ctx, Model &m, int l, bool defer = false)
             a.rng = m.rng;
             a.nu[0] = m.nu[0]; a.nu[1] = m.nu[1]; a.nd[0] = m.nd[0]; a.nd[1] = m.nd[1];
             a.acc = m.acc; a.att = m.att; a.sign = m.sign; a.status = m.w.status;
-            a.Cp = m.w.W3; a.Vp = m.w.W4; a.sP = nn; a.pcount = m.pcount;
+            a.Cp = m.w.W3; a.Vp = m.w.W4; a.sP = nn; a.pcount = m.pcount; a.rlog = m.rlog;
             {
                 ProfScope psp_(DQMC_GPU_PROF_PROPOSE, m.w.s);
                 launch_metro_w<double>(a, m.w.s);
@@ -2374,6 +2382,8 @@ int dqmc_gpu_add_model(dqmc_gpu_ctx *ctx, int n, int L, int stab, int delay_k, c
     if (const char *e = getenv("DQMC_GPU_REBUILD")) m->two_sided = strcmp(e, "combine") != 0;
     if (const char *e = getenv("DQMC_GPU_PIVOT")) m->pivot = e[0] != '0';
     if (const char *e = getenv("DQMC_GPU_FUSE")) m->fuse = e[0] != '0';
+    if (const char *e = getenv("DQMC_GPU_RATIO_LOG")) m->rlog_on = e[0] == '1';
+    if (const char *e = getenv("DQMC_GPU_FLUSH64")) m->flush64 = e[0] != '0';
     // FP64: DQMC_GPU_METRO_SUB=k runs the Metropolis on the submatrix path
     // with k site segments (at least enough for segments of MSUB_MAX sites);
     // the autotuner also times it
@@ -2713,6 +2723,20 @@ int dqmc_gpu_read_tuning(dqmc_gpu_ctx *ctx, int model_id, int *window, int *segm
     if (window) *window = sub ? 0 : is_mixed(ctx) ? m.Wf : m.W;
     if (segments) *segments = sub ? m.sub : m.msplit;
     if (separable) *separable = m.kronF != nullptr ? 1 : 0;
+    return 0;
+}
+
+int dqmc_gpu_read_ratios(dqmc_gpu_ctx *ctx, int replica, double *ratios)
+{
+    Model *m;
+    int r;
+    if (!ratios || model_for_replica(ctx, replica, &m, &r)) return 1;
+    if (!m->rlog) {
+        set_err("ratio log off: set DQMC_GPU_RATIO_LOG=1 before adding the model");
+        return 1;
+    }
+    if (sync_all(ctx)) return 1;
+    HIPCHK(hipMemcpy(ratios, m->rlog + (size_t)r * m->n, sizeof(double) * m->n, hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -51,6 +51,7 @@ template <typename T> struct MetroArgs {
     T *Cp, *Vp;         // pending pairs, n x n per replica (q-major)
     long long sP;
     int *pcount;        // pending pair count per replica (the flush GEMM's K)
+    double *rlog;       // null, or the ratio of every proposal, n per replica
 };
 
 #define METRO_MAX_N 512
@@ -303,6 +304,7 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
                     const double Ru = 1.0 + (1.0 - gii) * Nu;
                     const double Rd = 1.0 + gii * Nd;
                     const double R = Ru * Rd;
+                    if (a.rlog && lane == 0) a.rlog[(long long)b * n + i] = R;
                     if (bcast_any(dr_l, w) < fabs(R)) {
                         const T fac = (T)(Nu / Ru);
                         // column w of the block -> c, row w -> v (values before the update)

@@ -59,6 +59,7 @@ struct MetroSubArgs {
     int *aq;                  // out: accepted sites, offsets from s0, in accept order
     long long sA;
     int *pcount;              // out: p
+    double *rlog;             // null, or the ratio of every proposal, n per replica
 };
 
 // LDS: the submatrix at entry m x (m+1), column and row buffers 2 x 2 x
@@ -159,6 +160,7 @@ __global__ __launch_bounds__(MSUB_THREADS) void k_metro_sub(MetroSubArgs a)
             const double Ru = 1.0 + (1.0 - gii) * Nu;
             const double Rd = 1.0 + gii * Nd;
             const double R = Ru * Rd;
+            if (a.rlog && tid == 0) a.rlog[(long long)b * n + s0 + i] = R;
             if (dri < fabs(R)) {
                 const double f = Nu / Ru;
                 double *cb = colb + (p & 1) * MSUB_MAX, *rb = rowb + (p & 1) * MSUB_MAX;

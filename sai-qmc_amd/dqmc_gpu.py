@@ -52,6 +52,7 @@ SIGNATURES = {
     "dqmc_gpu_read_drift": (_I, [_P, _P, _P, _P]),
     "dqmc_gpu_read_drift_at": (_I, [_P, _P, _P]),
     "dqmc_gpu_read_tuning": (_I, [_P, _I, _P, _P, _P]),
+    "dqmc_gpu_read_ratios": (_I, [_P, _I, _P]),
     "dqmc_gpu_read_green": (_I, [_P, _I, _P]),
     "dqmc_gpu_write_green": (_I, [_P, _I, _P]),
     "dqmc_gpu_read_field": (_I, [_P, _I, _P]),
@@ -338,6 +339,14 @@ class Engine:
         _check(lib().dqmc_gpu_read_tuning(self.h, model_id, C.byref(w), C.byref(sg), C.byref(sp)),
                "dqmc_gpu_read_tuning")
         return w.value, sg.value, sp.value
+
+    def ratios(self, r):
+        """R_up R_dn of each site at the latest Metropolis slice of replica r
+        (dqmc_gpu_read_ratios; the model needs DQMC_GPU_RATIO_LOG=1)."""
+        m = self._model_of(r)
+        x = np.zeros(m.n)
+        _check(lib().dqmc_gpu_read_ratios(self.h, r, _ptr(x)), "dqmc_gpu_read_ratios")
+        return x
 
     def green(self, r):
         m = self._model_of(r)

@@ -397,3 +397,118 @@ def test_submatrix_metropolis_matches_windowed(eng_mod, exact, monkeypatch, Lx, 
         e0 = np.abs(runs[1][-1][2][r] - gx).max() / np.abs(gx).max()
         print(f"replica {r}: submatrix G vs exact {e:.2e}, windowed {e0:.2e}", flush=True)
         assert e < 1e-10, (r, e)
+
+
+def exact_slice_ratios(eng_mod, exact, kind, Lx, Ly, U, dtau, L, l, before, after):
+    """R_up R_dn of every site of slice l, evaluated on the exact G(l) of the
+    field before the slice (80-bit, oracle/exact.c) with the slice's
+    accepted flips applied in order as rank-1 updates in extended precision
+    (green_delay_accept, src/green.c:418-460)."""
+    th, _, _ = eng_mod.lattice(kind, Lx, Ly)
+    lam = eng_mod.model(th, U, dtau)[3]
+    n = th.shape[0]
+    g = exact(kind, Lx, Ly, U, dtau, L, before, tau=l).astype(np.longdouble)
+    fb, fa = before.reshape(L, n)[l], after.reshape(L, n)[l]
+    out = np.zeros(n)
+    for i in range(n):
+        s = float(fb[i])
+        nu = np.expm1(np.longdouble(-2.0 * lam * s))
+        nd = np.expm1(np.longdouble(2.0 * lam * s))
+        gii = g[i, i]
+        ru = 1 + (1 - gii) * nu
+        out[i] = float(ru * (1 + gii * nd))
+        if fa[i] != fb[i]:
+            c = -(nu / ru) * g[:, i]
+            c[i] += nu / ru
+            g -= np.outer(c, g[i, :])
+    return out
+
+
+@pytest.mark.parametrize("kind,Lx,Ly,L,stab,sub", [("chain", 4, 1, 40, 10, "0"), ("square", 4, 4, 40, 10, "0"),
+                                                    ("square", 8, 8, 40, 10, "0"), ("square", 8, 8, 40, 10, "1"),
+                                                    ("square", 16, 16, 24, 8, "0")])
+def test_ratios_exact(eng_mod, orc, exact, monkeypatch, kind, Lx, Ly, L, stab, sub):
+    # Every proposal's ratio R_up R_dn against the exact ratio within 1e-10
+    # relative: slices 0, 1 and stab-1 stepped through the first block, and
+    # the last slice of a whole sweep (after every boundary rebuild).  The
+    # exact ratios use the exact G of the field before the slice; the
+    # oracle's own ratios (pinned to the determinant ratio on the CPU) are
+    # printed beside them: its FP64 G is ~1e-10 from the exact G at 8x8
+    # beta = 4 and beyond, so the engine is held to the exact ratios.  The
+    # accept sequence is the oracle's.  sub = "1": the submatrix path.
+    monkeypatch.setenv("DQMC_GPU_RATIO_LOG", "1")
+    monkeypatch.setenv("DQMC_GPU_METRO_SUB", sub)
+    U, dtau, seed = 4.0, 0.1, 7
+    ref = orc.Replica(kind, Lx, Ly, U, dtau, L, stab, seed, ratio_log=True)
+    assert ref.sweep() == 0
+    rref = ref.ratios()
+
+    def rel(x, y):
+        return (np.abs(x - y) / np.maximum(1.0, np.abs(y))).max()
+
+    print(f"{kind} {Lx}x{Ly} L={L}: stepping the first block", flush=True)
+    eng = eng_mod.Engine(eng_mod.FP64)
+    mid = eng.add_model(kind, Lx, Ly, U, dtau, L, stab, delay_k=16)
+    eng.add_replicas(mid, [seed])
+    eng.init()
+    for l in range(stab):
+        before = eng.field(0).copy()
+        eng.step(eng_mod.STEP_PROPOSE_SLICE, l)
+        if l in (0, 1, stab - 1):
+            x = eng.ratios(0)
+            rx = exact_slice_ratios(eng_mod, exact, kind, Lx, Ly, U, dtau, L, l, before, eng.field(0))
+            e, eo = rel(x, rx), rel(rref[l], rx)
+            print(f"slice {l}: ratios vs exact {e:.2e} (oracle {eo:.2e})", flush=True)
+            assert e < 1e-10, (l, e)
+        eng.step(eng_mod.STEP_WRAP, l)
+    n = eng.field(0).size // L
+    assert np.array_equal(eng.field(0)[:stab * n], ref.field()[:stab * n])
+    eng.close()
+    eng = eng_mod.Engine(eng_mod.FP64)
+    mid = eng.add_model(kind, Lx, Ly, U, dtau, L, stab, delay_k=16)
+    eng.add_replicas(mid, [seed])
+    eng.init()
+    init = eng.field(0).copy()
+    eng.sweep(1)
+    after = eng.field(0).copy()
+    assert np.array_equal(after, ref.field())
+    before = after.copy()
+    before[(L - 1) * n:] = init[(L - 1) * n:]
+    rx = exact_slice_ratios(eng_mod, exact, kind, Lx, Ly, U, dtau, L, L - 1, before, after)
+    e, eo = rel(eng.ratios(0), rx), rel(rref[L - 1], rx)
+    print(f"last slice of a sweep: ratios vs exact {e:.2e} (oracle {eo:.2e})", flush=True)
+    assert e < 1e-10
+    eng.close()
+
+
+@pytest.mark.parametrize("Lx,L,stab", [(8, 40, 10), (16, 24, 8)])
+def test_flush_kernel_bit_identical_to_gemm(eng_mod, monkeypatch, Lx, L, stab):
+    # k_flush64 sums the pending pairs in the GEMM's order (q four at a time,
+    # increasing) and subtracts once, so the chains with the rank-P flush on
+    # k_flush64 and on the batched GEMM (DQMC_GPU_FLUSH64=0) are the same
+    # bits: fields, accept counts and G after every sweep.  The wrap's fused
+    # flush is off in both so that every flush takes the switched path.
+    monkeypatch.setenv("DQMC_GPU_FUSE", "0")
+    seeds = [1, 2, 3, 4]
+    runs = []
+    for fk in ("1", "0"):
+        monkeypatch.setenv("DQMC_GPU_FLUSH64", fk)
+        print(f"{Lx}x{Lx}: DQMC_GPU_FLUSH64={fk}", flush=True)
+        eng = eng_mod.Engine(eng_mod.FP64)
+        mid = eng.add_model("square", Lx, Lx, 4.0, 0.1, L, stab, delay_k=16)
+        eng.add_replicas(mid, seeds)
+        eng.init()
+        traj = []
+        for _ in range(3):
+            eng.sweep(1)
+            st = eng.status()
+            assert (st["status"] == 0).all()
+            traj.append(([eng.field(r).copy() for r in range(len(seeds))], st["accepted"].copy(),
+                         [eng.green(r).copy() for r in range(len(seeds))]))
+        eng.close()
+        runs.append(traj)
+    for s, (a, b) in enumerate(zip(*runs)):
+        assert np.array_equal(a[1], b[1]), s
+        for r in range(len(seeds)):
+            assert np.array_equal(a[0][r], b[0][r]), (s, r)
+            assert np.array_equal(a[2][r], b[2][r]), (s, r, np.abs(a[2][r] - b[2][r]).max())

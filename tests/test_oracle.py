@@ -342,6 +342,55 @@ def test_u0_green_closed_form(orc):
     assert np.abs(r.green()[0] - Gx).max() < 1e-12
 
 
+@pytest.mark.parametrize("kind,Lx,Ly,L", [("chain", 4, 1, 10), ("square", 4, 4, 10), ("square", 4, 4, 20)])
+def test_ratio_log_is_the_determinant_ratio(orc, kind, Lx, Ly, L):
+    # The logged ratio of each proposal of slice 0 is the ratio of the
+    # determinants det(1 + B_{L-1} .. B_0) after and before the flip, for
+    # both spin species, with B_l = expK diag(expv_sigma[s_l]) built
+    # directly from the field (src/green.c:63-67, 126-137): the definition,
+    # independent of the R = (1 + (1 - G_ii) N_up)(1 + G_ii N_dn) form and of
+    # the PH map it uses.  Slice 0 is final after one forward sweep, so the
+    # field before proposal i is the initial one with sites < i taken from
+    # the swept one.  beta <= 2 keeps cond(1 + B..B) small enough for FP64
+    # determinants to 1e-9.
+    U, dtau, stab = 4.0, 0.1, 5
+    r = orc.Replica(kind, Lx, Ly, U, dtau, L, stab, 11, ratio_log=True)
+    n = r.n
+    s0 = r.field().reshape(L, n).copy()
+    assert r.sweep() == 0
+    s1 = r.field().reshape(L, n)
+    R = r.ratios()[0]
+    eK, _, lam = r.model()
+    ev = {}
+    for sg in (1.0, -1.0):
+        e = np.zeros(2)
+        orc.orc().orc_expv(lam, U, dtau, sg, e.ctypes.data_as(C.c_void_p))
+        ev[sg] = e
+
+    def det_ratio(field, i):
+        out = 1.0
+        for sg in (1.0, -1.0):
+            vals = []
+            for f in (field, None):
+                g = field.copy()
+                if f is None:
+                    g[0, i] = -g[0, i]
+                P = np.eye(n)
+                for l in range(L):
+                    P = (eK * ev[sg][(g[l] > 0).astype(int)][None, :]) @ P
+                sign, logdet = np.linalg.slogdet(np.eye(n) + P)
+                vals.append((sign, logdet))
+            out *= vals[1][0] * vals[0][0] * np.exp(vals[1][1] - vals[0][1])
+        return out
+
+    field = s0.copy()
+    for i in range(n):
+        x = det_ratio(field, i)
+        assert abs(R[i] - x) < 1e-9 * max(1.0, abs(x)), (i, R[i], x)
+        field[0, i] = s1[0, i]
+    assert np.array_equal(field, np.vstack([s1[:1], s0[1:]]))
+
+
 def test_drift_bound(orc):
     # tests/test_dqmc_stab_drift.c:53-58 (PH case): Ltr=8, stab=4, U=4,
     # 2 sweeps, 2 samples per sweep, max drift < 1e-8
