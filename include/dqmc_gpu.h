@@ -226,7 +226,8 @@ int dqmc_gpu_last_elapsed_ms(dqmc_gpu_ctx *ctx, double *ms);
 #define DQMC_GPU_PROF_GEMM 10   /* every n x n x n GEMM launch (also in its stage) */
 #define DQMC_GPU_PROF_FLUSH 11  /* every delayed-update flush launch (also in METRO) */
 #define DQMC_GPU_PROF_PROPOSE 12 /* every Metropolis proposal launch (also in METRO) */
-#define DQMC_GPU_PROF_N 13
+#define DQMC_GPU_PROF_PERSIST 13 /* one launch of the persistent sweep kernel (whole sweeps, n <= 64) */
+#define DQMC_GPU_PROF_N 14
 
 /* Profiling aid without a counterpart in the reference (which uses its
    PROF_BEGIN/PROF_END region timers, src/profiler.h:93-94): runs nsweep

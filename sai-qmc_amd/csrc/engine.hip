@@ -24,6 +24,7 @@
 #include "measure.hpp"
 #include "metro.hpp"
 #include "metro_sub.hpp"
+#include "persist.hpp"
 #include "qr.hpp"
 
 // ------------------------------------------------------------------ errors
@@ -1022,6 +1023,8 @@ struct Model {
     int sub = 0;       // FP64 Metropolis segments on the submatrix path (0: windowed path)
     bool flush64 = true; // rank-P flush on k_flush64 where n % 64 == 0 (DQMC_GPU_FLUSH64=0: the GEMM)
     bool two_sided = true; // boundary rebuilds by udv_inv_two_sided (one QR) instead of combine + inv_one_plus
+    bool persist_ok = true; // may run on the persistent sweep kernel (DQMC_GPU_PERSIST=0: never)
+    bool in_pset = false;   // runs on it in the current launch set
     int Lx = 1, Ly = 1, nq = 0, nbin = 0;
     double U = 0, dtau = 0, mu = 0, lambda = 0;
     double expv[2], expvinv[2], nu[2], nd[2];
@@ -1127,6 +1130,15 @@ struct dqmc_gpu_ctx {
     int device = 0;
     dqmc_gpu_opts opts;
     std::vector<Model *> models;
+    // the models swept by k_sweep_persist, in launch order (most work
+    // first), their descriptors on the device and the launch's workgroups
+    std::vector<Model *> pset;
+    PersistModel *d_pm = nullptr;
+    size_t d_pm_cap = 0;
+    int pblocks = 0;
+    hipEvent_t pev = nullptr;
+    unsigned long long *ptimers = nullptr; // phase timers of profiled persistent launches
+
     int nrep_total = 0;
     bool initialised = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -2056,9 +2068,10 @@ static int launch_sweep(dqmc_gpu_ctx *ctx, Model &m)
     return 0;
 }
 
-static int stage_measure(Model &m, int bin)
+static int stage_measure(Model &m, int bin, hipStream_t st = nullptr)
 {
-    ProfScope ps_(DQMC_GPU_PROF_MEASURE, m.w.s);
+    if (!st) st = m.w.s;
+    ProfScope ps_(DQMC_GPU_PROF_MEASURE, st);
     MeasureArgs a;
     memset(&a, 0, sizeof a);
     a.n = m.n; a.Lx = m.Lx; a.Ly = m.Ly; a.nq = m.nq; a.batch = m.nrep;
@@ -2069,8 +2082,112 @@ static int stage_measure(Model &m, int bin)
     a.bins = m.bins; a.counts = m.counts; a.bszz = m.bszz; a.bsperp = m.bsperp;
     a.sign = m.sign; a.acc = m.acc; a.att = m.att; a.acc_prev = m.acc_prev; a.att_prev = m.att_prev;
     a.status = m.w.status; a.reason = m.w.reason;
-    hipLaunchKernelGGL(k_measure, dim3(m.nrep), dim3(256), measure_lds_bytes(m.n), m.w.s, a);
+    hipLaunchKernelGGL(k_measure, dim3(m.nrep), dim3(256), measure_lds_bytes(m.n), st, a);
     return launch_check("k_measure");
+}
+
+// ------------------------------------------------------ persistent sweeps
+//
+// Models with n <= 64 in FP64, forward sweep order, the two-sided rebuild
+// and drift off or against the rebuild run their whole sweep in
+// k_sweep_persist (persist.hpp): one launch per sweep for all of them, on
+// the stream of the first, with its ticks and measurements after it.
+
+static bool persist_eligible(const dqmc_gpu_ctx *ctx, const Model &m)
+{
+    return m.persist_ok && m.nrep > 0 && m.allocated && m.n <= PS_NP && !is_mixed(ctx) && !m.alternating &&
+           !m.centered && m.two_sided && m.sub == 0 && m.BB != nullptr &&
+           ctx->opts.drift != DQMC_GPU_DRIFT_SCRATCH;
+}
+
+// the launch set and its device descriptors, rebuilt when the set changes
+static int persist_prepare(dqmc_gpu_ctx *ctx)
+{
+    std::vector<Model *> set;
+    for (Model *m : ctx->models)
+        if (persist_eligible(ctx, *m)) set.push_back(m);
+    // most work first, so that the longest sweeps start first
+    std::stable_sort(set.begin(), set.end(), [](const Model *a, const Model *b) {
+        return (double)a->n * a->n * a->n * a->L > (double)b->n * b->n * b->n * b->L;
+    });
+    if (set == ctx->pset) return 0;
+    for (Model *m : ctx->models) m->in_pset = false;
+    ctx->pset = set;
+    ctx->pblocks = 0;
+    if (set.empty()) return 0;
+    std::vector<PersistModel> pm(set.size());
+    for (size_t k = 0; k < set.size(); k++) {
+        Model &m = *set[k];
+        m.in_pset = true;
+        PersistModel &d = pm[k];
+        memset(&d, 0, sizeof d);
+        d.n = m.n; d.L = m.L; d.stab = m.stab; d.M = m.M; d.first = ctx->pblocks; d.nrep = m.nrep;
+        d.pivot = m.pivot ? 1 : 0;
+        d.record = ctx->opts.drift == DQMC_GPU_DRIFT_REBUILD ? 1 : 0;
+        d.guard = ctx->opts.drift_guard > 0.0 ? ctx->opts.drift_guard : 0.0;
+        for (int q = 0; q < 2; q++) {
+            d.e[q] = m.expv[q]; d.einv[q] = m.expvinv[q]; d.nu[q] = m.nu[q]; d.nd[q] = m.nd[q];
+        }
+        d.expK = m.expK; d.expKinv = m.expKinv; d.G = m.G; d.field = m.field; d.dvec = m.dvec; d.dinv = m.dinv;
+        d.rng = m.rng; d.BB = m.BB; d.SU = m.SU; d.SD = m.SD; d.ST = m.ST; d.Sq = m.Sq;
+        d.LU = m.LU_; d.LD = m.LD; d.LT = m.LT; d.Lq = m.Lq;
+        d.acc = m.acc; d.att = m.att; d.sign = m.sign; d.status = m.w.status; d.reason = m.w.reason;
+        d.dmax = m.dmax; d.dsum = m.dsum; d.dcnt = m.dcnt; d.dtaumax = m.dtaumax; d.dsweepmax = m.dsweepmax;
+        d.sweeps = m.sweeps; d.rlog = m.rlog;
+        ctx->pblocks += m.nrep;
+    }
+    const size_t bytes = pm.size() * sizeof(PersistModel);
+    if (bytes > ctx->d_pm_cap) {
+        if (ctx->d_pm) HIPCHK(hipFree(ctx->d_pm));
+        ctx->d_pm = nullptr;
+        ctx->d_pm_cap = 0;
+        HIPCHK(hipMalloc(&ctx->d_pm, bytes));
+        ctx->d_pm_cap = bytes;
+    }
+    HIPCHK(hipMemcpy(ctx->d_pm, pm.data(), bytes, hipMemcpyHostToDevice));
+    if (!ctx->pev) HIPCHK(hipEventCreateWithFlags(&ctx->pev, hipEventDisableTiming));
+    return 0;
+}
+
+// one sweep of every model of the launch set (bin >= 0: measured into bin)
+static int persist_sweep(dqmc_gpu_ctx *ctx, int bin)
+{
+    if (ctx->pset.empty()) return 0;
+    hipStream_t st = ctx->pset[0]->w.s;
+    for (Model *m : ctx->pset)
+        if (!m->bb_valid) {
+            // block products from fields written by the host or by single steps
+            if (bblock_refresh(*m)) return 1;
+            HIPCHK(hipStreamSynchronize(m->w.s));
+        }
+    {
+        ProfScope ps_(DQMC_GPU_PROF_PERSIST, st);
+        // a profiled sweep also takes the phase timers of workgroup 0
+        unsigned long long *tm = nullptr;
+        if (g_prof) {
+            if (!ctx->ptimers) HIPCHK(hipMalloc(&ctx->ptimers, PS_NTIMER * sizeof(unsigned long long)));
+            HIPCHK(hipMemsetAsync(ctx->ptimers, 0, PS_NTIMER * sizeof(unsigned long long), st));
+            tm = ctx->ptimers;
+        }
+        hipLaunchKernelGGL(k_sweep_persist<PS_W>, dim3(ctx->pblocks), dim3(PS_THREADS), ps_lds_bytes(), st, ctx->d_pm,
+                           (int)ctx->pset.size(), tm);
+        if (launch_check("k_sweep_persist")) return 1;
+    }
+    for (Model *m : ctx->pset) {
+        hipLaunchKernelGGL(k_sweep_tick, dim3(1), dim3(64), 0, st, m->sweeps);
+        if (launch_check("k_sweep_tick")) return 1;
+        if (bin >= 0 && stage_measure(*m, bin, st)) return 1;
+    }
+    return 0;
+}
+
+// the other models of the launch set wait for its stream
+static int persist_join(dqmc_gpu_ctx *ctx)
+{
+    if (ctx->pset.size() < 2) return 0;
+    HIPCHK(hipEventRecord(ctx->pev, ctx->pset[0]->w.s));
+    for (size_t k = 1; k < ctx->pset.size(); k++) HIPCHK(hipStreamWaitEvent(ctx->pset[k]->w.s, ctx->pev, 0));
+    return 0;
 }
 
 static int set_lds_limits()
@@ -2085,7 +2202,7 @@ static int set_lds_limits()
     }
     if (done_dev[dev]) return 0;
     // 160 KiB of LDS per CU on gfx950, less each kernel's static LDS
-    const void *kernels[] = {(const void *)k_metro_sub<1, 8>, (const void *)k_metro_sub<2, 12>,
+    const void *kernels[] = {(const void *)k_sweep_persist<PS_W>, (const void *)k_metro_sub<1, 8>, (const void *)k_metro_sub<2, 12>,
                              (const void *)k_metro_slice<double, 16>,
                              (const void *)k_metro_slice<double, 8>,  (const void *)k_metro_slice<float, 32>,
                              (const void *)k_metro_slice<float, 16>,  (const void *)k_metro_slice<float, 8>,
@@ -2384,6 +2501,7 @@ int dqmc_gpu_add_model(dqmc_gpu_ctx *ctx, int n, int L, int stab, int delay_k, c
     if (const char *e = getenv("DQMC_GPU_FUSE")) m->fuse = e[0] != '0';
     if (const char *e = getenv("DQMC_GPU_RATIO_LOG")) m->rlog_on = e[0] == '1';
     if (const char *e = getenv("DQMC_GPU_FLUSH64")) m->flush64 = e[0] != '0';
+    if (const char *e = getenv("DQMC_GPU_PERSIST")) m->persist_ok = e[0] != '0';
     // FP64: DQMC_GPU_METRO_SUB=k runs the Metropolis on the submatrix path
     // with k site segments (at least enough for segments of MSUB_MAX sites);
     // the autotuner also times it
@@ -2555,9 +2673,17 @@ int dqmc_gpu_init(dqmc_gpu_ctx *ctx)
         if (stage_from_scratch(ctx, *m, 0)) return 1; // also forms every block product
         m->bb_valid = m->BB != nullptr;
     }
+    if (persist_prepare(ctx)) return 1;
     if (ctx->opts.autotune)
         for (size_t k = 0; k < ctx->models.size(); k++) {
             Model *m = ctx->models[k];
+            if (m->nrep > 0 && m->in_pset) {
+                fprintf(stderr,
+                        "dqmc_gpu autotune: model %zu (n=%d, L=%d, %d replicas): persistent sweep kernel "
+                        "(one workgroup per replica, G in LDS, 16-site windows)\n",
+                        k, m->n, m->L, m->nrep);
+                continue;
+            }
             if (m->nrep > 0 && tune_model(ctx, *m, (int)k)) return 1;
         }
     if (sync_all(ctx)) return 1;
@@ -2588,14 +2714,17 @@ int dqmc_gpu_sweep(dqmc_gpu_ctx *ctx, int nsweep, int bin)
             return 1;
         }
     if (sync_all(ctx)) return 1;
+    if (persist_prepare(ctx)) return 1;
     HIPCHK(hipEventRecord(ctx->ev0, ctx->models.empty() ? nullptr : ctx->models[0]->w.s));
     for (int s = 0; s < nsweep; s++) {
+        if (persist_sweep(ctx, bin)) return 1;
         for (Model *m : ctx->models) {
-            if (m->nrep == 0) continue;
+            if (m->nrep == 0 || m->in_pset) continue;
             if (launch_sweep(ctx, *m)) return 1;
             if (bin >= 0 && stage_measure(*m, bin)) return 1;
         }
     }
+    if (persist_join(ctx)) return 1;
     if (bin < 0)
         for (Model *m : ctx->models) {
             if (m->nrep == 0) continue;
@@ -2867,12 +2996,25 @@ int dqmc_gpu_profile_sweep(dqmc_gpu_ctx *ctx, int nsweep, double *ms, int *launc
         return 1;
     }
     if (sync_all(ctx)) return 1;
+    if (persist_prepare(ctx)) return 1;
     std::vector<ProfRec> recs;
     g_prof = &recs;
     int rc = 0;
-    for (int s = 0; s < nsweep && !rc; s++)
+    // phase times of the persistent kernel's workgroup 0 (100 MHz ticks)
+    unsigned long long ptm[PS_NTIMER] = {};
+    for (int s = 0; s < nsweep && !rc; s++) {
+        rc = persist_sweep(ctx, -1);
+        if (!rc && !ctx->pset.empty()) {
+            unsigned long long t[PS_NTIMER];
+            HIPCHK(hipStreamSynchronize(ctx->pset[0]->w.s));
+            HIPCHK(hipMemcpy(t, ctx->ptimers, sizeof t, hipMemcpyDeviceToHost));
+            for (int k = 0; k < PS_NTIMER; k++) ptm[k] += t[k];
+        }
+        for (Model *m : ctx->pset)
+            if (!rc && m->nbin > 0) rc = stage_measure(*m, 0, ctx->pset[0]->w.s);
         for (Model *m : ctx->models) {
-            if (m->nrep == 0) continue;
+            if (rc) break;
+            if (m->nrep == 0 || m->in_pset) continue;
             rc = (m->BB && !m->bb_valid) ? bblock_refresh(*m) : 0;
             const int v = sweep_variant(*m);
             if (!rc) rc = sweep_body(ctx, *m, v);
@@ -2880,7 +3022,9 @@ int dqmc_gpu_profile_sweep(dqmc_gpu_ctx *ctx, int nsweep, double *ms, int *launc
             if (!rc && m->nbin > 0) rc = stage_measure(*m, 0);
             if (rc) break;
         }
+    }
     g_prof = nullptr;
+    if (!rc) rc = persist_join(ctx);
     const int sync_rc = sync_all(ctx);
     for (int c = 0; c < DQMC_GPU_PROF_N; c++) {
         ms[c] = 0.0;
@@ -2894,6 +3038,24 @@ int dqmc_gpu_profile_sweep(dqmc_gpu_ctx *ctx, int nsweep, double *ms, int *launc
         }
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
+    }
+    // the persistent kernel's phases (workgroup 0, in-kernel wall clock)
+    // into the stage classes they correspond to; launches stay those of the
+    // batched stages
+    if (!ctx->pset.empty()) {
+        const double tick_ms = 1e-5; // 100 MHz
+        ms[DQMC_GPU_PROF_STACK] += ptm[PT_SUFFIX] * tick_ms;
+        ms[DQMC_GPU_PROF_METRO] += ptm[PT_METRO] * tick_ms;
+        ms[DQMC_GPU_PROF_WRAP] += (ptm[PT_PRODUCT] + ptm[PT_WRAP]) * tick_ms;
+        ms[DQMC_GPU_PROF_LMUL] += ptm[PT_LMUL] * tick_ms;
+        ms[DQMC_GPU_PROF_INV] += (ptm[PT_REBUILD_QR] + ptm[PT_REBUILD_SOLVE] + ptm[PT_REBUILD_G]) * tick_ms;
+        if (getenv("DQMC_GPU_PERSIST_PHASES")) {
+            static const char *nm[PS_NTIMER] = {"suffix", "metro", "product", "wrap", "lmul", "rebuild_qr",
+                                                "rebuild_solve", "rebuild_g"};
+            fprintf(stderr, "dqmc_gpu persistent phases (ms, workgroup 0, %d sweeps):", nsweep);
+            for (int k = 0; k < PS_NTIMER; k++) fprintf(stderr, " %s %.4f", nm[k], ptm[k] * tick_ms);
+            fprintf(stderr, "\n");
+        }
     }
     return rc || sync_rc;
 }
@@ -2920,6 +3082,9 @@ void dqmc_gpu_destroy(dqmc_gpu_ctx *ctx)
     }
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->pev) (void)hipEventDestroy(ctx->pev);
+    if (ctx->d_pm) (void)hipFree(ctx->d_pm);
+    if (ctx->ptimers) (void)hipFree(ctx->ptimers);
     delete ctx;
 }
 

@@ -317,7 +317,7 @@ class Engine:
         return v.value
 
     PROF_NAMES = ["metro", "wrap", "stack", "lmul", "combine", "inv_one_plus", "accept", "measure", "qr",
-                  "trsm", "gemm", "flush", "propose"]
+                  "trsm", "gemm", "flush", "propose", "persist"]
 
     def profile(self, nsweep=1):
         """Per-stage milliseconds and stage counts of nsweep sweeps

@@ -35,12 +35,16 @@ def run_traj(eng_mod, seeds, nsw, drift=False, guard=0.0, order="forward", nan_a
     return eng, out
 
 
-def test_nonfinite_green_fails_sticky(eng_mod):
+@pytest.mark.parametrize("persist", ["1", "0"])
+def test_nonfinite_green_fails_sticky(eng_mod, monkeypatch, persist):
     # a NaN written into one replica's G: the rebuild that ends its block
     # finds the wrapped G non-finite and fails the replica with
     # DQMC_GPU_FAIL_NONFINITE; from then on its sweeps change nothing
     # (fields, accept and attempt counters frozen), and the other replicas
-    # follow exactly the trajectory of an undisturbed run
+    # follow exactly the trajectory of an undisturbed run.  On the
+    # persistent sweep kernel (4 x 4 runs there by default) and on the
+    # batched path.
+    monkeypatch.setenv("DQMC_GPU_PERSIST", persist)
     seeds = [3, 4, 5]
     print("NaN injection into replica 1 before sweep 1", flush=True)
     eng, bad = run_traj(eng_mod, seeds, 4, nan_at=(1, 1))

@@ -487,8 +487,10 @@ def test_flush_kernel_bit_identical_to_gemm(eng_mod, monkeypatch, Lx, L, stab):
     # increasing) and subtracts once, so the chains with the rank-P flush on
     # k_flush64 and on the batched GEMM (DQMC_GPU_FLUSH64=0) are the same
     # bits: fields, accept counts and G after every sweep.  The wrap's fused
-    # flush is off in both so that every flush takes the switched path.
+    # flush is off in both so that every flush takes the switched path, and
+    # the 8 x 8 case runs on the batched path (not the persistent kernel).
     monkeypatch.setenv("DQMC_GPU_FUSE", "0")
+    monkeypatch.setenv("DQMC_GPU_PERSIST", "0")
     seeds = [1, 2, 3, 4]
     runs = []
     for fk in ("1", "0"):
