@@ -45,7 +45,7 @@
 enum { PT_SUFFIX, PT_METRO, PT_PRODUCT, PT_WRAP, PT_LMUL, PT_REBUILD_QR, PT_REBUILD_SOLVE, PT_REBUILD_G };
 
 // One model of the launch (device memory, read-only during the sweep).
-struct PersistModel {
+struct __align__(8) PersistModel {
     int n, L, stab, M, first, nrep;  // workgroups [first, first + nrep) run this model
     int pivot;                       // column pre-pivoted UDV updates
     int record;                      // drift samples at rebuilds (DQMC_GPU_DRIFT_REBUILD)
@@ -88,30 +88,32 @@ struct PsState {
     unsigned long long tm[PS_NTIMER];
 };
 
-// LDS carve-up (doubles unless noted), 161,760 bytes
+// LDS carve-up (doubles unless noted), about 162.5 KB
 struct PsLds {
     double *slot[4];
     double *Cw, *Vw;     // window pairs, q-major (PS_W x PS_NP each); the QR
                          // phases reuse them as partial-sum and row buffers
-    double *cwin, *vwin, *facw, *draws;
+    double *cwin, *vwin, *facw;
+    double *draws;       // two slices' uniform draws (slice l in draws + (l & 1) PS_NP)
     double *dl, *dli;    // diagonal of the slice's B and its inverse
     double *DL, *DR;     // D of the left UDV, of the suffix entry
     double *lbi, *ls, *rbi, *rs; // Db^-1 and Ds of DL, DR
-    double *DM, *tau, *keys;
-    double *pn;          // 2 x 17 norm partials
+    double *DM, *tau, *keys, *scl;
+    double *rk;          // 2 x 64: row k of a QR step
     double *red;         // 16 reduction slots
     int *perm;
-    signed char *fs, *flip;
+    signed char *fs, *flip; // spins of two slices (slice l at fs + (l & 1) PS_NP), flips
     unsigned *mask;      // the accepts of a window, two alternating words
     int *flag;
     PsState *st;
+    PersistModel *pm;    // the workgroup's model, copied from device memory
 };
 
 __host__ __device__ static inline size_t ps_lds_bytes()
 {
-    return sizeof(double) * (4 * (size_t)PS_MAT + 2 * PS_W * PS_NP + 2 * PS_W * PS_W + PS_W + PS_NP + 13 * PS_NP +
-                             34 + 16) +
-           sizeof(int) * PS_NP + 2 * PS_NP + 16 + sizeof(PsState);
+    return sizeof(double) * (4 * (size_t)PS_MAT + 2 * PS_W * PS_NP + 2 * PS_W * PS_W + PS_W + 2 * PS_NP + 12 * PS_NP +
+                             128 + 16) +
+           sizeof(int) * PS_NP + 3 * PS_NP + 16 + sizeof(PsState) + sizeof(PersistModel);
 }
 
 
@@ -128,22 +130,29 @@ __device__ __forceinline__ PsLds ps_carve(unsigned char *raw)
     s.cwin = p; p += PS_W * PS_W;
     s.vwin = p; p += PS_W * PS_W;
     s.facw = p; p += PS_W;
-    s.draws = p; p += PS_NP;
-    double **vecs[] = {&s.dl, &s.dli, &s.DL, &s.DR, &s.lbi, &s.ls, &s.rbi, &s.rs, &s.DM, &s.tau, &s.keys};
-    for (double **v : vecs) {
-        *v = p;
-        p += PS_NP;
-    }
-    p += 2 * PS_NP; // spare vectors (keeps the byte count above in step)
-    s.pn = p; p += 34;
+    s.draws = p; p += 2 * PS_NP;
+    s.dl = p; p += PS_NP;
+    s.dli = p; p += PS_NP;
+    s.DL = p; p += PS_NP;
+    s.DR = p; p += PS_NP;
+    s.lbi = p; p += PS_NP;
+    s.ls = p; p += PS_NP;
+    s.rbi = p; p += PS_NP;
+    s.rs = p; p += PS_NP;
+    s.DM = p; p += PS_NP;
+    s.tau = p; p += PS_NP;
+    s.keys = p; p += PS_NP;
+    s.scl = p; p += PS_NP;
+    s.rk = p; p += 128;
     s.red = p; p += 16;
     s.perm = (int *)p;
     s.fs = (signed char *)(s.perm + PS_NP);
-    s.flip = s.fs + PS_NP;
+    s.flip = s.fs + 2 * PS_NP;
     s.mask = (unsigned *)(s.flip + PS_NP);
     s.flag = (int *)(s.mask + 2);
     s.st = (PsState *)(s.mask + 4);
     s.flag = &s.st->flag;
+    s.pm = (PersistModel *)(s.st + 1);
     return s;
 }
 
@@ -159,6 +168,16 @@ __device__ __forceinline__ void ps_tick(const PsLds &s, int k)
     }
 }
 
+// the wave index as a uniform (scalar) value
+__device__ __forceinline__ int ps_wave() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+// a double equal in every lane, as a uniform value
+__device__ __forceinline__ double ps_unid(double v)
+{
+    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
 // a double of a wave-uniform lane
 __device__ __forceinline__ double ps_rl(double v, int lane)
 {
@@ -166,6 +185,33 @@ __device__ __forceinline__ double ps_rl(double v, int lane)
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
     return __hiloint2double(hi, lo);
 }
+// the same, moved into vector registers at once (values that stay live
+// across a barrier would otherwise crowd the scalar registers)
+__device__ __forceinline__ double ps_rlv(double v, int lane)
+{
+    double r = ps_rl(v, lane);
+    asm volatile("" : "+v"(r));
+    return r;
+}
+// The value of 16-lane row h0 (0..3) of the wave in every row, lane for
+// lane (the gfx950 row swaps: the 32-lane swap of x with itself gives, in
+// its two results, rows h & 1 and 2 + (h & 1); the 16-lane swap gives rows
+// h & ~1 and h | 1).  h0 folds to a constant in unrolled code.
+__device__ __forceinline__ double ps_xrow(double v, int h0)
+{
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const auto l32 = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto h32 = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    const int lo1 = (h0 & 2) ? l32[1] : l32[0], hi1 = (h0 & 2) ? h32[1] : h32[0];
+    const auto l16 = __builtin_amdgcn_permlane16_swap(lo1, lo1, false, false);
+    const auto h16 = __builtin_amdgcn_permlane16_swap(hi1, hi1, false, false);
+    return __hiloint2double((h0 & 1) ? h16[1] : h16[0], (h0 & 1) ? l16[1] : l16[0]);
+}
+
+// a 0/1 factor for a wave-uniform condition (a scalar value, so that a
+// row of a register array is chosen by arithmetic, not by an index the
+// compiler could turn into a memory access)
+__device__ __forceinline__ double ps_ind(bool cnd) { return cnd ? 1.0 : 0.0; }
 
 // C = A B over the 16 x 16 tiles of an n x n product (at most 4 x 4 tiles,
 // two per wave), FP64 MFMA 16x16x4.  fa(i, k) and fb(k, j) give operand
@@ -175,7 +221,7 @@ __device__ __forceinline__ double ps_rl(double v, int lane)
 template <class FA, class FB, class FO>
 __device__ __forceinline__ void ps_gemm(int n, FA fa, FB fb, FO fo, bool inplace)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = ps_wave();
     const int lc = lane & 15, lr = lane >> 4;
     const int nt = (n + 15) >> 4, K = (n + 3) & ~3;
     f64x4 acc[2];
@@ -210,7 +256,7 @@ __device__ __forceinline__ void ps_gemm(int n, FA fa, FB fb, FO fo, bool inplace
 template <class FA1, class FB1, class FA2, class FB2, class FO>
 __device__ __forceinline__ void ps_gemm2(int n, FA1 fa1, FB1 fb1, FA2 fa2, FB2 fb2, FO fo)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = ps_wave();
     const int lc = lane & 15, lr = lane >> 4;
     const int nt = (n + 15) >> 4, K = (n + 3) & ~3;
 #pragma unroll
@@ -257,7 +303,7 @@ __device__ __forceinline__ void ps_identity(double *S, int n)
 // of S (col = null: column c).
 __device__ __forceinline__ void ps_to_regs(double (&a)[8], const double *S, int n, const int *col)
 {
-    const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int c = threadIdx.x & 63, w = ps_wave();
     const int cc = (c < n) ? (col ? col[c] : c) : 0;
 #pragma unroll
     for (int m = 0; m < 8; m++) {
@@ -267,7 +313,7 @@ __device__ __forceinline__ void ps_to_regs(double (&a)[8], const double *S, int 
 }
 __device__ __forceinline__ void ps_from_regs(double *S, const double (&a)[8], int n)
 {
-    const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int c = threadIdx.x & 63, w = ps_wave();
 #pragma unroll
     for (int m = 0; m < 8; m++) {
         const int r = 8 * w + m;
@@ -277,91 +323,109 @@ __device__ __forceinline__ void ps_from_regs(double *S, const double (&a)[8], in
 
 // Householder QR, column by column as LAPACK dgeqr2 with dlarfg's
 // reflectors (beta = -sign(alpha) ||(alpha, x)||, tau = (beta - alpha) /
-// beta, v = (1, x / (alpha - beta)); the norm is taken scaled by max|x| so
-// that no square overflows).  On return a[] holds R on and above the
-// diagonal and the reflectors below it, tau[k] is in LDS.  Returns the
+// beta, v = (1, x / (alpha - beta))).  On return a[] holds R on and above
+// the diagonal and the reflectors below it, tau[k] is in LDS.  Returns the
 // number of reflectors with tau != 0 (sgn det Q = (-1)^that); *bad is set
-// (by thread 0) when a reflector is not finite.  Two barriers per column:
-// one after the norm partials, one after the dot-product partials.
-__device__ int ps_qr(double (&a)[8], int n, double *tau, double *pn, double *pd, int *bad)
+// (by thread 0) when a reflector is not finite.  The input must be
+// equilibrated (entries far below 1e150, as the chain's inputs are after
+// ps_pivot's power-of-two column scales, and the rebuild's bounded M).
+//
+// One barrier per column: before it every wave writes, for every column c,
+// its partial of d_c = x^T a_c over its rows below the diagonal (x the part
+// of column k below it, so d_k = ||x||^2), and the wave holding row k writes
+// that row; after it every thread has alpha, ||x||^2, d_c and a_kc, forms
+// the reflector and w_c = v^T a_c = a_kc + d_c / (alpha - beta), and
+// updates its rows of column c.  The two buffers alternate between columns.
+__device__ int ps_qr(double (&a)[8], int n, double *tau, double *pd, double *rk, double *Vs, int *bad)
 {
-    const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int c = threadIdx.x & 63, w = ps_wave();
     int ntau = 0;
     for (int k = 0; k < n; k++) {
-        double *pk = pn + (k & 1) * 17;
-        if (c == k) {
-            double mx = 0.0, al = 0.0;
+        double *pb = pd + (k & 1) * 512, *rb = rk + (k & 1) * 64;
+        // rows 8w + m of this wave lie below the diagonal for m > rel: all of
+        // them for rel < 0, none for rel > 6 (wave-uniform branches)
+        const int rel = k - 8 * w;
+        // x: this wave's rows of column k below the diagonal (from lane k,
+        // kept in vector registers; rows past n hold 0); the partial dot
+        // products with every column, two accumulators
+        double x[8];
+        double p0 = 0.0, p1 = 0.0;
+        if (rel < 0) {
 #pragma unroll
             for (int m = 0; m < 8; m++) {
-                const int r = 8 * w + m;
-                if (r > k && r < n) mx = fmax(mx, fabs(a[m]));
-                if (r == k) al = a[m];
+                x[m] = ps_rlv(a[m], k);
+                if (m & 1) p1 += x[m] * a[m];
+                else p0 += x[m] * a[m];
             }
-            double s = 0.0;
-            if (mx > 0.0) {
+        } else if (rel < 7) {
 #pragma unroll
-                for (int m = 0; m < 8; m++) {
-                    const int r = 8 * w + m;
-                    const double t = a[m] / mx;
-                    if (r > k && r < n) s += t * t;
-                }
+            for (int m = 0; m < 8; m++) {
+                x[m] = ps_rlv(a[m], k) * ps_ind(m > rel);
+                if (m & 1) p1 += x[m] * a[m];
+                else p0 += x[m] * a[m];
             }
-            pk[2 * w] = mx;
-            pk[2 * w + 1] = s;
-            if (w == (k >> 3)) pk[16] = al;
+        } else {
+#pragma unroll
+            for (int m = 0; m < 8; m++) x[m] = 0.0;
+        }
+        pb[w * 64 + c] = p0 + p1;
+        if (rel >= 0 && rel < 8) {
+            double rv = 0.0;
+#pragma unroll
+            for (int m = 0; m < 8; m++) rv += ps_ind(m == rel) * a[m];
+            rb[c] = rv;
         }
         __syncthreads();
-        double mx = 0.0;
+        double q[8], e[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) mx = fmax(mx, pk[2 * u]);
-        double s = 0.0;
-        if (mx > 0.0) {
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const double q = pk[2 * u] / mx;
-                s += pk[2 * u + 1] * q * q;
-            }
+        for (int u = 0; u < 8; u++) {
+            q[u] = pb[u * 64 + k];
+            e[u] = pb[u * 64 + c];
         }
-        const double xn = mx * sqrt(s), alpha = pk[16];
+        const double xn2 = ps_unid(((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])));
+        const double dc = ((e[0] + e[1]) + (e[2] + e[3])) + ((e[4] + e[5]) + (e[6] + e[7]));
+        const double alpha = ps_unid(rb[k]), akc = rb[c];
         double t = 0.0, sc = 0.0, beta = alpha;
-        if (xn != 0.0) {
-            beta = -copysign(hypot(alpha, xn), alpha);
-            t = (beta - alpha) / beta;
-            sc = 1.0 / (alpha - beta);
+        if (xn2 != 0.0) {
+            // tau = (beta - alpha) / beta and sc = 1 / (alpha - beta) from one
+            // reciprocal of (alpha - beta) beta
+            beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+            const double d = alpha - beta, inv = 1.0 / (d * beta);
+            t = -(d * d) * inv;
+            sc = beta * inv;
         }
         if (threadIdx.x == 0) {
             tau[k] = t;
-            if (!isfinite(t) || !isfinite(sc)) *bad = 1;
+            if (!isfinite(t) || !isfinite(sc) || !isfinite(xn2)) *bad = 1;
         }
         if (t != 0.0) ntau++;
-        double v[8];
+        if (t != 0.0 && rel < 8) {
+            // the trailing columns (lanes k < c < n) take the reflector:
+            // a_c -= t v (v^T a_c), v = (1, sc x), v^T a_c = a_kc + sc d_c
+            const bool upd = c > k && c < n;
+            const double tw = upd ? t * (akc + sc * dc) : 0.0, tsw = tw * sc;
+            if (rel < 0) {
+#pragma unroll
+                for (int m = 0; m < 8; m++) a[m] -= tsw * x[m];
+            } else {
+#pragma unroll
+                for (int m = 0; m < 8; m++) a[m] -= tsw * x[m] + ps_ind(m == rel) * tw;
+            }
+        }
+        // lane k: the reflector below the diagonal (its x scaled), beta on it
+        if (c == k) {
+#pragma unroll
+            for (int m = 0; m < 8; m++)
+                a[m] = a[m] * ((m > rel) ? sc : ps_ind(m < rel)) + ps_ind(m == rel) * beta;
+        }
+    }
+    // the reflectors into Vs: column c is v_c, unit entry, zeros above, the
+    // stored entries below (rows past n stay 0)
+    if (c < n) {
 #pragma unroll
         for (int m = 0; m < 8; m++) {
             const int r = 8 * w + m;
-            const double x = ps_rl(a[m], k);
-            v[m] = (r == k) ? 1.0 : ((r > k && r < n) ? x * sc : 0.0);
-        }
-        if (t != 0.0) {
-            double p = 0.0;
-#pragma unroll
-            for (int m = 0; m < 8; m++) p += v[m] * a[m];
-            pd[w * 64 + c] = p;
-            __syncthreads();
-            double wc = 0.0;
-#pragma unroll
-            for (int u = 0; u < 8; u++) wc += pd[u * 64 + c];
-            if (c > k && c < n) {
-#pragma unroll
-                for (int m = 0; m < 8; m++) a[m] -= t * v[m] * wc;
-            }
-        }
-        if (c == k) {
-#pragma unroll
-            for (int m = 0; m < 8; m++) {
-                const int r = 8 * w + m;
-                if (r == k) a[m] = beta;
-                else if (r > k && r < n) a[m] = v[m];
-            }
+            if (r < n) Vs[r + c * PS_LD] = (r > c) ? a[m] : ((r == c) ? 1.0 : 0.0);
         }
     }
     __syncthreads();
@@ -369,46 +433,65 @@ __device__ int ps_qr(double (&a)[8], int n, double *tau, double *pn, double *pd,
 }
 
 // x <- H_k x for k in order (forward: H_0 first, i.e. x <- Q^T x; backward:
-// H_{n-1} first, i.e. x <- Q x), the reflectors in a[] below the diagonal
-// and tau in LDS.  One barrier per reflector; the partial-sum buffer
+// H_{n-1} first, i.e. x <- Q x), the reflectors in the LDS matrix Vs (as
+// ps_qr leaves them: column k is v_k with its unit entry and zeros above,
+// rows past n zero) and tau in LDS.  One barrier per reflector; the partial-sum buffer
 // alternates so that a step's reads never meet the next step's writes.
-__device__ void ps_apply_h(const double (&a)[8], double (&x)[8], int n, const double *tau, double *pd, bool forward)
+__device__ void ps_apply_h(const double *Vs, double (&x)[8], int n, const double *tau, double *pd, bool forward)
 {
-    const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int c = threadIdx.x & 63, w = ps_wave();
     int ph = 0;
+    // reflector k's tau and this wave's eight rows of it (zero past n), the
+    // next one's read while the current one is applied
+    int k = forward ? 0 : n - 1;
+    double t = ps_unid(tau[k]);
+    double v[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) v[m] = Vs[8 * w + m + k * PS_LD];
     for (int s = 0; s < n; s++) {
-        const int k = forward ? s : n - 1 - s;
-        const double t = tau[k];
-        if (t == 0.0) continue;
-        double v[8];
-        double p = 0.0;
+        const int kn = forward ? k + 1 : k - 1;
+        const bool more = s + 1 < n;
+        double tn = 0.0, vn[8];
+        if (more) {
+            tn = ps_unid(tau[kn]);
 #pragma unroll
-        for (int m = 0; m < 8; m++) {
-            const int r = 8 * w + m;
-            const double y = ps_rl(a[m], k);
-            v[m] = (r == k) ? 1.0 : ((r > k && r < n) ? y : 0.0);
-            p += v[m] * x[m];
+            for (int m = 0; m < 8; m++) vn[m] = Vs[8 * w + m + kn * PS_LD];
         }
-        double *pb = pd + ph * 512;
-        ph ^= 1;
-        pb[w * 64 + c] = p;
-        __syncthreads();
-        double wc = 0.0;
+        if (t != 0.0) {
+            double p0 = 0.0, p1 = 0.0;
 #pragma unroll
-        for (int u = 0; u < 8; u++) wc += pb[u * 64 + c];
+            for (int m = 0; m < 8; m++) {
+                if (m & 1) p1 += v[m] * x[m];
+                else p0 += v[m] * x[m];
+            }
+            double *pb = pd + ph * 512;
+            ph ^= 1;
+            pb[w * 64 + c] = p0 + p1;
+            __syncthreads();
+            double e[8];
 #pragma unroll
-        for (int m = 0; m < 8; m++) x[m] -= t * v[m] * wc;
+            for (int u = 0; u < 8; u++) e[u] = pb[u * 64 + c];
+            const double tw = t * (((e[0] + e[1]) + (e[2] + e[3])) + ((e[4] + e[5]) + (e[6] + e[7])));
+#pragma unroll
+            for (int m = 0; m < 8; m++) x[m] -= tw * v[m];
+        }
+        if (more) {
+            t = tn;
+#pragma unroll
+            for (int m = 0; m < 8; m++) v[m] = vn[m];
+        }
+        k = kn;
     }
     __syncthreads();
 }
 
 // Q = H_0 ... H_{n-1} applied to the identity (dorg2r)
-__device__ __forceinline__ void ps_formq(const double (&a)[8], double (&q)[8], int n, const double *tau, double *pd)
+__device__ __forceinline__ void ps_formq(const double *Vs, double (&q)[8], int n, const double *tau, double *pd)
 {
-    const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int c = threadIdx.x & 63, w = ps_wave();
 #pragma unroll
     for (int m = 0; m < 8; m++) q[m] = (8 * w + m == c && c < n) ? 1.0 : 0.0;
-    ps_apply_h(a, q, n, tau, pd, false);
+    ps_apply_h(Vs, q, n, tau, pd, false);
 }
 
 // x <- Rn^-1 x, Rn = D^-1 R unit upper triangular (R on and above the
@@ -417,7 +500,7 @@ __device__ __forceinline__ void ps_formq(const double (&a)[8], double (&q)[8], i
 // rows go to the blocks above through LDS (two alternating buffers).
 __device__ void ps_trsm(const double (&a)[8], double (&x)[8], int n, double *rowbuf)
 {
-    const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int c = threadIdx.x & 63, w = ps_wave();
     double di[8];
 #pragma unroll
     for (int m = 0; m < 8; m++) {
@@ -457,95 +540,123 @@ __device__ void ps_trsm(const double (&a)[8], double (&x)[8], int n, double *row
 }
 
 // Rn = D^-1 R (unit diagonal, zero below) from a[] into an LDS matrix, D
-// into Dv.  Returns through *bad (thread 0) a zero or non-finite D.
-__device__ __forceinline__ void ps_store_rn(double *S, double *Dv, const double (&a)[8], int n, int *bad)
+// into Dv, undoing the column scales of the QR input: column c of the
+// factored matrix was column col[c] of the input times sc[col[c]] (powers of
+// two), so D_c = R'_cc / sc[col[c]] and Rn_rc = Rn'_rc sc[col[r]] /
+// sc[col[c]] (sc = null: no scales).  Sets *bad = 2 for a zero or
+// non-finite D.
+__device__ __forceinline__ void ps_store_rn(double *S, double *Dv, const double (&a)[8], int n, int *bad,
+                                            const double *sc, const int *col)
 {
-    const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int c = threadIdx.x & 63, w = ps_wave();
+    const double scc = (sc && c < n) ? sc[col[c]] : 1.0;
 #pragma unroll
     for (int m = 0; m < 8; m++) {
         const int r = 8 * w + m;
-        const double d = ps_rl(a[m], r < 64 ? r : 63);
-        if (r < n && c < n) S[r + c * PS_LD] = (r < c) ? a[m] / d : (r == c ? 1.0 : 0.0);
+        const double d = ps_rl(a[m], r);
+        const double scr = (sc && r < n) ? sc[col[r]] : 1.0;
+        if (r < n && c < n) S[r + c * PS_LD] = (r < c) ? (a[m] / d) * (scr / scc) : (r == c ? 1.0 : 0.0);
         if (r < n && c == r) {
-            Dv[r] = d;
-            if (!isfinite(d) || d == 0.0) *bad = 2;
+            const double dt = d / scc;
+            Dv[r] = dt;
+            if (!isfinite(dt) || dt == 0.0) *bad = 2;
         }
     }
 }
 
-// Column pre-pivoting: perm = the columns of the LDS matrix S in order of
-// decreasing 2-norm, ties by index (the order of k_col_keys / k_col_sort),
-// the norms compared as log(max|x|) + log ||x / max|x|||.
-__device__ void ps_pivot(const double *S, int n, int *perm, double *keys, double *pd, int pivot)
+// Column pre-pivoting and equilibration of the LDS matrix S: perm = its
+// columns in order of decreasing 2-norm, ties by index (the order of
+// k_col_keys / k_col_sort; pivot = 0: the identity), and sc[c] = 2^-e with
+// e the binary exponent of max_r |S_rc| (1 for a zero column).  Norms are
+// compared as logarithms, each wave's partial sum of squares taken on its
+// entries scaled by a power of two, so nothing overflows and nothing is
+// divided.  A non-finite entry sets *bad = 1.
+__device__ void ps_pivot(const double *S, int n, int *perm, double *keys, double *sc, double *pd, int pivot,
+                         int *bad)
 {
-    const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (!pivot) {
-        if (threadIdx.x < n) perm[threadIdx.x] = threadIdx.x;
-        __syncthreads();
-        return;
-    }
-    double mx = 0.0;
-    double x[8];
+    const int c = threadIdx.x & 63, w = ps_wave();
+    double x[8], mx = 0.0;
+    bool fin = true;
 #pragma unroll
     for (int m = 0; m < 8; m++) {
         const int r = 8 * w + m;
         x[m] = (r < n && c < n) ? S[r + c * PS_LD] : 0.0;
         mx = fmax(mx, fabs(x[m]));
+        fin = fin && isfinite(x[m]);
     }
-    double s = 0.0;
-    if (mx > 0.0) {
+    const int e = (mx > 0.0 && isfinite(mx)) ? ilogb(mx) : 0;
+    double s2 = 0.0;
 #pragma unroll
-        for (int m = 0; m < 8; m++) {
-            const double t = x[m] / mx;
-            s += t * t;
-        }
+    for (int m = 0; m < 8; m++) {
+        const double t = ldexp(x[m], -e);
+        s2 += t * t;
     }
-    pd[w * 64 + c] = mx;
-    pd[512 + w * 64 + c] = s;
+    pd[w * 64 + c] = mx > 0.0 ? (double)e : -1e300;
+    pd[512 + w * 64 + c] = s2;
+    if (!fin) *bad = 1;
     __syncthreads();
     if (w == 0 && c < n) {
-        double M = 0.0;
+        double emax = -1e300;
 #pragma unroll
-        for (int u = 0; u < 8; u++) M = fmax(M, pd[u * 64 + c]);
-        double S2 = 0.0;
-        if (M > 0.0) {
+        for (int u = 0; u < 8; u++) emax = fmax(emax, pd[u * 64 + c]);
+        double key = -INFINITY, scale = 1.0;
+        if (emax > -1e299) {
+            const int em = (int)emax;
+            double S2 = 0.0;
 #pragma unroll
             for (int u = 0; u < 8; u++) {
-                const double q = pd[u * 64 + c] / M;
-                S2 += pd[512 + u * 64 + c] * q * q;
+                const double eu = pd[u * 64 + c];
+                if (eu > -1e299) S2 += ldexp(pd[512 + u * 64 + c], 2 * ((int)eu - em));
             }
+            key = 0.5 * log(S2) + em * 0.69314718055994530942;
+            if (isnan(key)) key = -INFINITY;
+            scale = ldexp(1.0, -em);
         }
-        double key = M > 0.0 ? log(M) + 0.5 * log(S2) : -INFINITY;
-        if (isnan(key)) key = -INFINITY;
         keys[c] = key;
+        sc[c] = scale;
     }
     __syncthreads();
     if (w == 0 && c < n) {
-        const double key = keys[c];
-        int rank = 0;
-        for (int k = 0; k < n; k++) {
-            const double kk = keys[k];
-            rank += (kk > key || (kk == key && k < c)) ? 1 : 0;
+        if (!pivot) {
+            perm[c] = c;
+        } else {
+            const double key = keys[c];
+            int rank = 0;
+            for (int k = 0; k < n; k++) {
+                const double kk = keys[k];
+                rank += (kk > key || (kk == key && k < c)) ? 1 : 0;
+            }
+            perm[rank] = c;
         }
-        perm[rank] = c;
     }
     __syncthreads();
 }
 
-// One UDV update of the chain from an LDS input M (n x n, left in place):
-// M P = Q R with the pre-pivoting order P; Q into Uout (LDS), D into Dv,
-// Rn = D^-1 R into Rs (LDS), the order into perm.  Returns sgn det Q; *bad
-// as in ps_qr / ps_store_rn.
+// One UDV update of the chain from an LDS input M (n x n): M P = Q R with
+// the pre-pivoting order P (QR of the equilibrated M P); Q into Uout (LDS,
+// may be M), D into Dv, Rn = D^-1 R into Rs (LDS, which holds the
+// reflectors meanwhile), the order into perm.  Returns sgn det Q; *bad as in ps_qr / ps_store_rn.
 __device__ int ps_update_qr(const double *M, double *Uout, double *Rs, double *Dv, int n, const PsLds &s,
                             int pivot, int *bad)
 {
-    ps_pivot(M, n, s.perm, s.keys, s.Cw, pivot);
+    ps_pivot(M, n, s.perm, s.keys, s.scl, s.Cw, pivot, bad);
     double a[8], q[8];
-    ps_to_regs(a, M, n, s.perm);
-    const int ntau = ps_qr(a, n, s.tau, s.pn, s.Cw, bad);
-    ps_formq(a, q, n, s.tau, s.Cw);
+    {
+        const int c = threadIdx.x & 63, w = ps_wave();
+        const int cc = c < n ? s.perm[c] : 0;
+        const double f = c < n ? s.scl[cc] : 0.0;
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const int r = 8 * w + m;
+            a[m] = (c < n && r < n) ? M[r + cc * PS_LD] * f : 0.0;
+        }
+    }
+    // the reflectors go to Rs until Rn replaces them
+    const int ntau = ps_qr(a, n, s.tau, s.Cw, s.rk, Rs, bad);
+    ps_formq(Rs, q, n, s.tau, s.Cw);
     ps_from_regs(Uout, q, n);
-    ps_store_rn(Rs, Dv, a, n, bad);
+    __syncthreads();
+    ps_store_rn(Rs, Dv, a, n, bad, s.scl, s.perm);
     __syncthreads();
     return (ntau & 1) ? -1 : 1;
 }
@@ -561,6 +672,11 @@ __device__ __forceinline__ void ps_split(double d, double &bi, double &sm)
 
 // the dynamic LDS of k_sweep_persist, seen by its phase functions
 extern __shared__ __align__(16) unsigned char ps_lds_raw[];
+
+// LDS matrix k (0..3) by arithmetic on the LDS base, so that the compiler
+// keeps the LDS address space (a pointer picked out of an array by a
+// run-time index would turn every access into a flat memory instruction)
+__device__ __forceinline__ double *ps_slot(int k) { return (double *)ps_lds_raw + k * PS_MAT; }
 
 // a pointer or an int known to be equal in every lane, as the compiler's
 // uniform value (the phase functions receive their arguments in vector
@@ -579,18 +695,18 @@ template <typename T> __device__ __forceinline__ T *ps_uni(T *p)
 // M P = Q R, U' = Q, D' = diag R, T' = (D'^-1 R) P^T T (the batched path's
 // stage_rmul); S[M] is the identity.  Slots 1, 2 carry U, T of S[j+1],
 // slot 0 the block product, slot 3 the QR input.  Returns the flag.
-__device__ __attribute__((noinline)) int ps_suffix(const PersistModel *pm_, int r_)
+__device__ __attribute__((noinline)) int ps_suffix(int r_)
 {
-    const PersistModel *pm = ps_uni(pm_);
     const int r = ps_uni(r_);
     const PsLds s = ps_carve(ps_lds_raw);
+    const PersistModel *pm = s.pm;
     const int tid = threadIdx.x;
-    const int n = pm->n, M = pm->M, R = pm->nrep;
+    const int n = ps_uni(pm->n), M = ps_uni(pm->M), R = ps_uni(pm->nrep);
     const long long nn = (long long)n * n;
     double *SUr = pm->SU + (long long)r * (M + 1) * nn, *STr = pm->ST + (long long)r * (M + 1) * nn;
     double *SDr = pm->SD + (long long)r * (M + 1) * n;
     int *Sqr = pm->Sq + (long long)r * (M + 1);
-    double *Us = s.slot[1], *Ts = s.slot[2], *Bs = s.slot[0], *Ms = s.slot[3];
+    double *Us = ps_slot(1), *Ts = ps_slot(2), *Bs = ps_slot(0), *Ms = ps_slot(3);
     ps_identity(Us, n);
     ps_identity(Ts, n);
     if (tid < n) s.DR[tid] = 1.0;
@@ -618,32 +734,58 @@ __device__ __attribute__((noinline)) int ps_suffix(const PersistModel *pm_, int 
     return f;
 }
 
+// Draws i0 .. i1-1 of the replica's stream (xoshiro256**, src/rng.c:23-37)
+// into out, by one whole wave on wave-uniform values (scalar instructions),
+// lane 0 storing; the state lives in the LDS sweep state.
+__device__ __forceinline__ void ps_draws(PsState *st, double *out, int i0, int i1)
+{
+    uint64_t x[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint64_t v = st->rng[q];
+        const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+        const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+        x[q] = ((uint64_t)hi << 32) | lo;
+    }
+    for (int i = i0; i < i1; i++) {
+        const double d = xoshiro_double(x);
+        if ((threadIdx.x & 63) == 0) out[i] = d;
+    }
+    if ((threadIdx.x & 63) == 0)
+        for (int q = 0; q < 4; q++) st->rng[q] = x[q];
+}
+
 // Slice l of replica r: Metropolis over its n sites in windows of W (G in
 // slot st->g), the slice's spins and B diagonal written back, then the
 // block product P <- B_l P in slot st->p and the wrap of G.
 template <int W>
-__device__ __attribute__((noinline)) void ps_slice(const PersistModel *pm_, int r_, int l_)
+__device__ __forceinline__ void ps_slice(const PersistModel *pm, int r, int l)
 {
     constexpr int RPL = W * W / 64;
-    const PersistModel *pm = ps_uni(pm_);
-    const int r = ps_uni(r_), l = ps_uni(l_);
     const PsLds s = ps_carve(ps_lds_raw);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = ps_wave();
     const int lc = lane & 15, lr = lane >> 4;
-    const int n = pm->n, L = pm->L, stab = pm->stab;
-    double *Gs = s.slot[ps_uni(s.st->g)], *Ks = s.slot[ps_uni(s.st->k)], *Kis = s.slot[ps_uni(s.st->ki)],
-           *Ps = s.slot[ps_uni(s.st->p)];
+    const int n = ps_uni(pm->n), L = ps_uni(pm->L), stab = ps_uni(pm->stab);
+    double *Gs = ps_slot(ps_uni(s.st->g)), *Ks = ps_slot(ps_uni(s.st->k)), *Kis = ps_slot(ps_uni(s.st->ki)),
+           *Ps = ps_slot(ps_uni(s.st->p));
     signed char *fld = pm->field + ((long long)r * L + l) * n;
-    if (tid < n) {
-        s.fs[tid] = fld[tid];
-        s.flip[tid] = 0;
-    }
-    if (tid == 0) {
-        uint64_t st[4];
-        for (int q = 0; q < 4; q++) st[q] = s.st->rng[q];
-        for (int i = 0; i < n; i++) s.draws[i] = xoshiro_double(st);
-        for (int q = 0; q < 4; q++) s.st->rng[q] = st[q];
-    }
+    // the site constants as uniform values, read once
+    const double nu0 = pm->nu[0], nu1 = pm->nu[1], nd0 = pm->nd[0], nd1 = pm->nd[1];
+    const double e0 = pm->e[0], e1 = pm->e[1], ei0 = pm->einv[0], ei1 = pm->einv[1];
+    // this slice's spins were fetched during the previous slice (slice 0:
+    // here); the next slice's are fetched now and stored at the end
+    signed char *fs = s.fs + (l & 1) * PS_NP, *fsn = s.fs + ((l & 1) ^ 1) * PS_NP;
+    if (l == 0 && tid < n) fs[tid] = fld[tid];
+    signed char fnext = 0;
+    if (l + 1 < L && tid < n) fnext = fld[n + tid];
+    if (tid < n) s.flip[tid] = 0;
+    // The slice's n draws: taken ahead by wave 1 during the previous slice's
+    // proposals, except for the first slice of a block, whose draws are
+    // taken here (so a sweep that fails at a rebuild has taken no draw of
+    // the slices after it, as on the batched path).
+    double *dcur = s.draws + (l & 1) * PS_NP, *dnext = s.draws + ((l & 1) ^ 1) * PS_NP;
+    if (l % stab == 0 && wave == 1) ps_draws(s.st, dcur, 0, n);
+    const bool ahead = l + 1 < L && (l + 1) % stab != 0;
     __syncthreads();
     int negflips = 0, naccept = 0;
     for (int i0 = 0; i0 < n; i0 += W) {
@@ -661,10 +803,10 @@ __device__ __attribute__((noinline)) void ps_slice(const PersistModel *pm_, int 
             }
             double dr_l = 0.0, nu_l = 0.0, nd_l = 0.0;
             if (lane < wn) {
-                const int sp = s.fs[i0 + lane] > 0;
-                dr_l = s.draws[i0 + lane];
-                nu_l = pm->nu[sp];
-                nd_l = pm->nd[sp];
+                const bool sp = fs[i0 + lane] > 0;
+                dr_l = dcur[i0 + lane];
+                nu_l = sp ? nu1 : nu0;
+                nd_l = sp ? nd1 : nd0;
             }
             double *rlog = pm->rlog;
             unsigned mask = 0;
@@ -678,16 +820,25 @@ __device__ __attribute__((noinline)) void ps_slice(const PersistModel *pm_, int 
                     const double Rd = 1.0 + gii * Nd;
                     const double Rt = Ru * Rd;
                     if (rlog && lane == 0) rlog[(long long)r * n + i] = Rt;
+                    // N / R_up of an accept, started before the decision
+                    const double fac = Nu / Ru;
                     if (ps_rl(dr_l, w) < fabs(Rt)) {
-                        const double fac = Nu / Ru;
+                        // column w of the block to every lane of each 16-lane
+                        // row (DPP row broadcast) gives c; row w, held by
+                        // 16-lane row w / RPL, to all four rows (row swaps)
+                        // gives v; G~ -= c v^T in registers.  The window
+                        // pairs also go to LDS for the pair formation after
+                        // the next barrier.
+                        double cw[RPL];
+#pragma unroll
+                        for (int k = 0; k < RPL; k++)
+                            cw[k] = fac * ((h * RPL + k == w ? 1.0 : 0.0) - row_bcast(bw[k], w));
+                        const double vc = ps_xrow(bw[w % RPL], w / RPL);
                         if (c == w) {
 #pragma unroll
-                            for (int k = 0; k < RPL; k++) {
-                                const int rr = h * RPL + k;
-                                s.cwin[w * W + rr] = fac * ((rr == w ? 1.0 : 0.0) - bw[k]);
-                            }
+                            for (int k = 0; k < RPL; k++) s.cwin[w * W + h * RPL + k] = cw[k];
                         }
-                        if (h == w / RPL) s.vwin[w * W + c] = bw[w % RPL];
+                        if (h == w / RPL) s.vwin[w * W + c] = vc;
                         if (lane == 0) {
                             s.facw[w] = fac;
                             s.flip[i] = 1;
@@ -695,24 +846,24 @@ __device__ __attribute__((noinline)) void ps_slice(const PersistModel *pm_, int 
                         mask |= 1u << w;
                         naccept++;
                         if (Rt < 0.0) negflips ^= 1;
-                        __builtin_amdgcn_s_waitcnt(0xc07f);
-                        __builtin_amdgcn_wave_barrier();
-                        const double vc = s.vwin[w * W + c];
 #pragma unroll
-                        for (int k = 0; k < RPL; k++) bw[k] -= s.cwin[w * W + h * RPL + k] * vc;
-                        __builtin_amdgcn_s_waitcnt(0xc07f);
-                        __builtin_amdgcn_wave_barrier();
+                        for (int k = 0; k < RPL; k++) bw[k] -= cw[k] * vc;
                     }
                 }
             }
             if (lane == 0) *mword = mask;
+        } else if (wave == 1 && ahead) {
+            ps_draws(s.st, dnext, i0, i0 + wn);
         }
         __syncthreads();
         const unsigned mask = __builtin_amdgcn_readfirstlane(*mword);
         const int p = __builtin_popcount(mask);
         if (p == 0) continue;
         // the window's pairs over all sites: thread j < n forms c_q(j) from
-        // column i0 + w of G, thread n + j forms v_q(j) from row i0 + w
+        // column i0 + w of G, thread n + j forms v_q(j) from row i0 + w, the
+        // accepts in order, each final pair entry removed at once from the
+        // later window sites (right-looking: the coefficients of one accept
+        // are one row of the window pairs, read together)
         {
             const bool vside = tid >= n;
             const int jj = vside ? tid - n : tid;
@@ -727,15 +878,13 @@ __device__ __attribute__((noinline)) void ps_slice(const PersistModel *pm_, int 
                     gc[w] = (w < wn) ? x : 0.0;
                 }
 #pragma unroll
-                for (int w = 0; w < W; w++) {
-                    if ((mask >> w) & 1u) {
-                        double x = gc[w];
+                for (int u = 0; u < W; u++) {
+                    if ((mask >> u) & 1u) {
+                        gc[u] = vside ? gc[u] : s.facw[u] * ((jj == i0 + u ? 1.0 : 0.0) - gc[u]);
+                        const int q = __builtin_popcount(mask & ((1u << u) - 1u));
+                        dst[q * PS_NP + jj] = gc[u];
 #pragma unroll
-                        for (int u = 0; u < w; u++)
-                            if ((mask >> u) & 1u) x -= gc[u] * coef[u * W + w];
-                        gc[w] = vside ? x : s.facw[w] * ((jj == i0 + w ? 1.0 : 0.0) - x);
-                        const int q = __builtin_popcount(mask & ((1u << w) - 1u));
-                        dst[q * PS_NP + jj] = gc[w];
+                        for (int w = u + 1; w < W; w++) gc[w] -= gc[u] * coef[u * W + w];
                     }
                 }
             }
@@ -771,9 +920,10 @@ __device__ __attribute__((noinline)) void ps_slice(const PersistModel *pm_, int 
     ps_tick(s, PT_METRO);
     // the slice's fields are final: spins, the diagonal of B_l
     if (tid < n) {
-        const signed char sp = s.flip[tid] ? (signed char)-s.fs[tid] : s.fs[tid];
+        const signed char sp = s.flip[tid] ? (signed char)-fs[tid] : fs[tid];
         fld[tid] = sp;
-        const double dv = pm->e[sp > 0], di = pm->einv[sp > 0];
+        if (l + 1 < L) fsn[tid] = fnext;
+        const double dv = sp > 0 ? e1 : e0, di = sp > 0 ? ei1 : ei0;
         const long long o = ((long long)r * L + l) * n + tid;
         pm->dvec[o] = dv;
         pm->dinv[o] = di;
@@ -811,6 +961,16 @@ __device__ __attribute__((noinline)) void ps_slice(const PersistModel *pm_, int 
     ps_tick(s, PT_WRAP);
 }
 
+// The slices l0 .. l1-1 of one block (one call per block: the slice code
+// lives in this function's registers, and its calls are few)
+template <int W>
+__device__ __attribute__((noinline)) void ps_block(int r_, int l0_, int l1_)
+{
+    const PersistModel *pm = ps_carve(ps_lds_raw).pm;
+    const int r = ps_uni(r_), l0 = ps_uni(l0_), l1 = ps_uni(l1_);
+    for (int l = l0; l < l1; l++) ps_slice<W>(pm, r, l);
+}
+
 // The block boundary after slice l (tau = l + 1): the block product goes to
 // the cache, the left UDV is updated (lmul with a pre-pivoted QR, the left
 // UDV being the identity before the first block), and G is rebuilt from it
@@ -820,16 +980,17 @@ __device__ __attribute__((noinline)) void ps_slice(const PersistModel *pm_, int 
 // then compared with the wrapped G (drift sample, guard, finiteness), the
 // sign reset to +1 and G replaced.  Returns the flag; on a QR failure G is
 // the wrapped one.
-__device__ __attribute__((noinline)) int ps_boundary(const PersistModel *pm_, int r_, int l_)
+__device__ __attribute__((noinline)) int ps_boundary(int r_, int l_)
 {
-    const PersistModel *pm = ps_uni(pm_);
     const int r = ps_uni(r_), l = ps_uni(l_);
     const PsLds s = ps_carve(ps_lds_raw);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int n = pm->n, L = pm->L, stab = pm->stab, M = pm->M, R = pm->nrep;
+    const PersistModel *pm = s.pm;
+    const int tid = threadIdx.x, lane = tid & 63, wave = ps_wave();
+    const int n = ps_uni(pm->n), L = ps_uni(pm->L), stab = ps_uni(pm->stab), M = ps_uni(pm->M),
+              R = ps_uni(pm->nrep);
     const long long nn = (long long)n * n;
     const int gi = ps_uni(s.st->g), ki = ps_uni(s.st->k), kii = ps_uni(s.st->ki), pi = ps_uni(s.st->p);
-    double *Gs = s.slot[gi], *Us = s.slot[ki], *Ts = s.slot[kii], *Ps = s.slot[pi];
+    double *Gs = ps_slot(gi), *Us = ps_slot(ki), *Ts = ps_slot(kii), *Ps = ps_slot(pi);
     double *LUr = pm->LU + r * nn, *LTr = pm->LT + r * nn, *LDr = pm->LD + (long long)r * n;
     const int tau = l + 1, jb = l / stab;
     ps_store(pm->BB + ((long long)jb * R + r) * nn, Ps, n);
@@ -890,7 +1051,9 @@ __device__ __attribute__((noinline)) int ps_boundary(const PersistModel *pm_, in
     {
         double a[8], x[8];
         ps_to_regs(a, Ms, n, nullptr);
-        (void)ps_qr(a, n, s.tau, s.pn, s.Cw, s.flag);
+        __syncthreads();
+        // the reflectors replace M in its slot
+        (void)ps_qr(a, n, s.tau, s.Cw, s.rk, Ms, s.flag);
 #pragma unroll
         for (int m = 0; m < 8; m++) {
             const int rr = 8 * wave + m;
@@ -908,7 +1071,7 @@ __device__ __attribute__((noinline)) int ps_boundary(const PersistModel *pm_, in
             const int rr = 8 * wave + m;
             x[m] = (rr < n && lane < n) ? s.lbi[rr] * Us[lane + rr * PS_LD] : 0.0;
         }
-        ps_apply_h(a, x, n, s.tau, s.Cw, true);
+        ps_apply_h(Ms, x, n, s.tau, s.Cw, true);
 #pragma unroll
         for (int m = 0; m < 8; m++) {
             const int rr = 8 * wave + m;
@@ -1016,8 +1179,14 @@ __global__ __launch_bounds__(PS_THREADS) void k_sweep_persist(const PersistModel
         s.st->timing = (timers && blockIdx.x == 0) ? 1 : 0;
         s.st->t0 = wall_clock64();
     }
+    // the model's descriptor into LDS, where the phases read it
+    {
+        const unsigned long long *src = (const unsigned long long *)pm;
+        unsigned long long *dst = (unsigned long long *)s.pm;
+        for (int q = tid; q < (int)(sizeof(PersistModel) / 8); q += PS_THREADS) dst[q] = src[q];
+    }
     __syncthreads();
-    int f = ps_suffix(pm, r);
+    int f = ps_suffix(r);
     if (f) {
         if (tid == 0) {
             pm->status[r] = 1;
@@ -1025,21 +1194,20 @@ __global__ __launch_bounds__(PS_THREADS) void k_sweep_persist(const PersistModel
         }
         return;
     }
-    ps_load(s.slot[0], pm->G + r * nn, n);
-    ps_load(s.slot[1], pm->expK, n);
-    ps_load(s.slot[2], pm->expKinv, n);
+    ps_load(ps_slot(0), pm->G + r * nn, n);
+    ps_load(ps_slot(1), pm->expK, n);
+    ps_load(ps_slot(2), pm->expKinv, n);
     __syncthreads();
-    for (int l = 0; l < L; l++) {
-        ps_slice<W>(pm, r, l);
-        if ((l + 1) % stab == 0 || l + 1 == L) {
-            f = ps_boundary(pm, r, l);
-            if (f) break;
-        }
+    for (int l0 = 0; l0 < L; l0 += stab) {
+        const int l1 = min(L, l0 + stab);
+        ps_block<W>(r, l0, l1);
+        f = ps_boundary(r, l1 - 1);
+        if (f) break;
     }
     // f: 1, 2 a failed QR (G stays the wrapped one, the sign is not reset);
     // 3, 4 a rebuilt G that is not finite or drifted beyond the guard (kept,
     // as the batched copy keeps it)
-    ps_store(pm->G + r * nn, s.slot[s.st->g], n);
+    ps_store(pm->G + r * nn, ps_slot(ps_uni(s.st->g)), n);
     if (tid == 0) {
         if (f) {
             pm->status[r] = 1;

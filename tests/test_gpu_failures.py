@@ -120,7 +120,7 @@ def test_boundary_rebuild_matches_scratch_16x16_beta8(eng_mod):
 
 
 @pytest.mark.parametrize("order,nsw", [("forward", 3), ("alternating", 4)])
-def test_drift_from_scratch(eng_mod, orc, order, nsw):
+def test_drift_from_scratch(eng_mod, orc, monkeypatch, order, nsw):
     # DQMC_GPU_DRIFT_SCRATCH (the reference's definition) and
     # DQMC_GPU_DRIFT_REBUILD record the same trajectory (G itself is never
     # touched by the drift samples), one sample per rebuild, and values
@@ -131,6 +131,9 @@ def test_drift_from_scratch(eng_mod, orc, order, nsw):
     # from G rebuilt through the unpivoted chain and the LU-based inverse,
     # 1e-11 .. 1e-10 from the exact G here, and that error is most of its
     # drift (mean samples 5e-9 .. 1e-7 against the engine's 2e-10 .. 3e-10)
+    # Both modes on the batched path (the persistent kernel takes the
+    # rebuild mode at this size but not the from-scratch one).
+    monkeypatch.setenv("DQMC_GPU_PERSIST", "0")
     seeds = [1, 2, 3]
     kind, Lx, Ly, L, stab = "square", 8, 8, 40, 10
     runs = {}
