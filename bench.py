@@ -17,7 +17,11 @@ reported with the guard out of reach); the compiled reference on the host
 cores as the CPU baseline.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--fp64-only]
-                    [--calibrate-copy] [--groups G] [--delay K]
+                    [--calibrate-copy] [--groups G] [--delay K] [--no-small] [--only-small]
+
+Beside the metric's line: the 8x8 workload (beta=4, 64 replicas) on the
+persistent sweep kernel and on the batched path, and the uneven batch of 20
+models (256 replicas) in one context.
 """
 import argparse
 import json
@@ -179,6 +183,103 @@ def measure_gbs(prof, nrep_launch=NREP):
             "avg_launch_ms": round(t * 1e3, 4), "bytes_per_launch": byts}
 
 
+def work_per_replica_sweep(n, L, stab, acc):
+    """The algorithmic flops of one forward replica-sweep (the reference's
+    dense counts: wrap 4n^3 L, rank-k flushes 2a n^3 L, the B-block chains
+    2n^3 (s-1)(2M-1); FP64 QR class (2M-1)(20/3)n^3 + (M-1)(26/3)n^3 +
+    M (29/3)n^3)."""
+    M = -(-L // stab)
+    lo = 4 * n ** 3 * L + 2 * acc * n ** 3 * L + 2 * n ** 3 * (stab - 1) * (2 * M - 1)
+    hi = (2 * M - 1) * (20 / 3) * n ** 3 + (M - 1) * (26 / 3) * n ** 3 + M * (29 / 3) * n ** 3
+    return lo + hi
+
+
+def small_lattice_lines(dq, steps, warmup, which="all"):
+    """The 8x8 workload (U=4, dtau=0.1, beta=4: 40 slices, stab=10, 64
+    replicas with seeds 1..64, Szz/Sperp at all 64 q after every sweep) on
+    the persistent sweep kernel (one workgroup per replica, G in LDS) and on
+    the batched path (DQMC_GPU_PERSIST=0); and the uneven batch of 20 models
+    (4x4 .. 16x16 at beta 2, 4, 8, 12, stab=10, 256 replicas) in one
+    context, the lattices of at most 64 sites in one persistent launch."""
+    out = {}
+    n, L, stab, R = 64, 40, 10, 64
+    runs = {}
+    if which == "c4":
+        return dict(uneven_batch_line(dq, steps))
+    for mode in (("1",) if which == "c2" else ("1", "0")):
+        os.environ["DQMC_GPU_PERSIST"] = mode
+        eng = dq.Engine(dq.FP64, autotune=True)
+        mid = eng.add_model("square", 8, 8, U, DTAU, L, stab, delay_k=16)
+        eng.add_replicas(mid, list(range(1, R + 1)))
+        eng.set_measure(mid, nbin=1, q="all")
+        eng.init()
+        eng.sweep(warmup)
+        eng.sweep(steps, 0)
+        ms = eng.elapsed_ms()
+        st = eng.status()
+        acc = float(st["accepted"].sum()) / max(1.0, float(st["attempts"].sum()))
+        prof = eng.profile(2) if mode == "1" else None
+        eng.close()
+        runs[mode] = (ms / steps, R * steps / (ms / 1e3), acc, prof)
+    os.environ.pop("DQMC_GPU_PERSIST", None)
+    ms_p, v_p, acc, prof = runs["1"]
+    ms_b, v_b, _, _ = runs.get("0", (float("nan"), float("nan"), None, None))
+    flops = work_per_replica_sweep(n, L, stab, acc)
+    achieved = v_p * flops
+    out["c2_8x8_beta4"] = {
+        "workload": "8x8 periodic square, U=4, dtau=0.1, beta=4 (40 slices), stab=10, 64 replicas, Szz/Sperp at "
+                    "all 64 q after every sweep",
+        "persistent": {"value": round(v_p, 1), "ms_per_step": round(ms_p, 4),
+                       "phases_ms_per_sweep": {k: round(prof[k][0] / 2, 4) for k in
+                                               ("stack", "metro", "wrap", "lmul", "inv_one_plus", "measure")}},
+        "batched": ({"value": round(v_b, 1), "ms_per_step": round(ms_b, 4)} if "0" in runs else None),
+        "persistent_over_batched": round(v_p / v_b, 3) if "0" in runs else None,
+        "unit": "replica-sweeps/s",
+        "roofline": {"bound": "mfma", "flops_per_replica_sweep": round(flops), "achieved": round(achieved / 1e12, 3),
+                     "peak_chip": round(P64 / 1e12, 1), "frac_chip": round(achieved / P64, 4),
+                     "peak_of_cus_used": round(P64 * R / 256 / 1e12, 2),
+                     "frac_of_cus_used": round(achieved / (P64 * R / 256), 4), "unit": "TFLOP/s",
+                     "note": "one workgroup per replica: 64 of 256 CUs work; latency-bound (sequential proposals "
+                             "and Householder column steps)"}}
+    print(f"# c2 8x8: persistent {v_p:.1f}, batched {v_b:.1f} replica-sweeps/s", file=sys.stderr, flush=True)
+    if which == "c2":
+        return out
+    out.update(uneven_batch_line(dq, steps))
+    return out
+
+
+def uneven_batch_line(dq, steps):
+    """C4: 20 models (4x4 .. 16x16 at beta 2, 4, 8, 12), 256 replicas, one
+    context."""
+    out = {}
+    eng = dq.Engine(dq.FP64, autotune=True)
+    total, k = 256, 0
+    for Lx in (4, 6, 8, 12, 16):
+        for beta in (2.0, 4.0, 8.0, 12.0):
+            Lb = int(round(beta / DTAU))
+            nrep = total // 20 + (1 if k < total % 20 else 0)
+            mid = eng.add_model("square", Lx, Lx, U, DTAU, Lb, 10, delay_k=0)
+            eng.add_replicas(mid, [dq.replica_seed(1, k, r) for r in range(nrep)])
+            eng.set_measure(mid, nbin=1, q="all")
+            k += 1
+    eng.init()
+    eng.sweep(1)
+    eng.sweep(max(2, steps // 4), 0)
+    ms = eng.elapsed_ms() / max(2, steps // 4)
+    st = eng.status()
+    eng.close()
+    if (st["status"] != 0).any():
+        raise RuntimeError("C4 replica failure")
+    out["c4_uneven_256"] = {
+        "workload": "20 models: 4x4, 6x6, 8x8, 12x12, 16x16 square, each at beta 2, 4, 8, 12 (dtau=0.1), U=4, "
+                    "stab=10, 256 replicas, Szz/Sperp at all q; 4x4 .. 8x8 in one persistent launch per sweep, "
+                    "12x12 and 16x16 on the batched path",
+        "value": round(256 / (ms / 1e3), 1), "ms_per_step": round(ms, 3), "unit": "replica-sweeps/s"}
+    print(f"# c4: {256 / (ms / 1e3):.1f} replica-sweeps/s, {ms:.2f} ms per sweep of all 256", file=sys.stderr,
+          flush=True)
+    return out
+
+
 def cpu_model():
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True, check=True).stdout
@@ -254,6 +355,9 @@ def main():
     ap.add_argument("--groups", type=int, default=1, help="replica groups swept concurrently")
     ap.add_argument("--delay", type=int, default=DELAY, help="delayed-update window (sites)")
     ap.add_argument("--fp64-only", action="store_true", help="skip the mixed, drift and alternating runs")
+    ap.add_argument("--no-small", action="store_true", help="skip the 8x8 and uneven-batch lines")
+    ap.add_argument("--only-small", choices=("all", "c2", "c4"), default=None,
+                    help="print only the 8x8 (c2) and/or uneven-batch (c4) lines (for profiling the persistent kernel)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the engine's default launch shape (DQMC_GPU_METRO_SPLIT sets the segments)")
     a = ap.parse_args()
@@ -263,6 +367,9 @@ def main():
         sys.exit(2)
     import dqmc_gpu as dq
 
+    if a.only_small:
+        print(json.dumps(small_lattice_lines(dq, a.steps, a.warmup, a.only_small)))
+        return
     if a.calibrate_copy:
         calibrate_copy()
     main_run = run_engine(dq, dq.FP64, a.steps, a.warmup, groups=a.groups, delay=a.delay,
@@ -297,6 +404,8 @@ def main():
                            "drift_max_vs_rebuilt": r["drift_max"], "drift_mean_vs_rebuilt": r["drift_mean"]}
             print(f"# {name}: {g['failed']}/{NREP} replicas fail at drift_guard=1e-6; unguarded "
                   f"{r['value']:.1f} replica-sweeps/s, drift max {r['drift_max']:.3e}", file=sys.stderr, flush=True)
+    if not a.no_small:
+        extra.update(small_lattice_lines(dq, a.steps, a.warmup))
     cpu = None if a.no_cpu else cpu_baseline()
     n = LX * LY
     acc = main_run["acc"]
