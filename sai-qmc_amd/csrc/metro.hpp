@@ -25,7 +25,8 @@
 // The ratio is R = (1 + (1 - G~_ii) N_up)(1 + G~_ii N_dn)
 // (src/green.c:401-416), always evaluated in FP64, accepted iff
 // rng_double() < |R| (src/dqmc.c:668).  Every proposal consumes one draw of
-// the replica's xoshiro256** stream; the slice's n draws are taken up front.
+// the replica's xoshiro256** stream, in site order: a window's draws are
+// taken while the window before it is decided.
 #pragma once
 #include "common.hpp"
 
@@ -234,12 +235,15 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
         fs[j] = fld[j];
         flip[j] = 0;
     }
-    if (j == 0) {
-        uint64_t st[4];
-        uint64_t *rs = a.rng + 4 * (long long)b;
+    // The draws, one per proposal in site order: the first window's here,
+    // each later window's by thread 64 (wave 1) while wave 0 decides the
+    // window before it.
+    const bool rng_thread = j == 64;
+    uint64_t st[4] = {0, 0, 0, 0};
+    uint64_t *rs = a.rng + 4 * (long long)b;
+    if (rng_thread) {
         for (int q = 0; q < 4; q++) st[q] = rs[q];
-        for (int i = a.s0; i < a.s1; i++) draws[i] = xoshiro_double(st);
-        for (int q = 0; q < 4; q++) rs[q] = st[q];
+        for (int i = a.s0; i < min(a.s1, a.s0 + W); i++) draws[i] = xoshiro_double(st);
     }
     __syncthreads();
 
@@ -334,6 +338,8 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
                 }
             }
             if (lane == 0) s_mask = mask;
+        } else if (rng_thread) {
+            for (int i = i0 + W; i < min(a.s1, i0 + 2 * W); i++) draws[i] = xoshiro_double(st);
         }
         __syncthreads();
 
@@ -391,6 +397,8 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
             a.dinv64[o] = a.einv64[s > 0];
         }
     }
+    if (rng_thread)
+        for (int q = 0; q < 4; q++) rs[q] = st[q];
     if (j == 0) {
         a.pcount[b] = P;
         a.acc[b] += (unsigned long long)naccept;
