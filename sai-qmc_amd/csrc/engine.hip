@@ -1529,7 +1529,12 @@ static int kron(Model &m, Work &w, int rows, int f, const double *in, double *ou
     a.post = post;
     a.oth = oth;
     a.status = w.status;
-    hipLaunchKernelGGL(k_kron16, dim3(KRON_L * m.nrep), dim3(256), 0, w.s, a);
+    if (rows)
+        hipLaunchKernelGGL(k_kron16<KRON_ROWS>, dim3(KRON_L * m.nrep), dim3(256), 0, w.s, a);
+    else if (pending)
+        hipLaunchKernelGGL(k_kron16<KRON_COLS_PENDING>, dim3(KRON_L * m.nrep), dim3(256), 0, w.s, a);
+    else
+        hipLaunchKernelGGL(k_kron16<KRON_COLS>, dim3(KRON_L * m.nrep), dim3(256), 0, w.s, a);
     return launch_check("k_kron16");
 }
 

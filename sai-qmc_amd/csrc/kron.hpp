@@ -36,7 +36,8 @@
 #define KRON_LD 272 // LDS doubles per vector: 16 grid rows of 17 (one pad)
 
 struct KronArgs {
-    int batch, rows;     // rows = 0: transform the columns of X; 1: its rows
+    int batch, rows;     // rows = 0: transform the columns of X; 1: its rows (the launcher
+                         // picks the kernel mode from it)
     const double *in;    // n x n per replica, column-major
     long long sIn;
     double *out;         // may not alias `in`
@@ -54,6 +55,13 @@ struct KronArgs {
 
 __device__ __forceinline__ int kron_lds(int e) { return (e & 15) + 17 * (e >> 4); }
 
+// MODE: KRON_COLS, KRON_ROWS, or KRON_COLS_PENDING (a column pass that
+// applies the pending update first); one instantiation per mode, so the
+// plain passes carry none of the pending update's registers
+#define KRON_COLS 0
+#define KRON_ROWS 1
+#define KRON_COLS_PENDING 2
+template <int MODE>
 __global__ __launch_bounds__(256) void k_kron16(KronArgs a)
 {
     const int logical = xcd_remap(blockIdx.x, gridDim.x);
@@ -68,31 +76,38 @@ __global__ __launch_bounds__(256) void k_kron16(KronArgs a)
 
     // 1. stage: vector t = column (row) v0 + t, entry e along it
     double x[KRON_L];
-    if (!a.rows) {
-        // thread = entry e = tid of 16 columns: 64 consecutive doubles per wave
+    if (MODE != KRON_ROWS) {
+        // thread = entry e = tid of 16 columns: 64 consecutive doubles per
+        // wave.  The pending rank-P update of these 16 columns, D[e][t] =
+        // sum_q C[q][e] V[q][v0 + t], is the longer dependence chain (its
+        // loads, the MFMAs, the exchange through Z), so its first sixteen
+        // q's are loaded ahead of the columns themselves; wave w takes rows
+        // 64w .. 64w + 63 (four 16-row tiles), q four at a time in
+        // increasing order, and D goes to Z in the staging layout
+        const int P = (MODE == KRON_COLS_PENDING && a.fc) ? a.pcount[b] : 0;
+        const double *C = a.fc + (long long)b * a.sP, *V = a.fv + (long long)b * a.sP;
+        const int e0 = 64 * wave;
+        double av[4][4], bv[4];
+        // clamped (always valid) rows, zeroed past P after the loads
+        auto load_q = [&](int q0) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const long long qr = (long long)min(q0 + 4 * u + lr, P - 1) * KRON_N;
+                bv[u] = V[qr + v0 + lc];
+#pragma unroll
+                for (int ti = 0; ti < 4; ti++) av[u][ti] = C[qr + e0 + 16 * ti + lc];
+            }
+        };
+        if (P > 0) load_q(0);
 #pragma unroll
         for (int t = 0; t < KRON_L; t++) x[t] = in[tid + (long long)(v0 + t) * KRON_N];
-        // the pending rank-P update of these 16 columns, D[e][t] =
-        // sum_q C[q][e] V[q][v0 + t], on FP64 MFMA: wave w takes rows
-        // 64w .. 64w + 63 (four 16-row tiles), q four at a time in
-        // increasing order; D goes to Z in the staging layout
-        const int P = a.fc ? a.pcount[b] : 0;
+        const double pe_raw = a.pre.raw(b, tid);
         if (P > 0) {
-            const double *C = a.fc + (long long)b * a.sP, *V = a.fv + (long long)b * a.sP;
-            const int e0 = 64 * wave;
             f64x4 acc[4];
 #pragma unroll
             for (int ti = 0; ti < 4; ti++) acc[ti] = f64x4{0, 0, 0, 0};
             for (int q0 = 0; q0 < P; q0 += 16) {
-                double av[4][4], bv[4];
-                // clamped (always valid) rows, zeroed past P after the loads
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const long long qr = (long long)min(q0 + 4 * u + lr, P - 1) * KRON_N;
-                    bv[u] = V[qr + v0 + lc];
-#pragma unroll
-                    for (int ti = 0; ti < 4; ti++) av[u][ti] = C[qr + e0 + 16 * ti + lc];
-                }
+                if (q0 > 0) load_q(q0);
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const double b_ = q0 + 4 * u + lr < P ? bv[u] : 0.0;
@@ -109,7 +124,7 @@ __global__ __launch_bounds__(256) void k_kron16(KronArgs a)
 #pragma unroll
             for (int t = 0; t < KRON_L; t++) x[t] -= Z[t * KRON_LD + kron_lds(tid)];
         }
-        const double pe = a.pre.at(b, tid);
+        const double pe = a.pre.inv ? 1.0 / pe_raw : pe_raw;
 #pragma unroll
         for (int t = 0; t < KRON_L; t++) Z[t * KRON_LD + kron_lds(tid)] = x[t] * pe * a.oth.at(b, v0 + t);
     } else {
@@ -153,7 +168,7 @@ __global__ __launch_bounds__(256) void k_kron16(KronArgs a)
     __syncthreads();
 
     // 3. store, post-scaled along the transformed index
-    if (!a.rows) {
+    if (MODE != KRON_ROWS) {
         const double po = a.post.at(b, tid);
 #pragma unroll
         for (int t = 0; t < KRON_L; t++) out[tid + (long long)(v0 + t) * KRON_N] = Z[t * KRON_LD + kron_lds(tid)] * po;

@@ -469,21 +469,20 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
     __syncthreads();
 }
 
-// T of the compact WY form Q = I - V T V^T of the panel.  For Householder
-// reflectors tau_i = 2 / v_i^T v_i, which gives T^-1 = diag(1/tau) +
-// triu(V^T V, 1) (reflectors with tau = 0 are the identity: their row and
-// column of T vanish).  V^T V on MFMA: the QR_NT^2 16x16 tiles are spread
-// over the waves, each tile's k range split over QR_WAVES / QR_NT^2 waves
-// whose partial sums meet in Y; then lane r of wave 0 builds row r of T
-// in column order, so the dependent chain is QR_NB steps long.  Y holds
-// QR_WAVES / QR_NT^2 blocks of QR_NB x QR_NB.
+// T of the compact WY form Q = I - V T V^T of the panel (reflectors with
+// tau = 0 are the identity: their row and column of T vanish).  Y = V^T V
+// on MFMA: the QR_NT^2 16x16 tiles are spread over the waves, each tile's
+// k range split over QR_WAVES / QR_NT^2 waves whose partial sums meet in
+// Y; then wave 0 builds the two diagonal 16 x 16 blocks of T at once (a
+// dependent chain of 16 column steps) and the off-diagonal block from
+// them by two MFMA products.  Y holds QR_WAVES / QR_NT^2 blocks of
+// QR_NB x QR_NB.
 __device__ void qr_tfactor(const double *P, int ld, int nb, int m, double *Tl, double *Y)
 {
     constexpr int KS = QR_WAVES / (QR_NT * QR_NT); // waves per tile
     static_assert(KS >= 2, "two partial sums at least");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lc = lane & 15, lr = lane >> 4;
-    double *Ts = Y + QR_NB * QR_NB; // columns of T, in the second block once Y is summed
     {
         const int tile = wave % (QR_NT * QR_NT), part = wave / (QR_NT * QR_NT);
         const int ti = tile / QR_NT, tj = tile % QR_NT;
@@ -523,28 +522,53 @@ __device__ void qr_tfactor(const double *P, int ld, int nb, int m, double *Tl, d
         Y[idx] = sum;
     }
     __syncthreads();
-    if (wave == 0 && lane < QR_NB) {
-        // lane r builds row r of T column by column (dlarft, forward,
-        // columnwise): T(r, i) = -tau_i sum_{k<i} T(r, k) Y(k, i), T(i, i) =
-        // tau_i; column i of Y is one broadcast read per entry and the sum
-        // runs as four independent chains
-        const int r = lane;
-        double trow[QR_NB];
+    static_assert(QR_NB == 32, "T in two 16-column halves");
+    if (wave == 0) {
+        // T = [T11 T12; 0 T22] over the two 16-column halves of the panel.
+        // The diagonal blocks come from the column recurrence (dlarft,
+        // forward, columnwise) restricted to their half, both at once: lane
+        // 16 h + r (h < 2) builds row r of T_hh column by column, T(r, i) =
+        // -tau_i sum_{k<i} T(r, k) Y(k, i), T(i, i) = tau_i, the sum as four
+        // independent chains over k mod 4.  The terms with k before the half
+        // are products with the exact zeros of T left of its diagonal, so
+        // the blocks equal those of the full recurrence.  Then
+        // (I - V1 T11 V1^T)(I - V2 T22 V2^T) = I - V1 T11 V1^T - V2 T22 V2^T
+        // + V1 T11 (V1^T V2) T22 V2^T gives T12 = -T11 Y12 T22, two
+        // 16 x 16 x 16 products on FP64 MFMA.  Lanes 32 .. 63 repeat the
+        // recurrence of lanes 0 .. 31 and store nothing.
+        const int h = (lane >> 4) & 1, r = lane & 15, base = 16 * h;
+        double trow[16];
 #pragma unroll
-        for (int i = 0; i < QR_NB; i++) {
-            const double tau = Tl[i + i * QR_NB];
+        for (int i = 0; i < 16; i++) {
+            const double tau = Tl[(base + i) * (QR_NB + 1)];
             double s4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int k = 0; k < i; k++) s4[k & 3] += trow[k] * Y[k + i * QR_NB];
+            for (int k = 0; k < i; k++) s4[k & 3] += trow[k] * Y[(base + k) + (base + i) * QR_NB];
             const double sum = (s4[0] + s4[1]) + (s4[2] + s4[3]);
             trow[i] = (r < i) ? -tau * sum : (r == i ? tau : 0.0);
             __builtin_amdgcn_sched_barrier(0);
         }
+        if (lane < QR_NB) {
 #pragma unroll
-        for (int i = 0; i < QR_NB; i++) Ts[r + i * QR_NB] = trow[i];
+            for (int i = 0; i < 16; i++) Tl[(base + r) + (base + i) * QR_NB] = trow[i];
+        }
+        __builtin_amdgcn_s_waitcnt(0xc07f);
+        __builtin_amdgcn_wave_barrier();
+        // M = Y12 T22: A[m][k] = Y(m, 16 + k), B[k][n] = T(16 + k, 16 + n)
+        f64x4 mt = {0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const int k = 4 * s + lr;
+            mt = Mfma<double>::mma(Y[lc + (16 + k) * QR_NB], Tl[(16 + k) + (16 + lc) * QR_NB], mt);
+        }
+        // T12 = -T11 M: A[m][k] = T(m, k); the B operand of k-step s, M[4s +
+        // lr][lc], is accumulator entry s of the same lane
+        f64x4 o = {0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 4; s++) o = Mfma<double>::mma(Tl[lc + (4 * s + lr) * QR_NB], mt[s], o);
+#pragma unroll
+        for (int v = 0; v < 4; v++) Tl[Mfma<double>::row(lane, v) + (16 + lc) * QR_NB] = -o[v];
     }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < QR_NB * QR_NB; idx += blockDim.x) Tl[idx] = Ts[idx];
     __syncthreads();
 }
 
