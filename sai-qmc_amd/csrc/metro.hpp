@@ -413,10 +413,11 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
 // replica at n = 256), 16 columns j per wave, all four 16-row i tiles:
 // D^T[j][i] = sum_q V[q][j] C[q][i] on FP64 MFMA 16x16x4 (A = V^T, B = C,
 // both read as 16 consecutive doubles per lane group, q in increasing order
-// four at a time), then G = G_old - D.  The old G values of a lane are
-// loaded before the q loop, so they arrive while it runs; no LDS, no
-// barrier.  The q order and the final G_old - D match the rank-P GEMM this
-// replaces.
+// four at a time), then G = G_old - D.  The first sixteen q's are loaded
+// first and the old G values of a lane only after the q loop: the loop's
+// loads head the memory queue, and the wait for G_old meets the MFMAs of
+// the other workgroups on the CU; no LDS, no barrier.  The q order and the
+// final G_old - D match the rank-P GEMM this replaces.
 struct FlushArgs {
     int n, batch;
     double *G;
@@ -443,18 +444,9 @@ __global__ __launch_bounds__(256) void k_flush64(FlushArgs a)
     const double *C = a.C + (long long)b * a.sP;
     const double *V = a.V + (long long)b * a.sP;
     const int jw = j0 + 16 * wave; // this wave's 16 columns
-    // accumulator entry v of lane (lr, lc) in i tile ti: D^T[jw + lr + 4v][i0 + 16 ti + lc]
-    double old[4][4];
-#pragma unroll
-    for (int ti = 0; ti < 4; ti++)
-#pragma unroll
-        for (int v = 0; v < 4; v++) old[ti][v] = G[(i0 + 16 * ti + lc) + (long long)(jw + lr + 4 * v) * n];
-    f64x4 acc[4];
-#pragma unroll
-    for (int ti = 0; ti < 4; ti++) acc[ti] = f64x4{0, 0, 0, 0};
-    for (int q0 = 0; q0 < P; q0 += 16) {
-        double av[4], bv[4][4];
-        // clamped (always valid) rows, zeroed past P after the loads
+    double av[4], bv[4][4];
+    // clamped (always valid) rows, zeroed past P after the loads
+    auto load_q = [&](int q0) {
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const long long qr = (long long)min(q0 + 4 * u + lr, P - 1) * n;
@@ -462,6 +454,12 @@ __global__ __launch_bounds__(256) void k_flush64(FlushArgs a)
 #pragma unroll
             for (int ti = 0; ti < 4; ti++) bv[u][ti] = C[qr + i0 + 16 * ti + lc];
         }
+    };
+    f64x4 acc[4];
+#pragma unroll
+    for (int ti = 0; ti < 4; ti++) acc[ti] = f64x4{0, 0, 0, 0};
+    for (int q0 = 0; q0 < P; q0 += 16) {
+        load_q(q0);
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const double a_ = q0 + 4 * u + lr < P ? av[u] : 0.0;
@@ -469,6 +467,12 @@ __global__ __launch_bounds__(256) void k_flush64(FlushArgs a)
             for (int ti = 0; ti < 4; ti++) acc[ti] = Mfma<double>::mma(a_, bv[u][ti], acc[ti]);
         }
     }
+    // accumulator entry v of lane (lr, lc) in i tile ti: D^T[jw + lr + 4v][i0 + 16 ti + lc]
+    double old[4][4];
+#pragma unroll
+    for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) old[ti][v] = G[(i0 + 16 * ti + lc) + (long long)(jw + lr + 4 * v) * n];
 #pragma unroll
     for (int ti = 0; ti < 4; ti++)
 #pragma unroll

@@ -750,10 +750,14 @@ __global__ __launch_bounds__(FQ_WAVES * 64) void k_qr_formq_reg(QrArgs a)
     double *P = lds;
     double *Tl = P + (size_t)QR_NB * ld;
     double *Wl = Tl + QR_NB * QR_NB + wave * QR_NB * 16;
-    const int jb = g * FQ_WAVES + wave;
+    // column blocks interleaved over the matrix's workgroups (block jb in
+    // workgroup jb % wg_per): a block needs the panels at or left of its
+    // columns only, so blocks far right carry the most work, and each
+    // workgroup gets its share of them
+    const int jb = wave * wg_per + g;
     const int j0 = 16 * jb;
     const int j = j0 + lc;
-    const int jlast = min(n, 16 * (g + 1) * FQ_WAVES) - 1;
+    const int jlast = min(n, 16 * ((FQ_WAVES - 1) * wg_per + g + 1)) - 1; // last column of this workgroup
     double xb[NU];
 #pragma unroll
     for (int u = 0; u < NU; u++) xb[u] = (4 * u + lr == j) ? 1.0 : 0.0;
