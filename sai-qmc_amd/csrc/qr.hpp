@@ -431,11 +431,8 @@ __device__ void qr_panel(double *A, int n, int p0, int nb, int m, double *P, int
                 for (int s = 0; s < MR; s++) acc += x[s] * pc[k][s];
                 d[k] = acc;
             }
-#pragma unroll
-            for (int k = 0; k < NS; k++) {
-                const int c2 = QR_WAVES * k + wave;
-                if (c2 > c && c2 < nb && !(owner && k == k1)) d[k] = wave_sum(d[k]);
-            }
+            static_assert(NS == 4, "the four column slots of a wave reduced together");
+            wave_sum4(d);
 #pragma unroll
             for (int k = 0; k < NS; k++) {
                 const int c2 = QR_WAVES * k + wave;
