@@ -77,7 +77,7 @@ __device__ __forceinline__ double bcast_lane(double v, int lane)
 // layout of a 16-row tile, so the registers that feed W = V^T X then receive
 // X - V W' in place.  Blocks of up to 4*NU rows are loaded once (64 loads in
 // flight per lane); taller blocks take a second pass that reloads each
-// chunk.  V is the panel P (LDS), T is Tl (LDS, QR_NB x QR_NB,
+// chunk but the last, under the MFMAs of the chunk before.  V is the panel P (LDS), T is Tl (LDS, QR_NB x QR_NB,
 // Tl[i + c*QR_NB], upper), Wl a wave-private LDS buffer of QR_NB x 16.
 // Columns at or beyond jend are untouched.
 #define WY_NU 32
@@ -120,10 +120,17 @@ __device__ __forceinline__ void wy_accum_w(const double (&xb)[WY_NU], const doub
     }
 }
 
+// X - V W' for the rows r0 .. r0 + 4 WY_NU - 1 held in xb, two 16-row tiles at
+// a time (their V fragments read ahead of 16 MFMAs), each pair stored as
+// soon as it is done; with rnext >= 0 the pair's registers are then
+// refilled with the same rows of the chunk at rnext, so that chunk's loads
+// run under the MFMAs of the tiles still to come.  Refilled rows past m are
+// clamped copies: their stores are skipped, and W is not formed from them.
 __device__ __forceinline__ void wy_update_store(double (&xb)[WY_NU], const double (&wv)[QR_NB / 4], const double *P,
-                                                int ld, int nb, int r0, int m, double *col, bool jok, int lc, int lr)
+                                                int ld, int nb, int r0, int m, double *col, bool jok, int lc, int lr,
+                                                int rnext)
 {
-    // two 16-row tiles at a time: their V fragments read ahead of 16 MFMAs
+    gdouble *g = (gdouble *)col;
 #pragma unroll
     for (int t = 0; t < WY_NU / 4; t += 2) {
         if (r0 + 16 * t < m) {
@@ -143,12 +150,15 @@ __device__ __forceinline__ void wy_update_store(double (&xb)[WY_NU], const doubl
                 for (int v = 0; v < 4; v++) xb[4 * (t + h) + v] = c[v];
             }
         }
-    }
-    gdouble *g = (gdouble *)col;
 #pragma unroll
-    for (int u = 0; u < WY_NU; u++) {
-        const int k = r0 + 4 * u + lr;
-        if (jok && k < m) g[k] = xb[u];
+        for (int u = 4 * t; u < 4 * t + 8; u++) {
+            const int k = r0 + 4 * u + lr;
+            if (jok && k < m) g[k] = xb[u];
+        }
+        if (rnext >= 0) {
+#pragma unroll
+            for (int u = 4 * t; u < 4 * t + 8; u++) xb[u] = g[min(rnext + 4 * u + lr, m - 1)];
+        }
     }
 }
 
@@ -201,12 +211,15 @@ __device__ void wy_apply_block(const double *P, int ld, const double *Tl, int nb
 #pragma unroll
     for (int kk = 0; kk < QR_NB / 4; kk++) wv[kk] = Wl[(4 * kk + lr) * 16 + lc];
     if (m <= chunk) {
-        wy_update_store(xb, wv, P, ld, nb, 0, m, col, jok, lc, lr);
+        wy_update_store(xb, wv, P, ld, nb, 0, m, col, jok, lc, lr, -1);
     } else {
-        for (int r0 = 0; r0 < m; r0 += chunk) {
-            wy_load(xb, col, r0, m, jok, lr);
-            wy_update_store(xb, wv, P, ld, nb, r0, m, col, jok, lc, lr);
-        }
+        // the last chunk of the W pass is still in registers: it goes
+        // first, refilling its registers with the first chunk as it goes,
+        // and so on (the chunks are disjoint rows)
+        const int rlast = ((m - 1) / chunk) * chunk;
+        wy_update_store(xb, wv, P, ld, nb, rlast, m, col, jok, lc, lr, 0);
+        for (int r0 = 0; r0 < rlast; r0 += chunk)
+            wy_update_store(xb, wv, P, ld, nb, r0, m, col, jok, lc, lr, r0 + chunk < rlast ? r0 + chunk : -1);
     }
     __builtin_amdgcn_wave_barrier();
 }
