@@ -2215,7 +2215,8 @@ static int set_lds_limits()
                              (const void *)k_trsm_unit_upper, (const void *)k_measure, (const void *)k_qr_formq,
                              (const void *)k_qr_formq_reg<16>, (const void *)k_qr_formq_reg<32>,
                              (const void *)k_qr_formq_reg<64>,
-                             (const void *)k_gemm_dl<0>, (const void *)k_gemm_dl<1>,
+                             (const void *)k_gemm_dl<0, 0>, (const void *)k_gemm_dl<0, 1>,
+                             (const void *)k_gemm_dl<1, 0>, (const void *)k_gemm_dl<1, 1>,
                              (const void *)k_gemm_big<double, 0, 0>, (const void *)k_gemm_big<double, 1, 0>,
                              (const void *)k_gemm_big<double, 0, 1>, (const void *)k_gemm_big<double, 1, 1>,
                              (const void *)k_gemm_big<float, 0, 0>, (const void *)k_gemm_big<float, 1, 0>,

@@ -407,10 +407,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm_big(GemmArgs<T> g)
     }
 }
 
-// FP64 variant for op(A) = A, op(B) = B or B^T (TB) with M, N multiples of
-// 128 and 16-byte aligned columns; K a multiple of 16, or, with TB = 1, any
-// per-entry K (Kb: rows past it are read from row K-1 and zeroed by the
-// k-scale).  The k-tiles go from memory
+// FP64 variant for op(A) = A or A^T (TA), op(B) = B or B^T (TB) with M, N
+// multiples of 128 and 16-byte aligned columns; K a multiple of 16, or,
+// with TA = 0 and TB = 1, any per-entry K (Kb: rows past it are read from
+// row K-1 and zeroed by the k-scale).  The k-tiles go from memory
 // straight into LDS (gfx950 global_load_lds_dwordx4, 16 bytes per lane,
 // written at the wave's LDS base + 16 * lane), so no registers stage the
 // next tile and no VALU work stores it; the tile of k + 16 is in flight
@@ -419,7 +419,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_big(GemmArgs<T> g)
 // k as 8 pairs whose order is XOR-swizzled by (j & 7), so the fragment
 // reads of 16 columns spread over the banks.  Scalings: k-scales (ac, br)
 // multiply the op(A) fragments, row / column scales (ar, cr / bc, cc) are
-// applied in the epilogue.  With TB = 1 the B tile is k-major like A.
+// applied in the epilogue.  With TB = 1 the B tile is k-major like A; with
+// TA = 1 the A tile is i-major with the B tile's swizzled k pairs (a column
+// of A is a row of op(A), 16 consecutive k per i).
 #define GD_LD 144
 #define GD_ABUF (GB_K * GD_LD)          // doubles of one A tile (and of a k-major B tile)
 #define GD_BUF (2 * GD_ABUF)             // one buffer: A tile + B tile
@@ -437,7 +439,7 @@ static inline bool gemm_dl_enabled()
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-template <int TB>
+template <int TA, int TB>
 __global__ __launch_bounds__(256, 2) void k_gemm_dl(GemmArgs<double> g)
 {
     static_assert(GB_K == 16 && GB_M == 128, "16 x 128 tiles");
@@ -473,7 +475,13 @@ __global__ __launch_bounds__(256, 2) void k_gemm_dl(GemmArgs<double> g)
         for (int q = 0; q < 4; q++) {
             const int k = 4 * wave + q;
             const int kr = min(k0 + k, K - 1); // rows past K: a valid row, zeroed by the k-scale
-            __builtin_amdgcn_global_load_lds(a_src + (long long)kr * g.lda, (lds_void *)(As + k * GD_LD), 16, 0, 0);
+            if (TA) {
+                const int grp = 4 * wave + q;
+                const double *asrc = A + (k0 + 2 * bkp) + (long long)(m0 + 8 * grp + bj) * g.lda;
+                __builtin_amdgcn_global_load_lds(asrc, (lds_void *)(As + grp * 8 * GB_K), 16, 0, 0);
+            } else {
+                __builtin_amdgcn_global_load_lds(a_src + (long long)kr * g.lda, (lds_void *)(As + k * GD_LD), 16, 0, 0);
+            }
             if (TB) {
                 __builtin_amdgcn_global_load_lds(b_src + (long long)kr * g.ldb, (lds_void *)(Bs + k * GD_LD), 16, 0, 0);
             } else {
@@ -535,7 +543,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm_dl(GemmArgs<double> g)
                 fa[jt] = TB ? Bs[k * GD_LD + jl] : Bs[jl * GB_K + 2 * ((k >> 1) ^ (jl & 7)) + (k & 1)];
             }
 #pragma unroll
-            for (int it = 0; it < 4; it++) fb[it] = As[k * GD_LD + wr * 64 + it * 16 + lc] * sk[kk >> 2];
+            for (int it = 0; it < 4; it++) {
+                const int il = wr * 64 + it * 16 + lc;
+                fb[it] = (TA ? As[il * GB_K + 2 * ((k >> 1) ^ (il & 7)) + (k & 1)] : As[k * GD_LD + il]) * sk[kk >> 2];
+            }
 #pragma unroll
             for (int it = 0; it < 4; it++)
 #pragma unroll
@@ -627,12 +638,18 @@ static inline void gemm_launch(const GemmArgs<T> &g0, hipStream_t s)
     if constexpr (sizeof(T) == 8) {
         const bool aligned = (reinterpret_cast<uintptr_t>(g.A) & 15) == 0 && (reinterpret_cast<uintptr_t>(g.B) & 15) == 0 &&
                              g.lda % 2 == 0 && g.ldb % 2 == 0 && g.sA % 2 == 0 && g.sB % 2 == 0;
-        if (aligned && !g.transA && g.M % GB_M == 0 && g.N % GB_M == 0 && g.K > 0 && gemm_dl_enabled() &&
-            ((!g.transB && !g.Kb && g.K % GB_K == 0) || g.transB)) {
+        // a k-major tile (op(A) = A, op(B) = B^T) takes any per-entry K; an
+        // i- or j-major one needs K a multiple of 16
+        const bool kfull = !g.Kb && g.K % GB_K == 0;
+        if (aligned && g.M % GB_M == 0 && g.N % GB_M == 0 && g.K > 0 && gemm_dl_enabled() &&
+            (!g.transA || kfull) && (g.transB || kfull)) {
             const int tb = (g.M / GB_M) * (g.N / GB_M) * g.batch;
             if (tb <= 0) return;
-            if (g.transB) hipLaunchKernelGGL(k_gemm_dl<1>, dim3(tb), dim3(256), gemm_dl_lds_bytes(), s, g);
-            else hipLaunchKernelGGL(k_gemm_dl<0>, dim3(tb), dim3(256), gemm_dl_lds_bytes(), s, g);
+            const size_t lds = gemm_dl_lds_bytes();
+            if (g.transA && g.transB) hipLaunchKernelGGL((k_gemm_dl<1, 1>), dim3(tb), dim3(256), lds, s, g);
+            else if (g.transA) hipLaunchKernelGGL((k_gemm_dl<1, 0>), dim3(tb), dim3(256), lds, s, g);
+            else if (g.transB) hipLaunchKernelGGL((k_gemm_dl<0, 1>), dim3(tb), dim3(256), lds, s, g);
+            else hipLaunchKernelGGL((k_gemm_dl<0, 0>), dim3(tb), dim3(256), lds, s, g);
             return;
         }
     }
