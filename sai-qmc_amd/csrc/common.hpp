@@ -75,6 +75,21 @@ __device__ __forceinline__ double row_bcast(double v, int l)
     }
 }
 
+// the value of 16-lane row h0 (lane l + 16 h0) to lane l of every row: the
+// 32-lane swap brings the half holding h0 to both halves, the 16-lane swap
+// the row of h0 within it to both rows; h0 folds to a constant in unrolled
+// code
+__device__ __forceinline__ double row_to_all(double v, int h0)
+{
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const auto l32 = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto h32 = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    const int lo1 = (h0 & 2) ? l32[1] : l32[0], hi1 = (h0 & 2) ? h32[1] : h32[0];
+    const auto l16 = __builtin_amdgcn_permlane16_swap(lo1, lo1, false, false);
+    const auto h16 = __builtin_amdgcn_permlane16_swap(hi1, hi1, false, false);
+    return __hiloint2double((h0 & 1) ? h16[1] : h16[0], (h0 & 1) ? l16[1] : l16[0]);
+}
+
 __device__ __forceinline__ double lane63(double v)
 {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);

@@ -309,17 +309,46 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
                     const double Rd = 1.0 + gii * Nd;
                     const double R = Ru * Rd;
                     if (a.rlog && lane == 0) a.rlog[(long long)b * n + i] = R;
+                    // N / R_up of an accept, started before the decision
+                    const T fac = (T)(Nu / Ru);
                     if (bcast_any(dr_l, w) < fabs(R)) {
-                        const T fac = (T)(Nu / Ru);
-                        // column w of the block -> c, row w -> v (values before the update)
-                        if (c == w) {
+                        if constexpr (W == 16 && sizeof(T) == 8) {
+                            // column w of the block to every lane of each
+                            // 16-lane row (DPP row broadcast) gives c; row w,
+                            // held by 16-lane row w / RPL, to all four rows
+                            // (row swaps) gives v; G~ -= c v^T in registers.
+                            // The pairs also go to LDS for the pair formation
+                            // after the next barrier.
+                            T cw[RPL];
 #pragma unroll
-                            for (int k = 0; k < RPL; k++) {
-                                const int r = h * RPL + k;
-                                cwin[w * W + r] = fac * ((r == w ? T(1) : T(0)) - bw[k]);
+                            for (int k = 0; k < RPL; k++)
+                                cw[k] = fac * ((h * RPL + k == w ? T(1) : T(0)) - row_bcast(bw[k], w));
+                            const T vc = row_to_all(bw[w % RPL], w / RPL);
+                            if (c == w) {
+#pragma unroll
+                                for (int k = 0; k < RPL; k++) cwin[w * W + h * RPL + k] = cw[k];
                             }
+                            if (h == w / RPL) vwin[w * W + c] = vc;
+#pragma unroll
+                            for (int k = 0; k < RPL; k++) bw[k] -= cw[k] * vc;
+                        } else {
+                            // column w of the block -> c, row w -> v (values before the update)
+                            if (c == w) {
+#pragma unroll
+                                for (int k = 0; k < RPL; k++) {
+                                    const int r = h * RPL + k;
+                                    cwin[w * W + r] = fac * ((r == w ? T(1) : T(0)) - bw[k]);
+                                }
+                            }
+                            if (h == w / RPL) vwin[w * W + c] = bw[w % RPL];
+                            __builtin_amdgcn_s_waitcnt(0xc07f);
+                            __builtin_amdgcn_wave_barrier();
+                            const T vc = vwin[w * W + c];
+#pragma unroll
+                            for (int k = 0; k < RPL; k++) bw[k] -= cwin[w * W + h * RPL + k] * vc;
+                            __builtin_amdgcn_s_waitcnt(0xc07f);
+                            __builtin_amdgcn_wave_barrier();
                         }
-                        if (h == w / RPL) vwin[w * W + c] = bw[w % RPL];
                         if (lane == 0) {
                             facw[w] = fac;
                             flip[i] = 1;
@@ -327,13 +356,6 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
                         mask |= 1u << w;
                         naccept++;
                         if (R < 0.0) negflips ^= 1;
-                        __builtin_amdgcn_s_waitcnt(0xc07f);
-                        __builtin_amdgcn_wave_barrier();
-                        const T vc = vwin[w * W + c];
-#pragma unroll
-                        for (int k = 0; k < RPL; k++) bw[k] -= cwin[w * W + h * RPL + k] * vc;
-                        __builtin_amdgcn_s_waitcnt(0xc07f);
-                        __builtin_amdgcn_wave_barrier();
                     }
                 }
             }

@@ -193,20 +193,6 @@ __device__ __forceinline__ double ps_rlv(double v, int lane)
     asm volatile("" : "+v"(r));
     return r;
 }
-// The value of 16-lane row h0 (0..3) of the wave in every row, lane for
-// lane (the gfx950 row swaps: the 32-lane swap of x with itself gives, in
-// its two results, rows h & 1 and 2 + (h & 1); the 16-lane swap gives rows
-// h & ~1 and h | 1).  h0 folds to a constant in unrolled code.
-__device__ __forceinline__ double ps_xrow(double v, int h0)
-{
-    const int lo = __double2loint(v), hi = __double2hiint(v);
-    const auto l32 = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
-    const auto h32 = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
-    const int lo1 = (h0 & 2) ? l32[1] : l32[0], hi1 = (h0 & 2) ? h32[1] : h32[0];
-    const auto l16 = __builtin_amdgcn_permlane16_swap(lo1, lo1, false, false);
-    const auto h16 = __builtin_amdgcn_permlane16_swap(hi1, hi1, false, false);
-    return __hiloint2double((h0 & 1) ? h16[1] : h16[0], (h0 & 1) ? l16[1] : l16[0]);
-}
 
 // a 0/1 factor for a wave-uniform condition (a scalar value, so that a
 // row of a register array is chosen by arithmetic, not by an index the
@@ -833,7 +819,7 @@ __device__ __forceinline__ void ps_slice(const PersistModel *pm, int r, int l)
 #pragma unroll
                         for (int k = 0; k < RPL; k++)
                             cw[k] = fac * ((h * RPL + k == w ? 1.0 : 0.0) - row_bcast(bw[k], w));
-                        const double vc = ps_xrow(bw[w % RPL], w / RPL);
+                        const double vc = row_to_all(bw[w % RPL], w / RPL);
                         if (c == w) {
 #pragma unroll
                             for (int k = 0; k < RPL; k++) s.cwin[w * W + h * RPL + k] = cw[k];

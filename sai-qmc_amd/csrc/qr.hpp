@@ -909,7 +909,10 @@ __global__ __launch_bounds__(256) void k_trsm_unit_upper(TrsmArgs a)
 {
     const int n = a.n;
     const int nblk = (n + TS_NB - 1) / TS_NB;
-    const int b = blockIdx.x / nblk, cb = blockIdx.x % nblk;
+    // a matrix's column blocks on one XCD: they read the same U, row block
+    // by row block, from that XCD's L2
+    const int logical = xcd_remap(blockIdx.x, gridDim.x);
+    const int b = logical / nblk, cb = logical % nblk;
     if (b >= a.batch) return;
     if (a.status && a.status[b]) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
