@@ -162,8 +162,11 @@ __device__ void window_correct(const T *G, int n, int i0, int wn, const T *C, co
                     cv[t][u] = (qok && jok[t]) ? cv[t][u] : T(0);
                 }
             }
+            // k-steps wholly past P (the batch is UB steps whatever P is)
+            // would only add products of zeros: skipped, wave-uniformly
 #pragma unroll
             for (int u = 0; u < UB; u++) {
+                if (q0 + 4 * u >= P) break;
                 const int q = q0 + 4 * u + lr;
 #pragma unroll
                 for (int tw = 0; tw < NTW; tw++) {
