@@ -487,6 +487,7 @@ __global__ __launch_bounds__(256) void k_flush64(FlushArgs a)
         load_q(q0);
 #pragma unroll
         for (int u = 0; u < 4; u++) {
+            if (q0 + 4 * u >= P) break; // k-steps wholly past P: products of zeros
             const double a_ = q0 + 4 * u + lr < P ? av[u] : 0.0;
 #pragma unroll
             for (int ti = 0; ti < 4; ti++) acc[ti] = Mfma<double>::mma(a_, bv[u][ti], acc[ti]);
