@@ -86,7 +86,8 @@ __device__ __forceinline__ double bcast_any(double v, int lane)
 // C[q][i0+w] are the window slices staged in LDS.  Each wave takes JT
 // 16-row blocks of j for both outputs at once, streaming C and V from
 // global memory UB k-steps (4*JT*UB loads per lane) at a time; the kernel
-// runs JT = 2, UB = 8 on 8 waves.  Each accumulator row of a tile covers 16
+// runs JT = 2, UB = 4 on 8 waves (UB = 8 was 0.6 % slower per sweep once
+// the k-steps past P were skipped: the loads past P are still issued).  Each accumulator row of a tile covers 16
 // consecutive rows of one column of G, so the G loads are 128-byte segments.
 template <typename T, int W, int JT, int UB>
 __device__ void window_correct(const T *G, int n, int i0, int wn, const T *C, const T *V, int P, const T *Cw,
@@ -276,7 +277,7 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
             }
         }
         __syncthreads();
-        window_correct<T, W, 2, 8>(G, n, i0, wn, Cp, Vp, P, Cw, Vw, Sc, Sr);
+        window_correct<T, W, 2, 4>(G, n, i0, wn, Cp, Vp, P, Cw, Vw, Sc, Sr);
         __syncthreads();
         for (int idx = threadIdx.x; idx < W * W; idx += blockDim.x) {
             const int r = idx / W, c = idx % W;
