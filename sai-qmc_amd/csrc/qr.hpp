@@ -643,17 +643,18 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_udv(QrArgs a)
     }
     __syncthreads();
     if (a.Rn && tid < (int)(blockDim.x / n) * n) {
-        // thread (i, jc) scales row i of columns jc + cpb t; eight columns in
-        // flight, loads from clamped addresses (n <= blockDim.x)
+        // thread (i, jc) scales row i of columns jc + cpb t; 32 columns in
+        // flight (the registers are free here), loads from clamped addresses
+        // (n <= blockDim.x)
         double *Rn = a.Rn + (long long)b * a.sR;
         const int cpb = blockDim.x / n, i = tid % n, jc = tid / n;
         const double ri = rinv[i];
-        for (int j0 = jc; j0 < n; j0 += 8 * cpb) {
-            double v[8];
+        for (int j0 = jc; j0 < n; j0 += 32 * cpb) {
+            double v[32];
 #pragma unroll
-            for (int u = 0; u < 8; u++) v[u] = A[i + (long long)min(j0 + u * cpb, n - 1) * n];
+            for (int u = 0; u < 32; u++) v[u] = A[i + (long long)min(j0 + u * cpb, n - 1) * n];
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
+            for (int u = 0; u < 32; u++) {
                 const int j = j0 + u * cpb;
                 if (j < n) Rn[i + (long long)j * n] = i <= j ? v[u] * ri : 0.0;
             }
