@@ -373,8 +373,7 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
         // and v halves are independent: with 2n threads or more, thread j <
         // n forms c_q(j) and thread n + j forms v_q(j); otherwise a thread
         // forms both, one after the other.  The mask is wave-uniform (scalar
-        // branches) and the window-restricted pair entries of one accept
-        // are read from LDS as one batch.
+        // branches).
         const unsigned mask = __builtin_amdgcn_readfirstlane(s_mask);
         const bool split = (int)blockDim.x >= 2 * n;
         const int jj = (split && j >= n) ? j - n : j;
@@ -390,19 +389,18 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
                     const T x = Sx[min(w, wn - 1) * (n + 1) + jj];
                     gc[w] = (w < wn) ? x : T(0);
                 }
+                // right-looking: once an accept's entry is final it is
+                // removed from every later site of the window at once (the
+                // subtractions of each entry keep their order, and the
+                // FMAs of different sites are independent)
 #pragma unroll
-                for (int w = 0; w < W; w++) {
-                    if ((mask >> w) & 1u) {
-                        T cf[W];
+                for (int u = 0; u < W; u++) {
+                    if ((mask >> u) & 1u) {
+                        gc[u] = vside ? gc[u] : facw[u] * ((jj == i0 + u ? T(1) : T(0)) - gc[u]);
+                        const int q = P + __builtin_popcount(mask & ((1u << u) - 1u));
+                        dst[(long long)q * n + jj] = gc[u];
 #pragma unroll
-                        for (int u = 0; u < w; u++) cf[u] = coef[u * W + w];
-                        T x = gc[w];
-#pragma unroll
-                        for (int u = 0; u < w; u++)
-                            if ((mask >> u) & 1u) x -= gc[u] * cf[u];
-                        gc[w] = vside ? x : facw[w] * ((jj == i0 + w ? T(1) : T(0)) - x);
-                        const int q = P + __builtin_popcount(mask & ((1u << w) - 1u));
-                        dst[(long long)q * n + jj] = gc[w];
+                        for (int w = u + 1; w < W; w++) gc[w] -= gc[u] * coef[u * W + w];
                     }
                 }
             }
