@@ -57,12 +57,14 @@ __device__ __forceinline__ int kron_lds(int e) { return (e & 15) + 17 * (e >> 4)
 
 // MODE: KRON_COLS, KRON_ROWS, or KRON_COLS_PENDING (a column pass that
 // applies the pending update first); one instantiation per mode, so the
-// plain passes carry none of the pending update's registers
+// plain passes carry none of the pending update's registers.  At least four
+// waves per SIMD (at most 128 VGPRs): with the LDS of four workgroups per
+// CU, the pending pass then holds four workgroups instead of three.
 #define KRON_COLS 0
 #define KRON_ROWS 1
 #define KRON_COLS_PENDING 2
 template <int MODE>
-__global__ __launch_bounds__(256) void k_kron16(KronArgs a)
+__global__ __launch_bounds__(256, 4) void k_kron16(KronArgs a)
 {
     const int logical = xcd_remap(blockIdx.x, gridDim.x);
     const int b = logical / KRON_L, v0 = (logical % KRON_L) * KRON_L;
