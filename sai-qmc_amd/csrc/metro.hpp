@@ -441,7 +441,10 @@ __global__ __launch_bounds__(METRO_THREADS) void k_metro_slice(MetroArgs<T> a)
 // first and the old G values of a lane only after the q loop: the loop's
 // loads head the memory queue, and the wait for G_old meets the MFMAs of
 // the other workgroups on the CU; no LDS, no barrier.  The q order and the
-// final G_old - D match the rank-P GEMM this replaces.
+// final G_old - D match the rank-P GEMM this replaces.  Bounded to at
+// least five waves per SIMD: the compiler then keeps it to 52 VGPRs, and
+// more workgroups per CU hide the loads better than more loads in flight
+// per workgroup (0.5 % per sweep at 16x16).
 struct FlushArgs {
     int n, batch;
     double *G;
@@ -452,7 +455,7 @@ struct FlushArgs {
     const int *status;
 };
 
-__global__ __launch_bounds__(256) void k_flush64(FlushArgs a)
+__global__ __launch_bounds__(256, 5) void k_flush64(FlushArgs a)
 {
     const int n = a.n, nb = n >> 6, per = nb * nb;
     const int logical = xcd_remap(blockIdx.x, gridDim.x);
