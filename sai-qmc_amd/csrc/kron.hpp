@@ -33,7 +33,8 @@
 
 #define KRON_L 16
 #define KRON_N (KRON_L * KRON_L)
-#define KRON_LD 272 // LDS doubles per vector: 16 grid rows of 17 (one pad)
+#define KRON_LD 273 // LDS doubles per vector: 16 grid rows of 17 (one pad), and one more so
+                    // that the 16 vectors of a row-pass lane group fall on 16 banks
 
 struct KronArgs {
     int batch, rows;     // rows = 0: transform the columns of X; 1: its rows (the launcher
